@@ -1,0 +1,315 @@
+"""apxchol_amd -- the MI355X (gfx950) solve path of apxchol, from Python.
+
+A thin ctypes layer over the C ABI in include/apxchol_amd.h (the library is
+apxchol_amd/libapxchol_amd.so, built in-tree by apxchol_amd.build). It mirrors
+the reference's Python surface (python/apxchol/__init__.py:111-338: ``Solver``
+with ``solve``/``apply``, ``factorize``, ``solve``) on top of the GPU solver.
+
+Vectors may be NumPy arrays (host; copied through the library's host entry
+points) or torch CUDA tensors of float64 (device; passed by pointer, no copy).
+There is no CPU fallback: without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["Solver", "Factor", "factorize", "solve", "SolveResult", "load_library", "ApxError"]
+
+_LIB_PATH = Path(__file__).resolve().parent / "libapxchol_amd.so"
+_lib = None
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int32
+U32 = ctypes.c_uint32
+F64 = ctypes.c_double
+
+
+class ApxError(RuntimeError):
+    """A failure reported through the C ABI (status + apx_last_error())."""
+
+
+class ApxInvalid(ApxError, ValueError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("tol", F64), ("max_iter", I32), ("stagnation_window", I32), ("seed", U32),
+                ("precision", I32), ("fp16_storage", I32), ("schedule", I32), ("drop_rel", F64),
+                ("keep_factor", I32), ("iters_per_check", I32)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", I64), ("residual", F64), ("lumped_offdiag", I64), ("solve_vram_mb", F64),
+                ("status", I32)]
+
+
+class SolverInfo(ctypes.Structure):
+    _fields_ = [("n", I64), ("m", I64), ("sddm", I32), ("factor_nnz", ctypes.c_uint64),
+                ("stored_nnz", ctypes.c_uint64), ("offdiag", ctypes.c_uint64), ("dropped", ctypes.c_uint64),
+                ("rounds", I64), ("levels", I64), ("fwd_batches", I64), ("bck_batches", I64), ("op_nnz", I64),
+                ("op_fp32", I32), ("spmv_lanes", I32), ("dataflow_grid", I32), ("fp16_storage", I32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+_SIGS = {
+    "apx_last_error": ([], ctypes.c_char_p),
+    "apx_options_default": ([P], None),
+    "apx_factorize": ([I64, P, P, P, U32, P], ctypes.c_int),
+    "apx_factor_info": ([P, P], ctypes.c_int),
+    "apx_factor_export": ([P, P, P, P, P, P], ctypes.c_int),
+    "apx_factor_destroy": ([P], None),
+    "apx_trsv_sizes": ([P, I32, F64, P], ctypes.c_int),
+    "apx_trsv_export": ([P, I32, F64] + [P] * 13, ctypes.c_int),
+    "apx_solver_create": ([I64, P, P, P, P, P], ctypes.c_int),
+    "apx_solver_create_with_factor": ([I64, P, P, P, P, P, P], ctypes.c_int),
+    "apx_solver_destroy": ([P], None),
+    "apx_solver_info_get": ([P, P], ctypes.c_int),
+    "apx_solver_solve": ([P, P, P, P, F64, I32, P, P], ctypes.c_int),
+    "apx_solver_solve_host": ([P, P, P, P, F64, I32, P], ctypes.c_int),
+    "apx_solver_apply": ([P, P, P, P], ctypes.c_int),
+    "apx_solver_apply_host": ([P, P, P], ctypes.c_int),
+    "apx_solver_reset_apply_count": ([P], ctypes.c_int),
+    "apx_solver_sweep": ([P, I32, P, P, P], ctypes.c_int),
+    "apx_solver_export_factor": ([P, P, P, P, P], ctypes.c_int),
+    "apx_solve": ([I64, P, P, P, P, P, P, P], ctypes.c_int),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load (once) the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else _LIB_PATH
+    # torch bundles its own HIP runtime under the same SONAME
+    # (libamdhip64.so.7); loading torch first makes this library bind to that
+    # copy, so a process never holds two HIP runtimes.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not p.exists():
+        raise ApxError(f"{p} is missing: build it with `python -m apxchol_amd.build`")
+    lib = ctypes.CDLL(str(p))
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = load_library().apx_last_error().decode(errors="replace")
+        raise (ApxInvalid if status == 1 else ApxError)(msg)
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+def _csc(A):
+    """Full-symmetric CSC arrays (int64 colptr, int32 rowidx, float64 vals)."""
+    import scipy.sparse as sp
+    if not sp.issparse(A):
+        raise TypeError("A must be a scipy.sparse matrix")
+    C = sp.csc_matrix(A)
+    C.sum_duplicates()
+    C.sort_indices()
+    n = C.shape[0]
+    if C.shape[0] != C.shape[1]:
+        raise ValueError("A must be square")
+    return (n, np.ascontiguousarray(C.indptr, dtype=np.int64), np.ascontiguousarray(C.indices, dtype=np.int32),
+            np.ascontiguousarray(C.data, dtype=np.float64))
+
+
+def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
+             schedule="dataflow", drop_rel=-1.0, keep_factor=False, iters_per_check=4) -> Options:
+    o = Options()
+    load_library().apx_options_default(ctypes.byref(o))
+    o.tol = float(tol)
+    o.max_iter = int(maxiter)
+    o.stagnation_window = int(stagnation_window)
+    o.seed = int(seed)
+    o.precision = {"fp64": 0, "fp32": 1}[precision]
+    o.fp16_storage = 1 if fp16_storage else 0
+    o.schedule = {"dataflow": 0, "levels": 1}[schedule]
+    o.drop_rel = float(drop_rel)
+    o.keep_factor = 1 if keep_factor else 0
+    o.iters_per_check = int(iters_per_check)
+    return o
+
+
+@dataclass
+class SolveResult:
+    x: object
+    iterations: int
+    residual: float
+    status: int
+    lumped_offdiag: int = 0
+    solve_vram_mb: float = -1.0
+
+
+class Factor:
+    """Host approximate-Cholesky factorization (apxchol::factorize at one thread)."""
+
+    def __init__(self, A, *, seed: int = 42):
+        lib = load_library()
+        self._arrays = _csc(A)
+        n, cp, ri, v = self._arrays
+        h = P()
+        _check(lib.apx_factorize(n, cp.ctypes.data, ri.ctypes.data, v.ctypes.data, seed, ctypes.byref(h)))
+        self._h = h
+        info = np.zeros(7, np.int64)
+        _check(lib.apx_factor_info(h, info.ctypes.data))
+        self.n, self.nnz, self.nrounds, self.sddm, self.lumped = (int(info[0]), int(info[1]), int(info[2]),
+                                                                   bool(info[3]), int(info[4]))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.apx_factor_destroy(self._h)
+            self._h = None
+
+    def export(self):
+        """(outer, inner, vals, perm, round_is) of the factor L (CSC, diagonal first)."""
+        outer = np.zeros(self.n + 1, np.uint64)
+        inner = np.zeros(self.nnz, np.uint32)
+        vals = np.zeros(self.nnz, np.float32)
+        perm = np.zeros(self.n, np.uint32)
+        rounds = np.zeros(max(self.nrounds, 1), np.uint64)
+        _check(load_library().apx_factor_export(self._h, outer.ctypes.data, inner.ctypes.data, vals.ctypes.data,
+                                                perm.ctypes.data, rounds.ctypes.data))
+        return outer, inner, vals, perm, rounds[:self.nrounds]
+
+    def trsv(self, fp16: bool = False, drop_rel: float = -1.0) -> dict:
+        """Host SpTRSV preparation: forward/back off-diagonal lists, diag, levels."""
+        lib = load_library()
+        sz = np.zeros(6, np.int64)
+        _check(lib.apx_trsv_sizes(self._h, int(fp16), drop_rel, sz.ctypes.data))
+        m, off, nl = int(sz[0]), int(sz[1]), int(sz[2])
+        out = dict(m=m, offdiag=off, levels=nl, nnz_factor=int(sz[3]), nnz_stored=int(sz[4]), fp16=bool(sz[5]))
+        arr = dict(f_ptr=np.zeros(m + 1, np.uint32), f_idx=np.zeros(off, np.int32),
+                   f_val=np.zeros(0 if fp16 else off, np.float32), f_h=np.zeros(off if fp16 else 0, np.uint16),
+                   b_ptr=np.zeros(m + 1, np.uint32), b_idx=np.zeros(off, np.int32),
+                   b_val=np.zeros(0 if fp16 else off, np.float32), b_h=np.zeros(off if fp16 else 0, np.uint16),
+                   diag=np.zeros(m, np.float32), inv_scale=np.zeros(m if fp16 else 0, np.float32),
+                   level_ptr=np.zeros(nl + 1, np.uint32))
+        order = ["f_ptr", "f_idx", "f_val", "f_h", "b_ptr", "b_idx", "b_val", "b_h", "diag", "inv_scale",
+                 "level_ptr"]
+        _check(lib.apx_trsv_export(self._h, int(fp16), drop_rel,
+                                   *[arr[k].ctypes.data if arr[k].size else None for k in order]))
+        out.update(arr)
+        return out
+
+
+def factorize(A, *, seed: int = 42) -> Factor:
+    return Factor(A, seed=seed)
+
+
+class Solver:
+    """GPU approximate-Cholesky PCG solver: factor built once, solve many b.
+
+    Mirrors the reference's ``apxchol.Solver`` (python/apxchol/__init__.py:111-270);
+    the solve runs on the current HIP device.
+    """
+
+    def __init__(self, A, *, seed: int = 42, factor: Factor | None = None, **opts):
+        lib = load_library()
+        self._arrays = _csc(A)
+        n, cp, ri, v = self._arrays
+        self._n = n
+        self._opts = _options(seed=seed, **opts)
+        h = P()
+        if factor is None:
+            _check(lib.apx_solver_create(n, cp.ctypes.data, ri.ctypes.data, v.ctypes.data,
+                                         ctypes.byref(self._opts), ctypes.byref(h)))
+        else:
+            _check(lib.apx_solver_create_with_factor(n, cp.ctypes.data, ri.ctypes.data, v.ctypes.data, factor._h,
+                                                     ctypes.byref(self._opts), ctypes.byref(h)))
+        self._h = h
+        inf = SolverInfo()
+        _check(lib.apx_solver_info_get(h, ctypes.byref(inf)))
+        self.info = {k: getattr(inf, k) for k, _ in SolverInfo._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.apx_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self._n, self._n)
+
+    @property
+    def sddm(self) -> bool:
+        return bool(self.info["sddm"])
+
+    def solve(self, b, tol: float | None = None, maxiter: int | None = None, x0=None, out=None,
+              stream=None) -> SolveResult:
+        lib = load_library()
+        tol = -1.0 if tol is None else float(tol)
+        maxiter = -1 if maxiter is None else int(maxiter)
+        info = SolveInfo()
+        if isinstance(b, np.ndarray):
+            b = np.ascontiguousarray(b, dtype=np.float64)
+            x = np.zeros(self._n) if out is None else out
+            x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+            _check(lib.apx_solver_solve_host(self._h, b.ctypes.data, x.ctypes.data, _ptr(x0a), tol, maxiter,
+                                             ctypes.byref(info)))
+        else:
+            import torch
+            x = torch.empty_like(b) if out is None else out
+            s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+            _check(lib.apx_solver_solve(self._h, _ptr(b), _ptr(x), _ptr(x0), tol, maxiter, ctypes.byref(info), s))
+        return SolveResult(x, int(info.iterations), float(info.residual), int(info.status),
+                           int(info.lumped_offdiag), float(info.solve_vram_mb))
+
+    def apply(self, r, out=None, stream=None):
+        lib = load_library()
+        if isinstance(r, np.ndarray):
+            r = np.ascontiguousarray(r, dtype=np.float64)
+            z = np.zeros(self._n) if out is None else out
+            _check(lib.apx_solver_apply_host(self._h, r.ctypes.data, z.ctypes.data))
+            return z
+        import torch
+        z = torch.empty_like(r) if out is None else out
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(lib.apx_solver_apply(self._h, _ptr(r), _ptr(z), s))
+        return z
+
+    def reset_apply_count(self):
+        _check(load_library().apx_solver_reset_apply_count(self._h))
+
+    def sweep(self, rhs, out, backward: bool, stream=None):
+        """One sweep in factor order (torch float64 CUDA tensors of length m)."""
+        import torch
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().apx_solver_sweep(self._h, int(backward), _ptr(rhs), _ptr(out), s))
+        return out
+
+
+def solve(A, b, tol: float = 1e-8, maxiter: int = 200, **opts) -> SolveResult:
+    """One-shot apxchol::solve on the GPU (factorize, upload, PCG)."""
+    with_opts = dict(opts)
+    s = Solver(A, **with_opts)
+    try:
+        return s.solve(np.asarray(b, dtype=np.float64), tol=tol, maxiter=maxiter)
+    finally:
+        s.close()

@@ -1,0 +1,312 @@
+// extern "C" boundary of the MI355X solve path (include/apxchol_amd.h). No
+// exception crosses it: every failure becomes an apx_status plus a
+// thread-local message.
+#include "../../include/apxchol_amd.h"
+
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "host/factor.hpp"
+#include "host/gpu_solver.hpp"
+#include "host/trsv_prep.hpp"
+
+using namespace apxchol_amd;
+
+struct apx_factor_s {
+    factorization F;
+};
+struct apx_solver_s {
+    std::unique_ptr<gpu_solver> s;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class Fn>
+apx_status guarded(Fn&& fn) {
+    try {
+        fn();
+        return APX_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return APX_ERR_INVALID;
+    } catch (const std::bad_alloc& e) {
+        g_err = std::string("out of memory: ") + e.what();
+        return APX_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return APX_ERR_RUNTIME;
+    } catch (...) {
+        g_err = "unknown error";
+        return APX_ERR_RUNTIME;
+    }
+}
+
+csc_view make_view(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals) {
+    if (n < 0) throw std::invalid_argument("n must be >= 0");
+    if (n > 0 && (!colptr || !rowidx || !vals)) throw std::invalid_argument("null matrix array");
+    csc_view v;
+    v.n = n;
+    v.colptr = colptr;
+    v.rowidx = rowidx;
+    v.vals = vals;
+    return v;
+}
+
+solve_options to_opts(const apx_options* o) {
+    solve_options so;
+    apx_options d;
+    apx_options_default(&d);
+    if (!o) o = &d;
+    so.tol = o->tol;
+    so.max_iter = o->max_iter;
+    so.stagnation_window = o->stagnation_window;
+    so.factor_opts.seed = o->seed;
+    if (o->precision != 0 && o->precision != 1) throw std::invalid_argument("precision must be 0 (fp64) or 1 (fp32)");
+    if (o->schedule != 0 && o->schedule != 1) throw std::invalid_argument("schedule must be 0 (dataflow) or 1 (levels)");
+    so.precision = o->precision ? sweep_precision::fp32 : sweep_precision::fp64;
+    so.fp16_storage = o->fp16_storage != 0;
+    so.schedule = o->schedule ? sweep_schedule::levels : sweep_schedule::dataflow;
+    so.drop_rel = o->drop_rel;
+    so.keep_factor_values = o->keep_factor != 0;
+    so.iters_per_check = o->iters_per_check > 0 ? o->iters_per_check : 4;
+    return so;
+}
+
+void fill_info(const solve_result& r, apx_solve_info* info) {
+    if (!info) return;
+    info->iterations = r.iterations;
+    info->residual = r.residual;
+    info->lumped_offdiag = r.lumped_offdiag;
+    info->solve_vram_mb = r.solve_vram_mb;
+    info->status = r.status;
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw std::invalid_argument(std::string("null ") + what);
+}
+
+}  // namespace
+
+extern "C" {
+
+void apx_options_default(apx_options* o) {
+    if (!o) return;
+    o->tol = 1e-8;
+    o->max_iter = 200;
+    o->stagnation_window = 50;
+    o->seed = 42;
+    o->precision = 0;
+    o->fp16_storage = 0;
+    o->schedule = 0;
+    o->drop_rel = -1.0;
+    o->keep_factor = 0;
+    o->iters_per_check = 4;
+}
+
+const char* apx_last_error(void) { return g_err.c_str(); }
+
+apx_status apx_factorize(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, uint32_t seed,
+                         apx_factor_t* out) {
+    return guarded([&] {
+        need(out, "out");
+        factor_options fo;
+        fo.seed = seed;
+        auto* f = new apx_factor_s{factorize(make_view(n, colptr, rowidx, vals), fo)};
+        *out = f;
+    });
+}
+
+apx_status apx_factor_info(apx_factor_t f, int64_t* out) {
+    return guarded([&] {
+        need(f, "factor");
+        need(out, "out");
+        out[0] = f->F.n;
+        out[1] = static_cast<int64_t>(f->F.nnz());
+        out[2] = static_cast<int64_t>(f->F.rounds.size());
+        out[3] = f->F.sddm ? 1 : 0;
+        out[4] = f->F.lumped_offdiag;
+        out[5] = 0;
+        out[6] = 0;
+    });
+}
+
+apx_status apx_factor_export(apx_factor_t f, uint64_t* outer, uint32_t* inner, float* vals, uint32_t* perm,
+                             uint64_t* round_is) {
+    return guarded([&] {
+        need(f, "factor");
+        const auto& F = f->F;
+        if (outer) std::memcpy(outer, F.outer.data(), F.outer.size() * sizeof(uint64_t));
+        if (inner) std::memcpy(inner, F.inner.data(), F.inner.size() * sizeof(uint32_t));
+        if (vals) std::memcpy(vals, F.vals.data(), F.vals.size() * sizeof(float));
+        if (perm) std::memcpy(perm, F.perm.data(), F.perm.size() * sizeof(uint32_t));
+        if (round_is)
+            for (std::size_t r = 0; r < F.rounds.size(); ++r) round_is[r] = F.rounds[r].is_size;
+    });
+}
+
+void apx_factor_destroy(apx_factor_t f) { delete f; }
+
+apx_status apx_trsv_sizes(apx_factor_t f, int32_t fp16, double drop_rel, int64_t* sizes) {
+    return guarded([&] {
+        need(f, "factor");
+        need(sizes, "sizes");
+        const trsv_factor T = prepare_trsv(f->F, fp16 != 0, drop_rel);
+        sizes[0] = T.m;
+        sizes[1] = static_cast<int64_t>(T.offdiag());
+        sizes[2] = static_cast<int64_t>(T.level_ptr.size()) - 1;
+        sizes[3] = static_cast<int64_t>(T.drop.nnz_factor);
+        sizes[4] = static_cast<int64_t>(T.drop.nnz_stored);
+        sizes[5] = T.fp16 ? 1 : 0;
+    });
+}
+
+apx_status apx_trsv_export(apx_factor_t f, int32_t fp16, double drop_rel, uint32_t* f_ptr, int32_t* f_idx,
+                           float* f_val, uint16_t* f_h, uint32_t* b_ptr, int32_t* b_idx, float* b_val, uint16_t* b_h,
+                           float* diag, float* inv_scale, uint32_t* level_ptr) {
+    return guarded([&] {
+        need(f, "factor");
+        const trsv_factor T = prepare_trsv(f->F, fp16 != 0, drop_rel);
+        auto cp = [](auto* dst, const auto& v) {
+            if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+        };
+        cp(f_ptr, T.f_ptr); cp(f_idx, T.f_idx); cp(f_val, T.f_val); cp(f_h, T.f_h);
+        cp(b_ptr, T.b_ptr); cp(b_idx, T.b_idx); cp(b_val, T.b_val); cp(b_h, T.b_h);
+        cp(diag, T.diag); cp(inv_scale, T.inv_scale); cp(level_ptr, T.level_ptr);
+    });
+}
+
+apx_status apx_solver_create(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
+                             const apx_options* opts, apx_solver_t* out) {
+    return guarded([&] {
+        need(out, "out");
+        auto* s = new apx_solver_s;
+        try {
+            s->s = std::make_unique<gpu_solver>(make_view(n, colptr, rowidx, vals), to_opts(opts));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+apx_status apx_solver_create_with_factor(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
+                                         apx_factor_t f, const apx_options* opts, apx_solver_t* out) {
+    return guarded([&] {
+        need(out, "out");
+        need(f, "factor");
+        auto* s = new apx_solver_s;
+        try {
+            s->s = std::make_unique<gpu_solver>(make_view(n, colptr, rowidx, vals), f->F, to_opts(opts));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+void apx_solver_destroy(apx_solver_t s) { delete s; }
+
+apx_status apx_solver_info_get(apx_solver_t s, apx_solver_info* info) {
+    return guarded([&] {
+        need(s, "solver");
+        need(info, "info");
+        const solver_stats& st = s->s->stats();
+        info->n = st.n; info->m = st.m; info->sddm = st.sddm ? 1 : 0;
+        info->factor_nnz = st.factor_nnz; info->stored_nnz = st.stored_nnz; info->offdiag = st.offdiag;
+        info->dropped = st.dropped; info->rounds = st.rounds; info->levels = st.levels;
+        info->fwd_batches = st.fwd_batches; info->bck_batches = st.bck_batches; info->op_nnz = st.op_nnz;
+        info->op_fp32 = st.op_fp32 ? 1 : 0; info->spmv_lanes = st.spmv_lanes; info->dataflow_grid = st.dataflow_grid;
+        info->fp16_storage = s->s->trsv().fp16 ? 1 : 0;
+        info->device_bytes = st.device_bytes;
+    });
+}
+
+apx_status apx_solver_solve(apx_solver_t s, const double* d_b, double* d_x, const double* d_x0, double tol,
+                            int32_t max_iter, apx_solve_info* info, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        if (s->s->rows() > 0) { need(d_b, "b"); need(d_x, "x"); }
+        fill_info(s->s->solve_device(d_b, d_x, tol, max_iter, d_x0, static_cast<hipStream_t>(stream)), info);
+    });
+}
+
+apx_status apx_solver_solve_host(apx_solver_t s, const double* b, double* x, const double* x0, double tol,
+                                 int32_t max_iter, apx_solve_info* info) {
+    return guarded([&] {
+        need(s, "solver");
+        const std::size_t n = static_cast<std::size_t>(s->s->rows());
+        if (n > 0) { need(b, "b"); need(x, "x"); }
+        std::vector<double> bv(b, b + n), x0v;
+        if (x0) x0v.assign(x0, x0 + n);
+        solve_result r = s->s->solve(bv, tol, max_iter, x0 ? &x0v : nullptr);
+        if (n > 0) std::memcpy(x, r.x.data(), n * sizeof(double));
+        fill_info(r, info);
+    });
+}
+
+apx_status apx_solver_apply(apx_solver_t s, const double* d_r, double* d_z, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        if (s->s->rows() > 0) { need(d_r, "r"); need(d_z, "z"); }
+        s->s->apply_device(d_r, d_z, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_solver_apply_host(apx_solver_t s, const double* r, double* z) {
+    return guarded([&] {
+        need(s, "solver");
+        const std::size_t n = static_cast<std::size_t>(s->s->rows());
+        if (n > 0) { need(r, "r"); need(z, "z"); }
+        const std::vector<double> zv = s->s->apply(std::vector<double>(r, r + n));
+        if (n > 0) std::memcpy(z, zv.data(), n * sizeof(double));
+    });
+}
+
+apx_status apx_solver_reset_apply_count(apx_solver_t s) {
+    return guarded([&] {
+        need(s, "solver");
+        s->s->reset_apply_count();
+    });
+}
+
+apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rhs, double* d_out, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        need(d_rhs, "rhs");
+        need(d_out, "out");
+        s->s->sweep_device(backward != 0, d_rhs, d_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* inner, float* vals, uint32_t* perm) {
+    return guarded([&] {
+        need(s, "solver");
+        const auto& F = s->s->factor();
+        if ((inner || vals) && F.nnz() > 0 && F.vals.empty())
+            throw std::invalid_argument("factor values were released (create the solver with keep_factor = 1)");
+        if (outer) std::memcpy(outer, F.outer.data(), F.outer.size() * sizeof(uint64_t));
+        if (inner) std::memcpy(inner, F.inner.data(), F.inner.size() * sizeof(uint32_t));
+        if (vals) std::memcpy(vals, F.vals.data(), F.vals.size() * sizeof(float));
+        if (perm) std::memcpy(perm, F.perm.data(), F.perm.size() * sizeof(uint32_t));
+    });
+}
+
+apx_status apx_solve(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, const double* b,
+                     double* x, const apx_options* opts, apx_solve_info* info) {
+    return guarded([&] {
+        const csc_view A = make_view(n, colptr, rowidx, vals);
+        if (n > 0) { need(b, "b"); need(x, "x"); }
+        const gpu_solver slv(A, to_opts(opts));
+        solve_result r = slv.solve(std::vector<double>(b, b + n));
+        if (n > 0) std::memcpy(x, r.x.data(), static_cast<std::size_t>(n) * sizeof(double));
+        fill_info(r, info);
+    });
+}
+
+}  // extern "C"

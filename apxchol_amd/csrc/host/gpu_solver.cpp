@@ -1,0 +1,502 @@
+// gpu_solver: host orchestration of the GPU solve path.
+//
+// Setup: host factorization (factorize.cpp) -> L11 prep (trsv_prep.cpp) ->
+// upload of the two sweep layouts, their batch tables, the permuted operator
+// and the PCG workspace. Solve: the PCG recurrence runs in the factor's
+// permuted space on the device (pcg.hip), with the two sweeps (sweep.hip) as
+// the preconditioner; the host only enqueues work and reads the completion
+// flag every iters_per_check iterations.
+#include "gpu_solver.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../device.hpp"
+#include "../kernels/launch.hpp"
+
+namespace apxchol_amd {
+
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("apxchol_amd: ") + what + ": " + hipGetErrorString(e));
+}
+
+class dbuf {
+public:
+    dbuf() = default;
+    explicit dbuf(std::size_t bytes) { alloc(bytes); }
+    ~dbuf() { if (p_) (void)hipFree(p_); }
+    dbuf(const dbuf&) = delete;
+    dbuf& operator=(const dbuf&) = delete;
+    void alloc(std::size_t bytes) {
+        if (p_) { (void)hipFree(p_); p_ = nullptr; }
+        bytes_ = bytes;
+        if (bytes == 0) return;
+        hip_check(hipMalloc(&p_, bytes), "hipMalloc");
+    }
+    template <class T>
+    void upload(const std::vector<T>& v) {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) hip_check(hipMemcpy(p_, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+    void zero() { if (p_) hip_check(hipMemset(p_, 0, bytes_), "hipMemset"); }
+    template <class T> T* as() const { return static_cast<T*>(p_); }
+    void* get() const { return p_; }
+    std::size_t bytes() const { return bytes_; }
+private:
+    void* p_ = nullptr;
+    std::size_t bytes_ = 0;
+};
+
+// Lanes of a row of `len` off-diagonals (the kernels' lanes_for).
+int lanes_for_host(std::uint32_t len) {
+    std::uint32_t g = (len + kEpl - 1) / kEpl;
+    if (g <= 1) return 1;
+    std::uint32_t p = 1;
+    while (p < g) p <<= 1;
+    return p > 64 ? 64 : static_cast<int>(p);
+}
+
+int center_k_from_env() {
+    int k = 10;
+    if (const char* e = std::getenv("APXCHOL_CENTER_K"); e && *e) {
+        const int v = std::atoi(e);
+        if (v >= 1) k = v;
+    }
+    if (const char* e = std::getenv("APXCHOL_GROUND"); e && *e && std::strcmp(e, "center") == 0) k = 1;
+    return k;
+}
+
+}  // namespace
+
+batch_table build_batches(const trsv_factor& T, bool backward) {
+    batch_table bt;
+    const std::uint32_t m = T.m;
+    const auto& ptr = backward ? T.b_ptr : T.f_ptr;
+    const std::size_t L = T.level_ptr.size() - 1;
+    std::uint64_t mask = 0;
+    int lane = 0;
+    bool open = false;
+    auto close = [&]() {
+        if (!open) return;
+        bt.mask.push_back(mask);
+        open = false;
+        mask = 0;
+        lane = 0;
+    };
+    auto open_at = [&](std::uint32_t t) {
+        bt.pos.push_back(t);
+        open = true;
+        mask = 0;
+        lane = 0;
+    };
+    for (std::size_t li = 0; li < L; ++li) {
+        // Level li in sweep order; positions [p0, p1).
+        std::uint32_t p0, p1;
+        if (!backward) { p0 = T.level_ptr[li]; p1 = T.level_ptr[li + 1]; }
+        else { p0 = m - T.level_ptr[L - li]; p1 = m - T.level_ptr[L - li - 1]; }
+        close();
+        bt.level_first.push_back(static_cast<int>(bt.mask.size()));
+        for (std::uint32_t t = p0; t < p1; ++t) {
+            const std::uint32_t row = backward ? m - 1 - t : t;
+            const std::uint32_t len = ptr[row + 1] - ptr[row];
+            if (len > static_cast<std::uint32_t>(kWave * kEpl)) {
+                close();
+                open_at(t);
+                mask = 1;
+                close();
+                continue;
+            }
+            const int G = lanes_for_host(len);
+            int a = (lane + G - 1) / G * G;
+            if (open && a + G > kWave) { close(); a = 0; }
+            if (!open) { open_at(t); a = 0; }
+            mask |= 1ull << a;
+            lane = a + G;
+        }
+    }
+    close();
+    bt.level_first.push_back(static_cast<int>(bt.mask.size()));
+    bt.pos.push_back(m);
+    return bt;
+}
+
+// Every invariant the sweep kernels rely on, checked on the host before any
+// upload: entry ids point strictly backwards in sweep order and inside [0, m);
+// batches tile [0, m) in order; a batch's mask has one bit per row; every
+// row's aligned lane group fits the wave without overlap; a row longer than
+// 64 * kEpl is alone in its batch; levels start on batch boundaries.
+void validate_sweep_layout(const trsv_factor& T, const batch_table& bt, bool backward) {
+    const std::uint32_t m = T.m;
+    const auto& ptr = backward ? T.b_ptr : T.f_ptr;
+    const auto& idx = backward ? T.b_idx : T.f_idx;
+    auto fail = [](const std::string& w) { throw std::runtime_error("apxchol_amd: invalid sweep layout: " + w); };
+    if (ptr.size() != static_cast<std::size_t>(m) + 1 || ptr[0] != 0 || ptr[m] != idx.size()) fail("ptr");
+    for (std::uint32_t r = 0; r < m; ++r) {
+        if (ptr[r + 1] < ptr[r]) fail("ptr order");
+        for (std::uint32_t p = ptr[r]; p < ptr[r + 1]; ++p) {
+            const std::int64_t j = idx[p];
+            if (j < 0 || j >= static_cast<std::int64_t>(m)) fail("index range");
+            if (backward ? j <= static_cast<std::int64_t>(r) : j >= static_cast<std::int64_t>(r)) fail("dependency direction");
+        }
+    }
+    const std::size_t nb = bt.mask.size();
+    if (bt.pos.size() != nb + 1 || bt.pos[0] != 0 || bt.pos[nb] != m) fail("batch bounds");
+    for (std::size_t b = 0; b < nb; ++b) {
+        const std::uint32_t p0 = bt.pos[b], p1 = bt.pos[b + 1];
+        if (p1 <= p0) fail("empty batch");
+        if (static_cast<std::uint32_t>(__builtin_popcountll(bt.mask[b])) != p1 - p0 || !(bt.mask[b] & 1ull))
+            fail("batch mask");
+        std::uint64_t used = 0;
+        int k = 0;
+        for (int lane = 0; lane < kWave; ++lane) {
+            if (!((bt.mask[b] >> lane) & 1ull)) continue;
+            const std::uint32_t t = p0 + static_cast<std::uint32_t>(k++);
+            const std::uint32_t row = backward ? m - 1 - t : t;
+            const std::uint32_t len = ptr[row + 1] - ptr[row];
+            const int G = lanes_for_host(len);
+            if (len > static_cast<std::uint32_t>(kWave * kEpl) && (p1 - p0 != 1 || G != kWave)) fail("long row not alone");
+            if (lane % G != 0 || lane + G > kWave) fail("lane group alignment");
+            const std::uint64_t g = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << lane;
+            if (used & g) fail("lane group overlap");
+            used |= g;
+            // Lanes up to the next row start map to this row (padding lanes idle).
+            const std::uint64_t after = bt.mask[b] >> lane >> 1;
+            const int next = after ? lane + 1 + __builtin_ctzll(after) : kWave;
+            if (next < lane + G) fail("next group inside this group");
+        }
+    }
+    const std::size_t L = T.level_ptr.size() - 1;
+    if (bt.level_first.size() != L + 1 || bt.level_first[0] != 0 || bt.level_first[L] != static_cast<int>(nb))
+        fail("level batch bounds");
+    for (std::size_t l = 0; l < L; ++l) {
+        if (bt.level_first[l + 1] < bt.level_first[l]) fail("level order");
+        const std::uint32_t lp0 = backward ? m - T.level_ptr[L - l] : T.level_ptr[l];
+        if (bt.level_first[l] < static_cast<int>(nb) && bt.pos[bt.level_first[l]] != lp0) fail("level start");
+    }
+}
+
+struct gpu_solver::device_state {
+    vec_prec prec = vec_prec::fp64;
+    bool fp16 = false;
+    int grid = 0;
+    dbuf f_ptr, f_ent, f_idx, f_h, b_ptr, b_ent, b_idx, b_h, diag, inv_scale;
+    dbuf f_pos, f_mask, b_pos, b_mask;
+    std::vector<int> f_levels, b_levels;
+    dbuf perm, op_ptr, op_idx, op_val;
+    bool op32 = false;
+    int lanes = 8;
+    dbuf r, x, p, ap, rp, y, z, xout;
+    dbuf part, counter, st, st_apply, ctrl, err;
+    pcg_state* host_st = nullptr;
+    int* host_err = nullptr;
+    sweep_desc fwd, bck;
+    ~device_state() {
+        if (host_st) (void)hipHostFree(host_st);
+        if (host_err) (void)hipHostFree(host_err);
+    }
+};
+
+gpu_solver::gpu_solver(const csc_view& L, const solve_options& opts) : opts_(opts) {
+    F_ = factorize(L, opts_.factor_opts);
+    setup(L);
+}
+
+gpu_solver::gpu_solver(const csc_view& L, factorization F, const solve_options& opts) : opts_(opts) {
+    if (static_cast<std::int64_t>(F.n) != L.n)
+        throw std::invalid_argument("gpu_solver: factorization dimension mismatch");
+    if (F.nnz() > 0 && F.vals.empty())
+        throw std::invalid_argument("gpu_solver: factorization values were released; pass a freshly computed factorization");
+    F_ = std::move(F);
+    setup(L);
+}
+
+gpu_solver::~gpu_solver() = default;
+
+void gpu_solver::setup(const csc_view& L) {
+    n_ = L.n;
+    if (n_ > (1ll << 31) - 2) throw std::invalid_argument("gpu_solver: n exceeds the 32-bit device index range");
+    center_k_ = center_k_from_env();
+    T_ = prepare_trsv(F_, opts_.fp16_storage, opts_.drop_rel);
+    const permuted_operator op = build_permuted_operator(L, F_.perm);
+    if (op.ptr.back() > static_cast<std::int64_t>(UINT32_MAX))
+        throw std::invalid_argument("gpu_solver: operator nnz exceeds the 32-bit device index range");
+    for (const std::int32_t c : op.idx)
+        if (c < 0 || c >= n_) throw std::runtime_error("apxchol_amd: operator index out of range");
+
+    dev_ = std::make_unique<device_state>();
+    auto& d = *dev_;
+    d.prec = opts_.precision == sweep_precision::fp32 ? vec_prec::fp32 : vec_prec::fp64;
+    d.fp16 = T_.fp16;
+    const int m = static_cast<int>(T_.m);
+    const int n = static_cast<int>(n_);
+
+    // Factor layout: per direction a CSR of the off-diagonals; fp32 values
+    // interleaved with their ids (entry32), fp16 values in their own array.
+    d.f_ptr.upload(T_.f_ptr);
+    d.b_ptr.upload(T_.b_ptr);
+    if (T_.fp16) {
+        d.f_idx.upload(T_.f_idx);
+        d.f_h.upload(T_.f_h);
+        d.b_idx.upload(T_.b_idx);
+        d.b_h.upload(T_.b_h);
+        d.inv_scale.upload(T_.inv_scale);
+    } else {
+        std::vector<entry32> fe(T_.f_idx.size()), be(T_.b_idx.size());
+        for (std::size_t i = 0; i < fe.size(); ++i) fe[i] = {T_.f_idx[i], T_.f_val[i]};
+        for (std::size_t i = 0; i < be.size(); ++i) be[i] = {T_.b_idx[i], T_.b_val[i]};
+        d.f_ent.upload(fe);
+        d.b_ent.upload(be);
+    }
+    d.diag.upload(T_.diag);
+    const batch_table fb = build_batches(T_, false), bb = build_batches(T_, true);
+    validate_sweep_layout(T_, fb, false);
+    validate_sweep_layout(T_, bb, true);
+    if (T_.diag.size() != static_cast<std::size_t>(m) || F_.perm.size() != static_cast<std::size_t>(n))
+        throw std::runtime_error("apxchol_amd: invalid factor layout");
+    {
+        std::vector<std::uint8_t> seen(n, 0);
+        for (const node_t q : F_.perm) {
+            if (q >= static_cast<node_t>(n) || seen[q]) throw std::runtime_error("apxchol_amd: perm is not a permutation");
+            seen[q] = 1;
+        }
+    }
+    d.f_pos.upload(fb.pos);
+    d.f_mask.upload(fb.mask);
+    d.b_pos.upload(bb.pos);
+    d.b_mask.upload(bb.mask);
+    d.f_levels = fb.level_first;
+    d.b_levels = bb.level_first;
+
+    std::vector<std::uint32_t> perm(F_.perm.begin(), F_.perm.end());
+    d.perm.upload(perm);
+    std::vector<std::uint32_t> optr(op.ptr.begin(), op.ptr.end());
+    d.op_ptr.upload(optr);
+    d.op_idx.upload(op.idx);
+    d.op32 = op.fp32_exact;
+    if (d.op32) d.op_val.upload(op.val32); else d.op_val.upload(op.val);
+    // Lanes per SpMV row: power of two nearest the mean row length (<= 64).
+    const double avg = n > 0 ? static_cast<double>(op.ptr.back()) / n : 1.0;
+    int lanes = 1;
+    while (lanes < 64 && lanes * 1.5 < avg) lanes <<= 1;
+    d.lanes = lanes;
+
+    const std::size_t vb = static_cast<std::size_t>(std::max(n, 1)) * sizeof(double);
+    d.r.alloc(vb); d.x.alloc(vb); d.p.alloc(vb); d.ap.alloc(vb); d.rp.alloc(vb); d.y.alloc(vb); d.z.alloc(vb);
+    d.xout.alloc(vb);
+    d.part.alloc(2 * 2048 * sizeof(double));
+    d.counter.alloc(64);
+    d.counter.zero();
+    d.st.alloc(sizeof(pcg_state));
+    d.st_apply.alloc(sizeof(pcg_state));
+    d.st_apply.zero();
+    d.ctrl.alloc(4 * sizeof(int));
+    d.ctrl.zero();
+    d.err.alloc(sizeof(int));
+    d.err.zero();
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&d.host_st), sizeof(pcg_state), hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&d.host_err), sizeof(int), hipHostMallocDefault), "hipHostMalloc");
+
+    auto fill_desc = [&](sweep_desc& s, bool backward, const batch_table& bt) {
+        s.m = m;
+        s.backward = backward ? 1 : 0;
+        s.ptr = (backward ? d.b_ptr : d.f_ptr).as<std::uint32_t>();
+        if (T_.fp16) {
+            s.idx = (backward ? d.b_idx : d.f_idx).as<std::int32_t>();
+            s.h = (backward ? d.b_h : d.f_h).as<std::uint16_t>();
+            s.inv_scale = backward ? d.inv_scale.as<float>() : nullptr;
+        } else {
+            s.ent = (backward ? d.b_ent : d.f_ent).as<entry32>();
+        }
+        s.diag = d.diag.as<float>();
+        s.batch_pos = (backward ? d.b_pos : d.f_pos).as<std::uint32_t>();
+        s.batch_mask = (backward ? d.b_mask : d.f_mask).as<std::uint64_t>();
+        s.nb = static_cast<int>(bt.mask.size());
+        s.ctrl = d.ctrl.as<int>() + (backward ? 2 : 0);
+    };
+    fill_desc(d.fwd, false, fb);
+    fill_desc(d.bck, true, bb);
+    d.grid = sweep_dataflow_grid(d.prec, d.fp16);
+
+    stats_.n = n_;
+    stats_.m = m;
+    stats_.sddm = F_.sddm;
+    stats_.factor_nnz = F_.nnz();
+    stats_.stored_nnz = T_.drop.nnz_stored;
+    stats_.offdiag = T_.offdiag();
+    stats_.dropped = T_.drop.dropped;
+    stats_.rounds = static_cast<std::int64_t>(F_.rounds.size());
+    stats_.levels = static_cast<std::int64_t>(T_.level_ptr.size()) - 1;
+    stats_.fwd_batches = d.fwd.nb;
+    stats_.bck_batches = d.bck.nb;
+    stats_.op_nnz = op.ptr.back();
+    stats_.op_fp32 = d.op32;
+    stats_.spmv_lanes = lanes;
+    stats_.dataflow_grid = d.grid;
+    std::uint64_t bytes = 0;
+    for (const dbuf* b : {&d.f_ptr, &d.f_ent, &d.f_idx, &d.f_h, &d.b_ptr, &d.b_ent, &d.b_idx, &d.b_h, &d.diag,
+                          &d.inv_scale, &d.f_pos, &d.f_mask, &d.b_pos, &d.b_mask, &d.perm, &d.op_ptr, &d.op_idx,
+                          &d.op_val, &d.r, &d.x, &d.p, &d.ap, &d.rp, &d.y, &d.z, &d.xout})
+        bytes += b->bytes();
+    stats_.device_bytes = bytes;
+
+    if (!opts_.keep_factor_values) {
+        std::vector<node_t>().swap(F_.inner);
+        std::vector<float>().swap(F_.vals);
+    }
+    hip_check(hipDeviceSynchronize(), "setup");
+}
+
+void gpu_solver::check_sweep_error() const {
+    hip_check(hipMemcpy(dev_->host_err, dev_->err.get(), sizeof(int), hipMemcpyDeviceToHost), "read sweep flag");
+    if (*dev_->host_err != 0) {
+        dev_->err.zero();
+        throw std::runtime_error("apxchol_amd: a sweep dependency did not arrive within the spin limit");
+    }
+}
+
+solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol, int max_iter, const double* d_x0,
+                                      hipStream_t s) const {
+    if (tol < 0.0) tol = opts_.tol;
+    if (max_iter < 0) max_iter = opts_.max_iter;
+    auto& d = *dev_;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    solve_result res;
+    res.lumped_offdiag = F_.lumped_offdiag;
+    if (n == 0) return res;
+    pcg_state& h = *d.host_st;
+    std::memset(&h, 0, sizeof h);
+    h.tol = tol;
+    h.max_iter = max_iter;
+    h.stag_window = opts_.stagnation_window;
+    h.center_k = center_k_;
+    h.laplacian = F_.sddm ? 0 : 1;
+    h.n = n;
+    h.prev_check = 1.0;
+    pcg_state* st = d.st.as<pcg_state>();
+    double* part = d.part.as<double>();
+    unsigned* cnt = d.counter.as<unsigned>();
+    int* err = d.err.as<int>();
+    const vec_prec pr = d.prec;
+    hip_check(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s), "state upload");
+
+    launch_scatter_init(n, d.perm.as<unsigned>(), d_b, d_x0, d.r.as<double>(), d.x.as<double>(), s);
+    if (d_x0) {
+        launch_norm2_b(n, d_b, st, part, cnt, s);
+        launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes, d.x.as<double>(),
+                    d.ap.as<double>(), st, part, cnt, false, s);
+        launch_init_residual(n, d.r.as<double>(), d.ap.as<double>(), st, part, cnt, s);
+    } else {
+        launch_init_residual(n, d.r.as<double>(), nullptr, st, part, cnt, s);
+    }
+    const int* fl = opts_.schedule == sweep_schedule::levels ? d.f_levels.data() : nullptr;
+    const int* bl = opts_.schedule == sweep_schedule::levels ? d.b_levels.data() : nullptr;
+    const int fnl = static_cast<int>(d.f_levels.size()) - 1, bnl = static_cast<int>(d.b_levels.size()) - 1;
+    auto precondition = [&](bool first) {
+        if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
+        launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, fl, fnl, d.grid, s);
+        launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, bl, bnl, d.grid, s);
+        launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
+        launch_update_p(pr, n, d.p.as<double>(), d.z.get(), st, first, s);
+    };
+    precondition(true);
+    const int J = std::max(1, opts_.iters_per_check);
+    int enqueued = 0;
+    for (;;) {
+        for (int j = 0; j < J; ++j) {
+            launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
+                        d.p.as<double>(), d.ap.as<double>(), st, part, cnt, true, s);
+            launch_update_xr(pr, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
+                             d.y.get(), d.z.get(), st, part, cnt, s);
+            precondition(false);
+        }
+        enqueued += J;
+        hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
+        hip_check(hipStreamSynchronize(s), "solve");
+        if (h.done || enqueued > max_iter + J) break;
+    }
+    if (!F_.sddm) launch_sum_vec(n, d.x.as<double>(), st, part, cnt, s);
+    launch_gather_shift(n, d.perm.as<unsigned>(), d.x.as<double>(), d_x, st, !F_.sddm, s);
+    hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
+    hip_check(hipStreamSynchronize(s), "solve");
+    if (h.status == 5) hip_check(hipMemsetAsync(d_x, 0, sizeof(double) * n, s), "zero x");
+    check_sweep_error();
+    res.iterations = h.iter;
+    res.residual = h.residual;
+    res.status = h.status;
+    std::size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && tot >= fr) res.solve_vram_mb = static_cast<double>(tot - fr) / (1024.0 * 1024.0);
+    return res;
+}
+
+solve_result gpu_solver::solve(const std::vector<double>& b, double tol, int max_iter,
+                               const std::vector<double>* x0) const {
+    if (static_cast<std::int64_t>(b.size()) != n_) throw std::invalid_argument("gpu_solver::solve: b length mismatch");
+    if (x0 && static_cast<std::int64_t>(x0->size()) != n_)
+        throw std::invalid_argument("gpu_solver::solve: x0 length mismatch");
+    const std::size_t bytes = static_cast<std::size_t>(n_) * sizeof(double);
+    dbuf db(bytes), dx(bytes), dx0(x0 ? bytes : 0);
+    hip_check(hipMemcpy(db.get(), b.data(), bytes, hipMemcpyHostToDevice), "upload b");
+    if (x0) hip_check(hipMemcpy(dx0.get(), x0->data(), bytes, hipMemcpyHostToDevice), "upload x0");
+    solve_result res = solve_device(db.as<double>(), dx.as<double>(), tol, max_iter, x0 ? dx0.as<double>() : nullptr, nullptr);
+    res.x.resize(n_);
+    hip_check(hipMemcpy(res.x.data(), dx.get(), bytes, hipMemcpyDeviceToHost), "download x");
+    return res;
+}
+
+void gpu_solver::apply_device(const double* d_r, double* d_z, hipStream_t s) const {
+    auto& d = *dev_;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    if (n == 0) return;
+    // The center-k schedule of the reference's _solve_impl
+    // (preconditioner.h:344-353, 393-397): every K-th application centres.
+    const bool center = !F_.sddm && (++apply_count_ % static_cast<std::uint64_t>(center_k_) == 0);
+    pcg_state* st = d.st_apply.as<pcg_state>();
+    double* part = d.part.as<double>();
+    unsigned* cnt = d.counter.as<unsigned>();
+    int* err = d.err.as<int>();
+    const int* fl = opts_.schedule == sweep_schedule::levels ? d.f_levels.data() : nullptr;
+    const int* bl = opts_.schedule == sweep_schedule::levels ? d.b_levels.data() : nullptr;
+    launch_sum_shift(n, d_r, st, part, cnt, center, s);
+    launch_apply_scatter(d.prec, n, m, d.perm.as<unsigned>(), d_r, d.rp.as<double>(), d.y.get(), d.z.get(), s);
+    launch_sweep(d.fwd, d.prec, d.fp16, d.rp.get(), d.y.get(), st, err, fl, static_cast<int>(d.f_levels.size()) - 1, d.grid, s);
+    launch_sweep(d.bck, d.prec, d.fp16, d.y.get(), d.z.get(), st, err, bl, static_cast<int>(d.b_levels.size()) - 1, d.grid, s);
+    if (center) launch_sum_out(d.prec, n, d.z.get(), st, part, cnt, s);
+    launch_apply_gather(d.prec, n, d.perm.as<unsigned>(), d.z.get(), d_z, st, center, s);
+}
+
+std::vector<double> gpu_solver::apply(const std::vector<double>& r) const {
+    if (static_cast<std::int64_t>(r.size()) != n_) throw std::invalid_argument("gpu_solver::apply: r length mismatch");
+    const std::size_t bytes = static_cast<std::size_t>(n_) * sizeof(double);
+    dbuf dr(bytes), dz(bytes);
+    hip_check(hipMemcpy(dr.get(), r.data(), bytes, hipMemcpyHostToDevice), "upload r");
+    apply_device(dr.as<double>(), dz.as<double>(), nullptr);
+    std::vector<double> z(n_);
+    hip_check(hipMemcpy(z.data(), dz.get(), bytes, hipMemcpyDeviceToHost), "download z");
+    check_sweep_error();
+    return z;
+}
+
+void gpu_solver::sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const {
+    auto& d = *dev_;
+    const int m = static_cast<int>(T_.m);
+    if (m == 0) return;
+    launch_fill_pending(vec_prec::fp64, d_out, m, s);
+    const bool lv = opts_.schedule == sweep_schedule::levels;
+    const auto& lvl = backward ? d.b_levels : d.f_levels;
+    launch_sweep(backward ? d.bck : d.fwd, vec_prec::fp64, d.fp16, d_rhs, d_out, nullptr, d.err.as<int>(),
+                 lv ? lvl.data() : nullptr, static_cast<int>(lvl.size()) - 1, d.grid, s);
+}
+
+solve_result solve(const csc_view& L, const std::vector<double>& b, const solve_options& opts) {
+    const gpu_solver slv(L, opts);
+    return slv.solve(b);
+}
+
+}  // namespace apxchol_amd

@@ -1,0 +1,109 @@
+// gpu_solver: the MI355X counterpart of the reference's cpu_solver
+// (include/apxchol/solver/solve.h:64-131) -- factor once on the host, upload,
+// then PCG-solve any number of right-hand sides on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "factor.hpp"
+#include "trsv_prep.hpp"
+
+namespace apxchol_amd {
+
+enum class sweep_precision { fp64 = 0, fp32 = 1 };   // vector / accumulation width of the two sweeps
+enum class sweep_schedule { dataflow = 0, levels = 1 };
+
+// solve.h:13-25 plus the device knobs.
+struct solve_options {
+    double tol = 1e-8;
+    int max_iter = 200;
+    int stagnation_window = 50;
+    bool keep_factor_values = false;
+    factor_options factor_opts;
+    sweep_precision precision = sweep_precision::fp64;
+    bool fp16_storage = false;
+    sweep_schedule schedule = sweep_schedule::dataflow;
+    double drop_rel = -1.0;       // < 0: APXCHOL_FACTOR_DROP / the 1e-4 default
+    int iters_per_check = 4;      // iterations enqueued between completion tests
+};
+
+// solve.h:27-42. status: 0 running/none, 1 converged, 2 pAp <= 0 breakdown,
+// 3 stagnation, 4 max_iter, 5 zero right-hand side.
+struct solve_result {
+    std::vector<double> x;
+    std::int64_t iterations = 0;
+    double residual = 0.0;
+    std::int64_t lumped_offdiag = 0;
+    double solve_vram_mb = -1.0;
+    int status = 0;
+};
+
+struct solver_stats {
+    std::int64_t n = 0, m = 0;
+    bool sddm = false;
+    std::uint64_t factor_nnz = 0, stored_nnz = 0, offdiag = 0, dropped = 0;
+    std::int64_t rounds = 0, levels = 0, fwd_batches = 0, bck_batches = 0;
+    std::int64_t op_nnz = 0;
+    bool op_fp32 = false;
+    int spmv_lanes = 0, dataflow_grid = 0;
+    std::uint64_t device_bytes = 0;
+};
+
+class gpu_solver {
+public:
+    explicit gpu_solver(const csc_view& L, const solve_options& opts = {});
+    gpu_solver(const csc_view& L, factorization F, const solve_options& opts = {});
+    ~gpu_solver();
+    gpu_solver(const gpu_solver&) = delete;
+    gpu_solver& operator=(const gpu_solver&) = delete;
+
+    // Host-vector forms (solve.h:83-103).
+    solve_result solve(const std::vector<double>& b, double tol = -1.0, int max_iter = -1,
+                       const std::vector<double>* x0 = nullptr) const;
+    std::vector<double> apply(const std::vector<double>& r) const;
+
+    // Device-vector forms: every pointer is device memory of n doubles; the
+    // result's x is left empty.
+    solve_result solve_device(const double* d_b, double* d_x, double tol, int max_iter, const double* d_x0,
+                              hipStream_t s) const;
+    void apply_device(const double* d_r, double* d_z, hipStream_t s) const;
+    // One sweep in factor order with fp64 vectors (rhs, out: m doubles).
+    void sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const;
+
+    void reset_apply_count() const { apply_count_ = 0; }
+    const factorization& factor() const { return F_; }
+    const trsv_factor& trsv() const { return T_; }
+    const solver_stats& stats() const { return stats_; }
+    std::int64_t rows() const { return n_; }
+
+private:
+    struct device_state;
+    void setup(const csc_view& L);
+    void check_sweep_error() const;
+
+    solve_options opts_;
+    factorization F_;
+    trsv_factor T_;
+    solver_stats stats_;
+    std::int64_t n_ = 0;
+    int center_k_ = 10;
+    mutable std::uint64_t apply_count_ = 0;
+    std::unique_ptr<device_state> dev_;
+};
+
+// One-shot solve (solve.h:136-138).
+solve_result solve(const csc_view& L, const std::vector<double>& b, const solve_options& opts = {});
+
+// Batch tables of one sweep direction (host side, exposed for tests).
+struct batch_table {
+    std::vector<std::uint32_t> pos;      // nb+1
+    std::vector<std::uint64_t> mask;     // nb
+    std::vector<int> level_first;        // levels+1
+};
+batch_table build_batches(const trsv_factor& T, bool backward);
+void validate_sweep_layout(const trsv_factor& T, const batch_table& bt, bool backward);
+
+}  // namespace apxchol_amd

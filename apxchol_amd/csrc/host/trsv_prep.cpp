@@ -1,0 +1,242 @@
+// Host preparation of the factor for the GPU sweeps (the reference's
+// omp_sptrsv::setup_impl, solver/sptrsv/omp.h:505-861, and cuda_sptrsv::setup,
+// solver/sptrsv/cuda.h:233-459 / cuda_host.h:62-232, restated) and of the
+// permuted PCG operator (solver/pcg_cuda.h:304-404).
+#include "trsv_prep.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+
+namespace apxchol_amd {
+
+std::uint16_t fp16_round(float x) {
+    const _Float16 h = static_cast<_Float16>(x);   // IEEE RNE (compiler-rt / F16C)
+    std::uint16_t b;
+    std::memcpy(&b, &h, sizeof b);
+    return b;
+}
+
+float fp16_widen(std::uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, sizeof h);
+    return static_cast<float>(h);
+}
+
+namespace {
+
+bool fp16_zero_or_subnormal(std::uint16_t b) { return (b & 0x7c00u) == 0; }
+
+// What the fp16 storage keeps for an off-diagonal v of a column with scale s:
+// fp16(v / s), its subnormals flushed to signed zero (omp.h:353-366).
+std::uint16_t narrow16(float v, float s) {
+    const std::uint16_t b = fp16_round(v / s);
+    return fp16_zero_or_subnormal(b) ? static_cast<std::uint16_t>(b & 0x8000u) : b;
+}
+
+double drop_rel_from_env() {
+    const char* e = std::getenv("APXCHOL_FACTOR_DROP");
+    if (!e || !*e) return kFactorDropRelDefault;
+    const double r = std::atof(e);
+    return r > 0.0 ? r : 0.0;
+}
+
+}  // namespace
+
+trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel) {
+    if (rel < 0.0) rel = drop_rel_from_env();
+    trsv_factor T;
+    const node_t n = F.n;
+    const node_t m = F.sddm ? n : (n > 0 ? n - 1 : 0);
+    T.m = m;
+    T.fp16 = fp16;
+    if (F.nnz() > std::numeric_limits<std::int32_t>::max())
+        throw std::runtime_error("apxchol_amd: factor nnz exceeds the 32-bit index range of the device layout");
+
+    // L11: columns 0..m-1 without row n-1 on the Laplacian path. Columns keep
+    // the diagonal first, rows ascending.
+    std::vector<std::uint64_t> c_ptr(static_cast<std::size_t>(m) + 1, 0);
+    std::vector<node_t> c_row;
+    std::vector<float> c_val;
+    c_row.reserve(F.nnz());
+    c_val.reserve(F.nnz());
+    for (node_t j = 0; j < m; ++j) {
+        for (off_t64 p = F.outer[j]; p < F.outer[j + 1]; ++p) {
+            if (F.inner[p] >= m) continue;
+            c_row.push_back(F.inner[p]);
+            c_val.push_back(F.vals[p]);
+        }
+        c_ptr[j + 1] = c_row.size();
+    }
+    T.drop.rel = rel;
+    T.drop.nnz_factor = c_row.size();
+
+    // Column scale s_j = max |off-diagonal| (1 if none), before the drop
+    // (factor_drop.h:83-89); the fp16 storage falls back to s_j = 1 where
+    // 1/s_j or L_jj/s_j would overflow fp32 (omp.h:601-615).
+    std::vector<float> scale(m, 1.0f);
+    if (rel > 0.0 || fp16) {
+        for (node_t j = 0; j < m; ++j) {
+            double mx = 0.0;
+            for (std::uint64_t p = c_ptr[j] + 1; p < c_ptr[j + 1]; ++p)
+                mx = std::max(mx, std::fabs(static_cast<double>(c_val[p])));
+            scale[j] = mx > 0.0 ? static_cast<float>(mx) : 1.0f;
+            if (fp16) {
+                const float s = scale[j];
+                const float d = c_val[c_ptr[j]];
+                if (!std::isfinite(1.0f / s) || !std::isfinite(d / s)) scale[j] = 1.0f;
+            }
+        }
+    }
+
+    // Compacting drop (factor_drop.h:131-211): keep an off-diagonal iff
+    // |v| >= rel * s_j and the storage keeps it nonzero; spread each column's
+    // dropped mass over its kept off-diagonals in proportion to |v|.
+    auto keep = [&](float v, float s) {
+        if (!(std::fabs(static_cast<double>(v)) >= rel * static_cast<double>(s))) return false;
+        if (fp16) return !fp16_zero_or_subnormal(fp16_round(v / s));
+        return v != 0.0f;
+    };
+    if (rel > 0.0) {
+        std::vector<std::uint64_t> d_ptr(static_cast<std::size_t>(m) + 1, 0);
+        std::vector<node_t> d_row;
+        std::vector<float> d_val;
+        d_row.reserve(c_row.size());
+        d_val.reserve(c_row.size());
+        for (node_t j = 0; j < m; ++j) {
+            const float s = scale[j];
+            const std::uint64_t first = d_row.size();
+            double dropped_sum = 0.0, kept_abs = 0.0;
+            for (std::uint64_t p = c_ptr[j]; p < c_ptr[j + 1]; ++p) {
+                const float v = c_val[p];
+                if (c_row[p] != j) {
+                    if (!keep(v, s)) { dropped_sum += static_cast<double>(v); continue; }
+                    kept_abs += std::fabs(static_cast<double>(v));
+                }
+                d_row.push_back(c_row[p]);
+                d_val.push_back(v);
+            }
+            if (dropped_sum != 0.0 && kept_abs > 0.0) {
+                const double per_abs = dropped_sum / kept_abs;
+                for (std::uint64_t q = first; q < d_row.size(); ++q) {
+                    if (d_row[q] == j) continue;
+                    const double v = static_cast<double>(d_val[q]);
+                    d_val[q] = static_cast<float>(v + per_abs * std::fabs(v));
+                }
+            }
+            d_ptr[j + 1] = d_row.size();
+        }
+        if (d_row.size() != c_row.size()) {
+            c_ptr.swap(d_ptr);
+            c_row.swap(d_row);
+            c_val.swap(d_val);
+        }
+    }
+    T.drop.nnz_stored = c_row.size();
+    T.drop.dropped = T.drop.nnz_factor - T.drop.nnz_stored;
+
+    // Diagonal and stored values.
+    T.diag.resize(m);
+    std::vector<std::uint16_t> c_h;
+    if (fp16) {
+        c_h.resize(c_row.size());
+        T.inv_scale.resize(m);
+        for (node_t j = 0; j < m; ++j) {
+            const float s = scale[j];
+            T.inv_scale[j] = 1.0f / s;
+            const float d = c_val[c_ptr[j]] / s;
+            double resid = 0.0;
+            for (std::uint64_t p = c_ptr[j]; p < c_ptr[j + 1]; ++p) {
+                const std::uint16_t h = narrow16(c_val[p], s);
+                c_h[p] = h;
+                if (c_row[p] == j) continue;
+                resid += static_cast<double>(c_val[p]) / static_cast<double>(s) - static_cast<double>(fp16_widen(h));
+            }
+            T.diag[j] = static_cast<float>(static_cast<double>(d) + resid);
+            const double r = static_cast<double>(T.inv_scale[j]);
+            if (!(std::isfinite(T.diag[j]) && T.diag[j] != 0.0f && std::isfinite(r * r)))
+                throw std::runtime_error("apxchol_amd: fp16 factor storage cannot represent column " + std::to_string(j));
+        }
+    } else {
+        for (node_t j = 0; j < m; ++j) T.diag[j] = c_val[c_ptr[j]];
+    }
+
+    // Back sweep: the columns without their diagonal.
+    const std::uint64_t off = c_row.size() - m;
+    T.b_ptr.assign(static_cast<std::size_t>(m) + 1, 0);
+    T.b_idx.resize(off);
+    if (fp16) T.b_h.resize(off); else T.b_val.resize(off);
+    {
+        std::uint64_t o = 0;
+        for (node_t j = 0; j < m; ++j) {
+            for (std::uint64_t p = c_ptr[j] + 1; p < c_ptr[j + 1]; ++p, ++o) {
+                T.b_idx[o] = static_cast<std::int32_t>(c_row[p]);
+                if (fp16) T.b_h[o] = c_h[p]; else T.b_val[o] = c_val[p];
+            }
+            T.b_ptr[j + 1] = static_cast<std::uint32_t>(o);
+        }
+    }
+    // Forward sweep: rows of L (counting-sort transpose; column order gives
+    // ascending columns within each row).
+    T.f_ptr.assign(static_cast<std::size_t>(m) + 1, 0);
+    for (std::uint64_t o = 0; o < off; ++o) ++T.f_ptr[T.b_idx[o] + 1];
+    for (node_t i = 0; i < m; ++i) T.f_ptr[i + 1] += T.f_ptr[i];
+    T.f_idx.resize(off);
+    if (fp16) T.f_h.resize(off); else T.f_val.resize(off);
+    {
+        std::vector<std::uint32_t> cur(T.f_ptr.begin(), T.f_ptr.end() - 1);
+        for (node_t j = 0; j < m; ++j)
+            for (std::uint32_t o = T.b_ptr[j]; o < T.b_ptr[j + 1]; ++o) {
+                const std::uint32_t i = static_cast<std::uint32_t>(T.b_idx[o]);
+                const std::uint32_t q = cur[i]++;
+                T.f_idx[q] = static_cast<std::int32_t>(j);
+                if (fp16) T.f_h[q] = T.b_h[o]; else T.f_val[q] = T.b_val[o];
+            }
+    }
+
+    // Round-as-level boundaries (omp.h:890-945): cumulative IS sizes clamped
+    // to m, then one level per residual-peel column.
+    T.level_ptr.assign(1, 0);
+    std::uint64_t acc = 0;
+    for (const auto& r : F.rounds) {
+        acc = std::min<std::uint64_t>(acc + r.is_size, m);
+        if (acc > T.level_ptr.back()) T.level_ptr.push_back(static_cast<std::uint32_t>(acc));
+    }
+    for (std::uint64_t c = T.level_ptr.back(); c < m; ++c) T.level_ptr.push_back(static_cast<std::uint32_t>(c + 1));
+    return T;
+}
+
+permuted_operator build_permuted_operator(const csc_view& A, const std::vector<node_t>& perm) {
+    permuted_operator op;
+    const std::int64_t n = A.n;
+    op.n = n;
+    op.ptr.assign(static_cast<std::size_t>(n) + 1, 0);
+    for (std::int64_t k = 0; k < n; ++k) op.ptr[perm[k] + 1] = A.colptr[k + 1] - A.colptr[k];
+    for (std::int64_t i = 0; i < n; ++i) op.ptr[i + 1] += op.ptr[i];
+    const std::int64_t nnz = op.ptr[n];
+    op.idx.resize(nnz);
+    op.val.resize(nnz);
+    std::vector<std::pair<std::int32_t, double>> row;
+    bool exact = true;
+    for (std::int64_t k = 0; k < n; ++k) {
+        row.clear();
+        for (std::int64_t p = A.colptr[k]; p < A.colptr[k + 1]; ++p) {
+            row.emplace_back(static_cast<std::int32_t>(perm[A.rowidx[p]]), A.vals[p]);
+            exact = exact && static_cast<double>(static_cast<float>(A.vals[p])) == A.vals[p];
+        }
+        std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        std::int64_t o = op.ptr[perm[k]];
+        for (const auto& [c, v] : row) { op.idx[o] = c; op.val[o] = v; ++o; }
+    }
+    op.fp32_exact = exact;
+    if (exact) {
+        op.val32.resize(nnz);
+        for (std::int64_t p = 0; p < nnz; ++p) op.val32[p] = static_cast<float>(op.val[p]);
+        std::vector<double>().swap(op.val);
+    }
+    return op;
+}
+
+}  // namespace apxchol_amd

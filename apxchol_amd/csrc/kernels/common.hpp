@@ -1,0 +1,110 @@
+// Wave64 / workgroup reductions with a FIXED combination order (the result is a
+// function of the inputs and the launch shape only, never of scheduling), and
+// the flagged loads/stores the sync-free sweeps use.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../device.hpp"
+
+namespace apxchol_amd {
+
+inline constexpr int kBlock = 256;          // threads per workgroup of the vector kernels
+inline constexpr int kWavesPerBlock = kBlock / kWave;
+
+// Butterfly over the whole wave: every lane ends with the same total.
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over a workgroup of kBlock threads: per-wave butterfly, then wave 0..3
+// in order. Valid in thread 0 only. `sh` holds kWavesPerBlock * N values.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* sh) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[w * N + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double s = sh[k];
+            for (int j = 1; j < kWavesPerBlock; ++j) s += sh[j * N + k];
+            v[k] = s;
+        }
+}
+
+// Two-stage deterministic grid reduction: each block stores its N partials in
+// part[block * N + k]; the last block to arrive (atomic counter) sums all
+// partials in block order and returns true in thread 0 with the totals in v.
+// The counter is reset by that block, so it is zero between launches.
+template <int N>
+__device__ __forceinline__ bool grid_sum(double (&v)[N], double* part, unsigned* counter, double* sh) {
+    block_sum<N>(v, sh);
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            __hip_atomic_store(&part[blockIdx.x * N + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        is_last = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            acc[k] += __hip_atomic_load(&part[b * N + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<N>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = acc[k];
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return threadIdx.x == 0;
+}
+
+// Grid shape of the vector kernels: a pure function of n (so reductions are
+// reproducible), at most kMaxBlocks workgroups.
+inline constexpr int kMaxBlocks = 2048;
+inline int vec_blocks(long long n) {
+    long long b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return static_cast<int>(b > kMaxBlocks ? kMaxBlocks : b);
+}
+
+template <class T> struct flag_bits;
+template <> struct flag_bits<double> { using type = unsigned long long; static constexpr type pending = kPending64; };
+template <> struct flag_bits<float> { using type = unsigned int; static constexpr type pending = kPending32; };
+
+template <class T>
+__device__ __forceinline__ typename flag_bits<T>::type load_flag(const T* p) {
+    using B = typename flag_bits<T>::type;
+    return __hip_atomic_load(reinterpret_cast<const B*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ __forceinline__ void store_flag(T* p, T v) {
+    using B = typename flag_bits<T>::type;
+    __hip_atomic_store(reinterpret_cast<B*>(p), __builtin_bit_cast(B, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ __forceinline__ T from_bits(typename flag_bits<T>::type b) { return __builtin_bit_cast(T, b); }
+
+template <class T>
+__device__ __forceinline__ T pending_value() { return __builtin_bit_cast(T, flag_bits<T>::pending); }
+
+}  // namespace apxchol_amd

@@ -1,0 +1,194 @@
+// Launchers of the gfx950 kernels (one translation unit so the templates
+// instantiate once).
+#include "launch.hpp"
+
+#include <stdexcept>
+#include <string>
+
+#include "pcg.hip"
+#include "sweep.hip"
+
+namespace apxchol_amd {
+
+namespace {
+
+void check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("apxchol_amd: launch of ") + what + " failed: " + hipGetErrorString(e));
+}
+
+template <class TI, class TO, bool F16>
+int dataflow_grid_for() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_dataflow<TI, TO, F16>, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return per_cu * cus;
+}
+
+template <class TI, class TO, bool F16>
+void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* st, int* err, const int* lb, int nl,
+             int grid, hipStream_t s) {
+    const TI* in = static_cast<const TI*>(rhs);
+    TO* o = static_cast<TO*>(out);
+    if (!lb) {
+        sweep_dataflow<TI, TO, F16><<<grid, kBlock, 0, s>>>(d, in, o, st, grid * kWavesPerBlock, err);
+        check_launch("sweep_dataflow");
+        return;
+    }
+    for (int l = 0; l < nl; ++l) {
+        const int b0 = lb[l], b1 = lb[l + 1];
+        if (b1 <= b0) continue;
+        int g = (b1 - b0 + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (g > grid) g = grid;
+        sweep_levels<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b0, b1, in, o, st, err);
+    }
+    check_launch("sweep_levels");
+}
+
+}  // namespace
+
+int sweep_dataflow_grid(vec_prec p, bool fp16) {
+    if (p == vec_prec::fp64) return fp16 ? dataflow_grid_for<double, double, true>() : dataflow_grid_for<double, double, false>();
+    return fp16 ? dataflow_grid_for<float, float, true>() : dataflow_grid_for<float, float, false>();
+}
+
+void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
+                  int* err, const int* lb, int nl, int grid, hipStream_t s) {
+    if (d.nb == 0) return;
+    if (p == vec_prec::fp64) {
+        if (fp16) sweep_t<double, double, true>(d, rhs, out, st, err, lb, nl, grid, s);
+        else sweep_t<double, double, false>(d, rhs, out, st, err, lb, nl, grid, s);
+    } else if (!d.backward) {
+        if (fp16) sweep_t<double, float, true>(d, rhs, out, st, err, lb, nl, grid, s);
+        else sweep_t<double, float, false>(d, rhs, out, st, err, lb, nl, grid, s);
+    } else {
+        if (fp16) sweep_t<float, float, true>(d, rhs, out, st, err, lb, nl, grid, s);
+        else sweep_t<float, float, false>(d, rhs, out, st, err, lb, nl, grid, s);
+    }
+}
+
+void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) fill_outputs<double><<<g, kBlock, 0, s>>>(n, m, static_cast<double*>(y), static_cast<double*>(z), st);
+    else fill_outputs<float><<<g, kBlock, 0, s>>>(n, m, static_cast<float*>(y), static_cast<float*>(z), st);
+    check_launch("fill_outputs");
+}
+
+template <class VT>
+static void spmv_t(int n, const unsigned* ptr, const int* idx, const VT* val, int lanes, const double* x, double* y,
+                   pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s) {
+    const int g = vec_blocks(static_cast<long long>(n) * lanes);
+    const int w = want_dot ? 1 : 0;
+    switch (lanes) {
+        case 1: spmv_pAp<VT, 1><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        case 2: spmv_pAp<VT, 2><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        case 4: spmv_pAp<VT, 4><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        case 8: spmv_pAp<VT, 8><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        case 16: spmv_pAp<VT, 16><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        case 32: spmv_pAp<VT, 32><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        default: spmv_pAp<VT, 64><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+    }
+    check_launch("spmv_pAp");
+}
+
+void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
+                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s) {
+    if (val32) spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, st, part, counter, want_dot, s);
+    else spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, st, part, counter, want_dot, s);
+}
+
+void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,
+                         hipStream_t s) {
+    scatter_init<<<vec_blocks(n), kBlock, 0, s>>>(n, perm, b, x0, r, x);
+    check_launch("scatter_init");
+}
+
+void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
+    norm2_b<<<vec_blocks(n), kBlock, 0, s>>>(n, b, st, part, counter);
+    check_launch("norm2_b");
+}
+
+void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, unsigned* counter,
+                          hipStream_t s) {
+    init_residual<<<vec_blocks(n), kBlock, 0, s>>>(n, r, ax0, st, part, counter);
+    check_launch("init_residual");
+}
+
+void launch_update_xr(vec_prec p, int n, int m, double* x, const double* pv, double* r, const double* ap, void* y,
+                      void* z, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64)
+        update_xr<double><<<g, kBlock, 0, s>>>(n, m, x, pv, r, ap, static_cast<double*>(y), static_cast<double*>(z), st, part, counter);
+    else
+        update_xr<float><<<g, kBlock, 0, s>>>(n, m, x, pv, r, ap, static_cast<float*>(y), static_cast<float*>(z), st, part, counter);
+    check_launch("update_xr");
+}
+
+void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part,
+                       unsigned* counter, bool first, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) dot_center<double><<<g, kBlock, 0, s>>>(n, r, static_cast<const double*>(out), st, part, counter, first ? 1 : 0);
+    else dot_center<float><<<g, kBlock, 0, s>>>(n, r, static_cast<const float*>(out), st, part, counter, first ? 1 : 0);
+    check_launch("dot_center");
+}
+
+void launch_update_p(vec_prec p, int n, double* pv, const void* out, const pcg_state* st, bool first, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) update_p<double><<<g, kBlock, 0, s>>>(n, pv, static_cast<const double*>(out), st, first ? 1 : 0);
+    else update_p<float><<<g, kBlock, 0, s>>>(n, pv, static_cast<const float*>(out), st, first ? 1 : 0);
+    check_launch("update_p");
+}
+
+void launch_sum_vec(int n, const double* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
+    sum_vec<<<vec_blocks(n), kBlock, 0, s>>>(n, a, st, part, counter);
+    check_launch("sum_vec");
+}
+
+void launch_gather_shift(int n, const unsigned* perm, const double* src, double* dst, const pcg_state* st,
+                         bool use_mean, hipStream_t s) {
+    gather_shift<<<vec_blocks(n), kBlock, 0, s>>>(n, perm, src, dst, st, use_mean ? 1 : 0);
+    check_launch("gather_shift");
+}
+
+void launch_sum_shift(int n, const double* r, pcg_state* st, double* part, unsigned* counter, bool center,
+                      hipStream_t s) {
+    sum_shift<<<vec_blocks(n), kBlock, 0, s>>>(n, r, st, part, counter, center ? 1 : 0);
+    check_launch("sum_shift");
+}
+
+void launch_apply_scatter(vec_prec p, int n, int m, const unsigned* perm, const double* r, double* rp, void* y,
+                          void* z, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) apply_scatter<double><<<g, kBlock, 0, s>>>(n, m, perm, r, rp, static_cast<double*>(y), static_cast<double*>(z));
+    else apply_scatter<float><<<g, kBlock, 0, s>>>(n, m, perm, r, rp, static_cast<float*>(y), static_cast<float*>(z));
+    check_launch("apply_scatter");
+}
+
+void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) sum_out<double><<<g, kBlock, 0, s>>>(n, static_cast<const double*>(a), st, part, counter);
+    else sum_out<float><<<g, kBlock, 0, s>>>(n, static_cast<const float*>(a), st, part, counter);
+    check_launch("sum_out");
+}
+
+void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* out, double* z, const pcg_state* st,
+                         bool center, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) apply_gather<double><<<g, kBlock, 0, s>>>(n, perm, static_cast<const double*>(out), z, st, center ? 1 : 0);
+    else apply_gather<float><<<g, kBlock, 0, s>>>(n, perm, static_cast<const float*>(out), z, st, center ? 1 : 0);
+    check_launch("apply_gather");
+}
+
+void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s) {
+    const int g = vec_blocks(n);
+    if (p == vec_prec::fp64) fill_pending<double><<<g, kBlock, 0, s>>>(static_cast<double*>(a), n, nullptr);
+    else fill_pending<float><<<g, kBlock, 0, s>>>(static_cast<float*>(a), n, nullptr);
+    check_launch("fill_pending");
+}
+
+}  // namespace apxchol_amd

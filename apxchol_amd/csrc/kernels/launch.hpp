@@ -1,0 +1,50 @@
+// Host-callable launchers of the gfx950 kernels (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../device.hpp"
+
+namespace apxchol_amd {
+
+enum class vec_prec { fp64 = 0, fp32 = 1 };
+
+// Persistent grid (workgroups) of the dataflow sweep for this device.
+int sweep_dataflow_grid(vec_prec p, bool fp16);
+
+// One sweep. rhs: fp64 r (forward) or the forward output (back, fp64 or fp32
+// as `p`); out: fp64 or fp32 as `p`, every slot pending on entry.
+// level_batches: per level, the first batch (levels+1 entries) for the
+// level-set schedule; null selects the dataflow schedule.
+void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
+                  int* err, const int* level_batches, int nlevels, int dataflow_grid, hipStream_t s);
+
+void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s);
+
+void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
+                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s);
+
+void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,
+                         hipStream_t s);
+void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
+void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, unsigned* counter,
+                          hipStream_t s);
+void launch_update_xr(vec_prec p, int n, int m, double* x, const double* pv, double* r, const double* ap, void* y,
+                      void* z, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
+void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part,
+                       unsigned* counter, bool first, hipStream_t s);
+void launch_update_p(vec_prec p, int n, double* pv, const void* out, const pcg_state* st, bool first, hipStream_t s);
+void launch_sum_vec(int n, const double* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
+void launch_gather_shift(int n, const unsigned* perm, const double* src, double* dst, const pcg_state* st,
+                         bool use_mean, hipStream_t s);
+
+// Stand-alone application pieces.
+void launch_sum_shift(int n, const double* r, pcg_state* st, double* part, unsigned* counter, bool center,
+                      hipStream_t s);
+void launch_apply_scatter(vec_prec p, int n, int m, const unsigned* perm, const double* r, double* rp, void* y,
+                          void* z, hipStream_t s);
+void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
+void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* out, double* z, const pcg_state* st,
+                         bool center, hipStream_t s);
+void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s);
+
+}  // namespace apxchol_amd

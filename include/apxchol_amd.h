@@ -1,0 +1,138 @@
+/* apxchol_amd -- C ABI of the MI355X (gfx950) solve path of apxchol.
+ *
+ * Plain C types only: host arrays are host pointers, "d_" arguments are
+ * device pointers (HIP device memory of the current device), `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream). No call throws;
+ * every call returns an apx_status and apx_last_error() holds the message of
+ * the last failure on the calling thread.
+ *
+ * Each entry point names the reference interface it replaces (paths are in
+ * the reference tree AlgOptGroup/apxchol).
+ */
+#ifndef APXCHOL_AMD_H
+#define APXCHOL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    APX_OK = 0,
+    APX_ERR_INVALID = 1, /* std::invalid_argument in the reference (bad input / misuse) */
+    APX_ERR_RUNTIME = 2, /* std::runtime_error in the reference (device / numerical failure) */
+    APX_ERR_MEMORY = 3
+} apx_status;
+
+typedef struct apx_factor_s* apx_factor_t; /* host factorization (factorization struct) */
+typedef struct apx_solver_s* apx_solver_t; /* factor + operator resident on one GPU */
+
+/* solve_options / factor_options (include/apxchol/solver/solve.h:13-25,
+ * include/apxchol/solver/factor_options.h:56-156) plus the device knobs. */
+typedef struct {
+    double tol;              /* default 1e-8 */
+    int32_t max_iter;        /* default 200 */
+    int32_t stagnation_window; /* default 50; 0 disables */
+    uint32_t seed;           /* factor seed (reference default 42) */
+    int32_t precision;       /* sweep vectors: 0 = fp64, 1 = fp32 (factor values are fp32 or fp16) */
+    int32_t fp16_storage;    /* 1 = per-column-scaled fp16 factor values */
+    int32_t schedule;        /* 0 = sync-free dataflow sweep, 1 = level-set launches */
+    double drop_rel;         /* < 0: APXCHOL_FACTOR_DROP or 1e-4; 0 disables the drop */
+    int32_t keep_factor;     /* keep the host factor values for apx_solver_export_factor */
+    int32_t iters_per_check; /* PCG iterations enqueued between completion tests (default 4) */
+} apx_options;
+
+/* Fill `o` with the defaults above. */
+void apx_options_default(apx_options* o);
+
+/* solve_result (solve.h:27-42). status: 1 converged, 2 pAp <= 0 breakdown,
+ * 3 stagnation, 4 max_iter reached, 5 zero right-hand side. */
+typedef struct {
+    int64_t iterations;
+    double residual; /* recurrence residual ||r|| / ||b|| */
+    int64_t lumped_offdiag;
+    double solve_vram_mb;
+    int32_t status;
+} apx_solve_info;
+
+typedef struct {
+    int64_t n, m;
+    int32_t sddm;
+    uint64_t factor_nnz, stored_nnz, offdiag, dropped;
+    int64_t rounds, levels, fwd_batches, bck_batches, op_nnz;
+    int32_t op_fp32, spmv_lanes, dataflow_grid, fp16_storage;
+    uint64_t device_bytes;
+} apx_solver_info;
+
+const char* apx_last_error(void);
+
+/* ---- host factorization -------------------------------------------------
+ * apxchol::factorize(L, storage, opts, cp) (include/apxchol/solver/factorization.h:156-159,
+ * src/factorization.cpp:125-175) with vec_pool storage and block_greedy, at one
+ * thread. A: n x n symmetric Laplacian / SDDM M-matrix in CSC with BOTH
+ * triangles stored (what Eigen::SparseMatrix<double> holds). */
+apx_status apx_factorize(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
+                         uint32_t seed, apx_factor_t* out);
+/* out[0..6] = n, nnz, rounds, sddm, lumped_offdiag, 0, 0 */
+apx_status apx_factor_info(apx_factor_t f, int64_t* out);
+/* factorization::L (CSC, diagonal first; outer n+1, inner/vals nnz), perm (n),
+ * per-round IS sizes (rounds) -- factorization.h:22-56. */
+apx_status apx_factor_export(apx_factor_t f, uint64_t* outer, uint32_t* inner, float* vals, uint32_t* perm,
+                             uint64_t* round_is);
+void apx_factor_destroy(apx_factor_t f);
+
+/* ---- host SpTRSV preparation (no GPU) ------------------------------------
+ * omp_sptrsv::setup / cuda_sptrsv::setup host part (include/apxchol/solver/sptrsv/omp.h:505-861,
+ * cuda.h:233-459, cuda_host.h:62-232): L11, compacting drop, fp16 narrowing,
+ * the forward (rows of L) and back (columns of L) off-diagonal lists and the
+ * round-as-level boundaries. sizes[0..5] = m, offdiag, levels, nnz_factor, nnz_stored, fp16. */
+apx_status apx_trsv_sizes(apx_factor_t f, int32_t fp16, double drop_rel, int64_t* sizes);
+apx_status apx_trsv_export(apx_factor_t f, int32_t fp16, double drop_rel, uint32_t* f_ptr, int32_t* f_idx,
+                           float* f_val, uint16_t* f_h, uint32_t* b_ptr, int32_t* b_idx, float* b_val, uint16_t* b_h,
+                           float* diag, float* inv_scale, uint32_t* level_ptr);
+
+/* ---- GPU solver ----------------------------------------------------------
+ * cpu_solver(L, opts) / cpu_solver(L, F, opts) (solve.h:66-76, src/solve.cpp:277-318):
+ * factorize (or adopt `f`, which is not consumed), prepare and upload. */
+apx_status apx_solver_create(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
+                             const apx_options* opts, apx_solver_t* out);
+apx_status apx_solver_create_with_factor(int64_t n, const int64_t* colptr, const int32_t* rowidx,
+                                         const double* vals, apx_factor_t f, const apx_options* opts,
+                                         apx_solver_t* out);
+void apx_solver_destroy(apx_solver_t s);
+apx_status apx_solver_info_get(apx_solver_t s, apx_solver_info* info);
+
+/* cpu_solver::solve(b, x, tol, max_iter, x0) (solve.h:83-100, src/solve.cpp:478-654),
+ * on device vectors of n doubles. tol < 0 / max_iter < 0 use the options;
+ * d_x0 may be NULL. The PCG runs on the GPU (cuda_pcg::solve, pcg_cuda.h:157-261). */
+apx_status apx_solver_solve(apx_solver_t s, const double* d_b, double* d_x, const double* d_x0, double tol,
+                            int32_t max_iter, apx_solve_info* info, void* stream);
+/* Host-vector convenience form of apx_solver_solve (b, x, x0 host arrays of n). */
+apx_status apx_solver_solve_host(apx_solver_t s, const double* b, double* x, const double* x0, double tol,
+                                 int32_t max_iter, apx_solve_info* info);
+
+/* cpu_solver::apply(r) = apx_cholesky::_solve_impl (solve.h:103, preconditioner.h:344-353):
+ * z = M^-1 r with the center-k schedule (every K-th application centres). */
+apx_status apx_solver_apply(apx_solver_t s, const double* d_r, double* d_z, void* stream);
+apx_status apx_solver_apply_host(apx_solver_t s, const double* r, double* z);
+/* apx_cholesky::reset_apply_count (preconditioner.h:327) */
+apx_status apx_solver_reset_apply_count(apx_solver_t s);
+
+/* One triangular sweep in factor order with fp64 vectors: forward L y = x
+ * (backward = 0) or back L^T z = y (backward = 1) on the stored L11 -- the
+ * device boundary of dataflow_solve (include/apxchol/solver/sptrsv/cuda_dataflow.h:185-208)
+ * and omp_sptrsv::forward_solve / transpose_solve (omp.h:1048-1058). d_rhs, d_out: m doubles. */
+apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rhs, double* d_out, void* stream);
+
+/* Host copies of the factor kept with keep_factor = 1 (factorization::L, perm, rounds). */
+apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* inner, float* vals, uint32_t* perm);
+
+/* apxchol::solve(L, b, opts) one-shot (solve.h:136-138, src/solve.cpp:681-737), host vectors. */
+apx_status apx_solve(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, const double* b,
+                     double* x, const apx_options* opts, apx_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APXCHOL_AMD_H */

@@ -1,0 +1,178 @@
+"""GPU parity of the HIP solve path against the CPU oracle (one-thread
+restatement of the reference's cpu_solver), through the C ABI.
+
+Bars (stated per test): sweeps and apply with fp64 vectors within 1e-12
+relative of the oracle and bit-identical run to run and across schedules;
+PCG to the same tolerance within +-2 iterations of the oracle; fp32/fp16 sweeps
+within their storage's precision and converging to 1e-8."""
+import numpy as np
+import pytest
+
+import apxchol_amd as ax
+import oracle
+from apxchol_amd import matgen as mg
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "grid2d_64": lambda: mg.grid_laplacian((64, 64)),
+    "grid2d_256": lambda: mg.grid_laplacian((256, 256)),          # fat levels (> 1024 rows)
+    "sddm3d_16": lambda: mg.add_excess(mg.grid_laplacian((16, 16, 16), "lognormal")),
+    "aniso3d_16": lambda: mg.grid_laplacian((16, 16, 16), "lognormal", 0.5, seed=5, axis_weights=(1, 10, 100)),
+    "rmat_12": lambda: mg.rmat_laplacian(12),
+    "rmat_16": lambda: mg.rmat_laplacian(16),                     # BK residual + 500-column peel chain
+}
+_cache = {}
+
+
+def case(name):
+    if name not in _cache:
+        A = CASES[name]()
+        F = ax.factorize(A, seed=0)
+        _cache[name] = (A, F, F.export())
+    return _cache[name]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("schedule", ["dataflow", "levels"])
+def test_sweeps_fp64_match_oracle(gpu, name, schedule):
+    import torch
+    A, F, fac = case(name)
+    print(f"[case] sweeps {name} {schedule}", flush=True)
+    s = ax.Solver(A, factor=F, schedule=schedule)
+    o = oracle.Oracle(A, fac, F.sddm)
+    m = s.info["m"]
+    x = np.random.default_rng(7).standard_normal(m)
+    xd = torch.tensor(x, device=gpu)
+    yd = torch.empty(m, dtype=torch.float64, device=gpu)
+    zd = torch.empty(m, dtype=torch.float64, device=gpu)
+    s.sweep(xd, yd, backward=False)
+    s.sweep(yd, zd, backward=True)
+    torch.cuda.synchronize()
+    y_ref = o.forward(x)
+    z_ref = o.back(y_ref)
+    y, z = yd.cpu().numpy(), zd.cpu().numpy()
+    assert rel(y, y_ref) < 1e-12
+    assert rel(z, o.back(y)) < 1e-12
+    assert rel(z, z_ref) < 1e-11
+    # Run to run: bit-identical.
+    y2 = torch.empty_like(yd)
+    s.sweep(xd, y2, backward=False)
+    torch.cuda.synchronize()
+    assert torch.equal(y2.view(torch.int64), yd.view(torch.int64))
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["grid2d_64", "sddm3d_16", "rmat_16"])
+def test_schedules_bit_identical(gpu, name):
+    import torch
+    A, F, fac = case(name)
+    print(f"[case] schedules {name}", flush=True)
+    outs = []
+    for schedule in ("dataflow", "levels"):
+        s = ax.Solver(A, factor=F, schedule=schedule)
+        m = s.info["m"]
+        x = torch.tensor(np.random.default_rng(3).standard_normal(m), device=gpu)
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        s.sweep(x, y, backward=False)
+        s.sweep(y, z, backward=True)
+        torch.cuda.synchronize()
+        outs.append(z.cpu().numpy().view(np.int64).copy())
+        s.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "aniso3d_16", "rmat_12"])
+def test_apply_matches_cpu_apply(gpu, name):
+    """cpu_solver::apply parity over 12 consecutive applications (the center-k
+    schedule centres the 10th): 1e-12 relative."""
+    A, F, fac = case(name)
+    print(f"[case] apply {name}", flush=True)
+    s = ax.Solver(A, factor=F)
+    o = oracle.Oracle(A, fac, F.sddm)
+    rg = np.random.default_rng(11)
+    for k in range(12):
+        r = rg.standard_normal(A.shape[0])
+        z = s.apply(r)
+        zr = o.apply(r)
+        assert rel(z, zr) < 1e-12, f"application {k + 1}"
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "aniso3d_16", "rmat_12", "rmat_16"])
+def test_pcg_fp64_matches_oracle(gpu, name):
+    A, F, fac = case(name)
+    print(f"[case] pcg {name}", flush=True)
+    b = np.random.default_rng(0).standard_normal(A.shape[0])
+    if not F.sddm:
+        b = mg.project_range(A, b)
+    s = ax.Solver(A, factor=F)
+    res = s.solve(b, tol=1e-8)
+    o = oracle.Oracle(A, fac, F.sddm)
+    xo, ito, reso = o.solve(b, tol=1e-8)
+    true_res = np.linalg.norm(b - A @ res.x) / np.linalg.norm(b)
+    assert res.status == 1, res
+    assert abs(res.iterations - ito) <= 2, (res.iterations, ito)
+    assert true_res < 2e-8
+    assert rel(res.x, xo) < 1e-4, rel(res.x, xo)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["grid2d_256", "sddm3d_16", "rmat_16"])
+def test_pcg_mixed_precision_converges(gpu, name):
+    """FP32 sweeps (fp32 vectors, fp32 accumulation) with FP64 CG reach 1e-8
+    within a few iterations of the FP64-sweep solve."""
+    A, F, fac = case(name)
+    print(f"[case] mixed {name}", flush=True)
+    b = np.random.default_rng(0).standard_normal(A.shape[0])
+    if not F.sddm:
+        b = mg.project_range(A, b)
+    s64 = ax.Solver(A, factor=F)
+    s32 = ax.Solver(A, factor=F, precision="fp32")
+    r64 = s64.solve(b, tol=1e-8)
+    r32 = s32.solve(b, tol=1e-8)
+    assert r32.status == 1
+    assert np.linalg.norm(b - A @ r32.x) / np.linalg.norm(b) < 2e-8
+    assert r32.iterations <= r64.iterations + 3
+    s64.close()
+    s32.close()
+
+
+@pytest.mark.parametrize("name", ["grid2d_64", "sddm3d_16", "rmat_12"])
+def test_fp16_storage_apply_matches_oracle(gpu, name):
+    """fp16 per-column-scaled storage, fp64 vectors: the stored factor is the
+    oracle's bit for bit, so the apply agrees to fp64 rounding."""
+    A, F, fac = case(name)
+    print(f"[case] fp16 {name}", flush=True)
+    s = ax.Solver(A, factor=F, fp16_storage=True)
+    o = oracle.Oracle(A, fac, F.sddm, fp16=True)
+    r = np.random.default_rng(5).standard_normal(A.shape[0])
+    assert rel(s.apply(r), o.apply(r)) < 1e-12
+    b = r if F.sddm else mg.project_range(A, r)
+    res = s.solve(b, tol=1e-8)
+    assert res.status == 1 and np.linalg.norm(b - A @ res.x) / np.linalg.norm(b) < 2e-8
+    s.close()
+
+
+def test_solve_device_vectors_and_warm_start(gpu):
+    import torch
+    A, F, fac = case("sddm3d_16")
+    s = ax.Solver(A, factor=F)
+    b = np.random.default_rng(2).standard_normal(A.shape[0])
+    bd = torch.tensor(b, device=gpu)
+    r1 = s.solve(bd)
+    x1 = r1.x.cpu().numpy()
+    assert r1.status == 1
+    # Warm start from the solution: converged at iteration 0.
+    r2 = s.solve(bd, x0=r1.x)
+    assert r2.iterations == 0 and r2.status == 1
+    # Repeated solves are bit-identical.
+    r3 = s.solve(bd)
+    assert np.array_equal(x1.view(np.int64), r3.x.cpu().numpy().view(np.int64))
+    assert r3.iterations == r1.iterations
+    s.close()
