@@ -41,7 +41,8 @@ class ApxInvalid(ApxError, ValueError):
 class Options(ctypes.Structure):
     _fields_ = [("tol", F64), ("max_iter", I32), ("stagnation_window", I32), ("seed", U32),
                 ("precision", I32), ("fp16_storage", I32), ("schedule", I32), ("drop_rel", F64),
-                ("keep_factor", I32), ("iters_per_check", I32)]
+                ("keep_factor", I32), ("iters_per_check", I32), ("sweep_chunk", I32), ("sweep_blocks", I32),
+                ("tail_rows", I32)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -65,7 +66,7 @@ _SIGS = {
     "apx_factor_export": ([P, P, P, P, P, P], ctypes.c_int),
     "apx_factor_destroy": ([P], None),
     "apx_trsv_sizes": ([P, I32, F64, P], ctypes.c_int),
-    "apx_trsv_export": ([P, I32, F64] + [P] * 13, ctypes.c_int),
+    "apx_trsv_export": ([P, I32, F64] + [P] * 11, ctypes.c_int),
     "apx_solver_create": ([I64, P, P, P, P, P], ctypes.c_int),
     "apx_solver_create_with_factor": ([I64, P, P, P, P, P, P], ctypes.c_int),
     "apx_solver_destroy": ([P], None),
@@ -77,6 +78,8 @@ _SIGS = {
     "apx_solver_reset_apply_count": ([P], ctypes.c_int),
     "apx_solver_sweep": ([P, I32, P, P, P], ctypes.c_int),
     "apx_solver_export_factor": ([P, P, P, P, P], ctypes.c_int),
+    "apx_solver_set_timing": ([P, I32], ctypes.c_int),
+    "apx_solver_take_timing": ([P, P, P], ctypes.c_int),
     "apx_solve": ([I64, P, P, P, P, P, P, P], ctypes.c_int),
 }
 
@@ -138,7 +141,8 @@ def _csc(A):
 
 
 def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
-             schedule="dataflow", drop_rel=-1.0, keep_factor=False, iters_per_check=4) -> Options:
+             schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=4, sweep_chunk=0,
+             sweep_blocks=0, tail_rows=0) -> Options:
     o = Options()
     load_library().apx_options_default(ctypes.byref(o))
     o.tol = float(tol)
@@ -147,10 +151,13 @@ def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp
     o.seed = int(seed)
     o.precision = {"fp64": 0, "fp32": 1}[precision]
     o.fp16_storage = 1 if fp16_storage else 0
-    o.schedule = {"dataflow": 0, "levels": 1}[schedule]
+    o.schedule = {"dataflow": 0, "levels": 1, "hybrid": 2}[schedule]
     o.drop_rel = float(drop_rel)
     o.keep_factor = 1 if keep_factor else 0
     o.iters_per_check = int(iters_per_check)
+    o.sweep_chunk = int(sweep_chunk)
+    o.sweep_blocks = int(sweep_blocks)
+    o.tail_rows = int(tail_rows)
     return o
 
 
@@ -293,6 +300,19 @@ class Solver:
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(lib.apx_solver_apply(self._h, _ptr(r), _ptr(z), s))
         return z
+
+    def set_timing(self, on: bool = True):
+        """HIP-event timing of kernel classes (forward sweep, back sweep, SpMV,
+        vector passes) on the launching stream."""
+        _check(load_library().apx_solver_set_timing(self._h, int(on)))
+
+    def take_timing(self):
+        """{class: (total_ms, launches)} since the last call (then reset)."""
+        ms = np.zeros(4)
+        cnt = np.zeros(4, np.int64)
+        _check(load_library().apx_solver_take_timing(self._h, ms.ctypes.data, cnt.ctypes.data))
+        names = ("fwd_sweep", "bck_sweep", "spmv", "vector")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(names)}
 
     def reset_apply_count(self):
         _check(load_library().apx_solver_reset_apply_count(self._h))

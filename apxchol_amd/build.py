@@ -3,7 +3,9 @@
 Host C++ (factorization, SpTRSV prep, solver orchestration, C ABI) is compiled
 with amdclang++ under ISO floating-point contraction rules (-ffp-contract=off:
 the factor is a bit-exact function of the input); the HIP kernels with hipcc
-for gfx950 only, packed-fp32 instructions disabled.
+for gfx950 only, packed-fp32 instructions disabled and no implicit FMA
+contraction (the kernels write every fused multiply-add out, so the same
+arithmetic runs whatever the schedule).
 
     python -m apxchol_amd.build        # or apxchol_amd.build.build()
 """
@@ -26,7 +28,7 @@ KERNEL_SOURCES = ["kernels/kernels.hip"]
 
 HOST_FLAGS = ["-std=c++20", "-O3", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
               f"-I{ROCM}/include", "-Wall", "-Wno-unused-result"]
-HIP_FLAGS = ["-std=c++20", "-O3", "-fPIC", f"--offload-arch={ARCH}",
+HIP_FLAGS = ["-std=c++20", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
 

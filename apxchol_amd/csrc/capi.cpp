@@ -66,13 +66,17 @@ solve_options to_opts(const apx_options* o) {
     so.stagnation_window = o->stagnation_window;
     so.factor_opts.seed = o->seed;
     if (o->precision != 0 && o->precision != 1) throw std::invalid_argument("precision must be 0 (fp64) or 1 (fp32)");
-    if (o->schedule != 0 && o->schedule != 1) throw std::invalid_argument("schedule must be 0 (dataflow) or 1 (levels)");
+    if (o->schedule < 0 || o->schedule > 2)
+        throw std::invalid_argument("schedule must be 0 (dataflow), 1 (levels) or 2 (hybrid)");
     so.precision = o->precision ? sweep_precision::fp32 : sweep_precision::fp64;
     so.fp16_storage = o->fp16_storage != 0;
-    so.schedule = o->schedule ? sweep_schedule::levels : sweep_schedule::dataflow;
+    so.schedule = static_cast<sweep_schedule>(o->schedule);
+    so.tail_rows = o->tail_rows;
     so.drop_rel = o->drop_rel;
     so.keep_factor_values = o->keep_factor != 0;
     so.iters_per_check = o->iters_per_check > 0 ? o->iters_per_check : 4;
+    so.sweep_chunk = o->sweep_chunk;
+    so.sweep_blocks = o->sweep_blocks;
     return so;
 }
 
@@ -101,10 +105,13 @@ void apx_options_default(apx_options* o) {
     o->seed = 42;
     o->precision = 0;
     o->fp16_storage = 0;
-    o->schedule = 0;
+    o->schedule = 2;
     o->drop_rel = -1.0;
     o->keep_factor = 0;
     o->iters_per_check = 4;
+    o->sweep_chunk = 0;
+    o->sweep_blocks = 0;
+    o->tail_rows = 0;
 }
 
 const char* apx_last_error(void) { return g_err.c_str(); }
@@ -281,6 +288,22 @@ apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rh
         need(d_rhs, "rhs");
         need(d_out, "out");
         s->s->sweep_device(backward != 0, d_rhs, d_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_solver_set_timing(apx_solver_t s, int32_t on) {
+    return guarded([&] {
+        need(s, "solver");
+        s->s->set_timing(on != 0);
+    });
+}
+
+apx_status apx_solver_take_timing(apx_solver_t s, double* ms, int64_t* launches) {
+    return guarded([&] {
+        need(s, "solver");
+        need(ms, "ms");
+        need(launches, "launches");
+        s->s->take_timing(ms, launches);
     });
 }
 

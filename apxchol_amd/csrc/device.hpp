@@ -42,6 +42,7 @@ struct sweep_desc {
     const std::uint32_t* batch_pos = nullptr;  // nb+1
     const std::uint64_t* batch_mask = nullptr; // nb
     int nb = 0;
+    int chunk = 1;                          // batches claimed per ticket (dataflow schedule)
     int* ctrl = nullptr;                    // 2 ints: ticket, finished waves (zero between sweeps)
 };
 

@@ -61,6 +61,13 @@ int lanes_for_host(std::uint32_t len) {
     return p > 64 ? 64 : static_cast<int>(p);
 }
 
+int env_int(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    if (!e || !*e) return dflt;
+    const int v = std::atoi(e);
+    return v > 0 ? v : dflt;
+}
+
 int center_k_from_env() {
     int k = 10;
     if (const char* e = std::getenv("APXCHOL_CENTER_K"); e && *e) {
@@ -187,6 +194,9 @@ struct gpu_solver::device_state {
     dbuf f_ptr, f_ent, f_idx, f_h, b_ptr, b_ent, b_idx, b_h, diag, inv_scale;
     dbuf f_pos, f_mask, b_pos, b_mask;
     std::vector<int> f_levels, b_levels;
+    dbuf f_levels_dev, b_levels_dev, partial;
+    std::vector<int> f_seg[5], b_seg[5];   // hybrid segments: l0, l1, tail flag, p0, p1
+    sweep_sched f_sched, b_sched;
     dbuf perm, op_ptr, op_idx, op_val;
     bool op32 = false;
     int lanes = 8;
@@ -200,6 +210,60 @@ struct gpu_solver::device_state {
         if (host_err) (void)hipHostFree(host_err);
     }
 };
+
+// HIP-event timing of kernel classes. Events are pooled and reused; each
+// record carries the PCG step it belongs to, so launches after convergence
+// (which return at once) can be dropped when the solve ends.
+struct gpu_solver::timer {
+    bool on = false;
+    struct rec { int cls; int step; hipEvent_t a, b; };
+    std::vector<hipEvent_t> pool;
+    std::size_t used = 0;
+    std::vector<rec> recs;
+    double ms[4] = {0, 0, 0, 0};
+    std::int64_t launches[4] = {0, 0, 0, 0};
+    ~timer() { for (auto e : pool) (void)hipEventDestroy(e); }
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    template <class F>
+    void time(int cls, int step, hipStream_t s, F&& launch) {
+        if (!on) { launch(); return; }
+        rec r{cls, step, get(), get()};
+        hip_check(hipEventRecord(r.a, s), "hipEventRecord");
+        launch();
+        hip_check(hipEventRecord(r.b, s), "hipEventRecord");
+        recs.push_back(r);
+    }
+    // Fold the records of a finished (synchronized) solve: keep steps <= last.
+    void collect(int last_step) {
+        for (const auto& r : recs) {
+            if (r.step > last_step) continue;
+            float t = 0.0f;
+            hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
+            ms[r.cls] += t;
+            launches[r.cls] += 1;
+        }
+        recs.clear();
+        used = 0;
+    }
+};
+
+void gpu_solver::set_timing(bool on) const { timer_->on = on; }
+
+void gpu_solver::take_timing(double* ms, std::int64_t* launches) const {
+    for (int k = 0; k < 4; ++k) {
+        ms[k] = timer_->ms[k];
+        launches[k] = timer_->launches[k];
+        timer_->ms[k] = 0.0;
+        timer_->launches[k] = 0;
+    }
+}
 
 gpu_solver::gpu_solver(const csc_view& L, const solve_options& opts) : opts_(opts) {
     F_ = factorize(L, opts_.factor_opts);
@@ -218,6 +282,7 @@ gpu_solver::gpu_solver(const csc_view& L, factorization F, const solve_options& 
 gpu_solver::~gpu_solver() = default;
 
 void gpu_solver::setup(const csc_view& L) {
+    timer_ = std::make_unique<timer>();
     n_ = L.n;
     if (n_ > (1ll << 31) - 2) throw std::invalid_argument("gpu_solver: n exceeds the 32-bit device index range");
     center_k_ = center_k_from_env();
@@ -239,16 +304,21 @@ void gpu_solver::setup(const csc_view& L) {
     // interleaved with their ids (entry32), fp16 values in their own array.
     d.f_ptr.upload(T_.f_ptr);
     d.b_ptr.upload(T_.b_ptr);
+    // Entry arrays hold at least one element: lanes without an entry read
+    // slot 0 (and discard it) so their loads need no branch.
     if (T_.fp16) {
-        d.f_idx.upload(T_.f_idx);
-        d.f_h.upload(T_.f_h);
-        d.b_idx.upload(T_.b_idx);
-        d.b_h.upload(T_.b_h);
+        auto pad_i = [](std::vector<std::int32_t> v) { if (v.empty()) v.push_back(0); return v; };
+        auto pad_h = [](std::vector<std::uint16_t> v) { if (v.empty()) v.push_back(0); return v; };
+        d.f_idx.upload(pad_i(T_.f_idx));
+        d.f_h.upload(pad_h(T_.f_h));
+        d.b_idx.upload(pad_i(T_.b_idx));
+        d.b_h.upload(pad_h(T_.b_h));
         d.inv_scale.upload(T_.inv_scale);
     } else {
-        std::vector<entry32> fe(T_.f_idx.size()), be(T_.b_idx.size());
-        for (std::size_t i = 0; i < fe.size(); ++i) fe[i] = {T_.f_idx[i], T_.f_val[i]};
-        for (std::size_t i = 0; i < be.size(); ++i) be[i] = {T_.b_idx[i], T_.b_val[i]};
+        std::vector<entry32> fe(std::max<std::size_t>(T_.f_idx.size(), 1), entry32{0, 0.0f}),
+            be(std::max<std::size_t>(T_.b_idx.size(), 1), entry32{0, 0.0f});
+        for (std::size_t i = 0; i < T_.f_idx.size(); ++i) fe[i] = {T_.f_idx[i], T_.f_val[i]};
+        for (std::size_t i = 0; i < T_.b_idx.size(); ++i) be[i] = {T_.b_idx[i], T_.b_val[i]};
         d.f_ent.upload(fe);
         d.b_ent.upload(be);
     }
@@ -316,11 +386,70 @@ void gpu_solver::setup(const csc_view& L) {
         s.batch_pos = (backward ? d.b_pos : d.f_pos).as<std::uint32_t>();
         s.batch_mask = (backward ? d.b_mask : d.f_mask).as<std::uint64_t>();
         s.nb = static_cast<int>(bt.mask.size());
+        s.chunk = opts_.sweep_chunk > 0 ? opts_.sweep_chunk : env_int("APXCHOL_AMD_SWEEP_CHUNK", 8);
         s.ctrl = d.ctrl.as<int>() + (backward ? 2 : 0);
     };
     fill_desc(d.fwd, false, fb);
     fill_desc(d.bck, true, bb);
-    d.grid = sweep_dataflow_grid(d.prec, d.fp16);
+    // Workgroups of the dataflow launch (and cap of a level launch). The
+    // dataflow default is 2 workgroups per CU: at the full resident grid (8
+    // per CU) its polling traffic grew superlinearly and two 128^3 sweeps
+    // exceeded the spin limit (reported, not hung), so it is kept well below.
+    d.grid = opts_.sweep_blocks > 0 ? opts_.sweep_blocks : env_int("APXCHOL_AMD_SWEEP_BLOCKS", 0);
+    if (d.grid <= 0) d.grid = std::max(1, sweep_dataflow_grid(d.prec, d.fp16) / 4);
+    // Hybrid segments: a level of at least tail_rows rows is launched on its
+    // own; every maximal run of smaller levels becomes one tail segment.
+    const int tail_rows = std::min(opts_.tail_rows > 0 ? opts_.tail_rows : env_int("APXCHOL_AMD_TAIL_ROWS", 1024),
+                                   sweep_tail_max_rows(d.prec));
+    // A tail run's outputs live in LDS: a run longer than that holds is split.
+    const int tail_cap = sweep_tail_max_rows(d.prec);
+    auto make_sched = [&](sweep_sched& sc, const batch_table& bt, const std::vector<int>& lf, dbuf& lf_dev,
+                          std::vector<int> (&seg)[5]) {
+        lf_dev.upload(lf);
+        const int L = static_cast<int>(lf.size()) - 1;
+        const auto pos = [&](int q) { return static_cast<int>(bt.pos[lf[q]]); };
+        const auto rows = [&](int q) { return pos(q + 1) - pos(q); };
+        for (int l = 0; l < L;) {
+            if (rows(l) >= tail_rows) {
+                seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
+                seg[3].push_back(pos(l)); seg[4].push_back(pos(l + 1));
+                ++l;
+                continue;
+            }
+            int e = l;
+            while (e < L && rows(e) < tail_rows && pos(e + 1) - pos(l) <= tail_cap) ++e;
+            if (e == l) ++e;   // a single small level always fits (rows < tail_rows <= cap)
+            seg[0].push_back(l); seg[1].push_back(e); seg[2].push_back(1);
+            seg[3].push_back(pos(l)); seg[4].push_back(pos(e));
+            l = e;
+        }
+        sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
+                                   : opts_.schedule == sweep_schedule::levels ? 1 : 2);
+        sc.level_first = lf.data();
+        sc.level_first_dev = lf_dev.as<int>();
+        sc.nlevels = L;
+        sc.seg_l0 = seg[0].data();
+        sc.seg_l1 = seg[1].data();
+        sc.seg_tail = seg[2].data();
+        sc.seg_p0 = seg[3].data();
+        sc.seg_p1 = seg[4].data();
+        sc.nseg = static_cast<int>(seg[0].size());
+        sc.grid = d.grid;
+    };
+    make_sched(d.f_sched, fb, d.f_levels, d.f_levels_dev, d.f_seg);
+    make_sched(d.b_sched, bb, d.b_levels, d.b_levels_dev, d.b_seg);
+    // Coupling buffer: one value per lane of the largest tail run's batches.
+    std::size_t max_tail_batches = 1;
+    for (const auto* sc : {&d.f_sched, &d.b_sched})
+        for (int k = 0; k < sc->nseg; ++k)
+            if (sc->seg_tail[k])
+                max_tail_batches = std::max<std::size_t>(
+                    max_tail_batches, static_cast<std::size_t>(sc->level_first[sc->seg_l1[k]] - sc->level_first[sc->seg_l0[k]]));
+    d.partial.alloc(max_tail_batches * kWave * sizeof(double));
+    for (auto* sc : {&d.f_sched, &d.b_sched}) {
+        sc->partial = d.partial.get();
+        sc->partial_bytes = d.partial.bytes();
+    }
 
     stats_.n = n_;
     stats_.m = m;
@@ -394,26 +523,34 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     } else {
         launch_init_residual(n, d.r.as<double>(), nullptr, st, part, cnt, s);
     }
-    const int* fl = opts_.schedule == sweep_schedule::levels ? d.f_levels.data() : nullptr;
-    const int* bl = opts_.schedule == sweep_schedule::levels ? d.b_levels.data() : nullptr;
-    const int fnl = static_cast<int>(d.f_levels.size()) - 1, bnl = static_cast<int>(d.b_levels.size()) - 1;
-    auto precondition = [&](bool first) {
+    timer& tm = *timer_;
+    // Step k: the preconditioner application after iteration k (k = 0 is the
+    // one before the loop); the SpMV / update of iteration k carry step k - 1,
+    // so a solve that stops at iteration `it` keeps steps 0 .. it-1.
+    auto precondition = [&](bool first, int step) {
         if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
-        launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, fl, fnl, d.grid, s);
-        launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, bl, bnl, d.grid, s);
-        launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
-        launch_update_p(pr, n, d.p.as<double>(), d.z.get(), st, first, s);
+        tm.time(0, step, s, [&] { launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, d.f_sched, s); });
+        tm.time(1, step, s, [&] { launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s); });
+        tm.time(3, step, s, [&] {
+            launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
+            launch_update_p(pr, n, d.p.as<double>(), d.z.get(), st, first, s);
+        });
     };
-    precondition(true);
+    precondition(true, 0);
     const int J = std::max(1, opts_.iters_per_check);
     int enqueued = 0;
     for (;;) {
         for (int j = 0; j < J; ++j) {
-            launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
-                        d.p.as<double>(), d.ap.as<double>(), st, part, cnt, true, s);
-            launch_update_xr(pr, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
-                             d.y.get(), d.z.get(), st, part, cnt, s);
-            precondition(false);
+            const int it = enqueued + j + 1;
+            tm.time(2, it - 1, s, [&] {
+                launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
+                            d.p.as<double>(), d.ap.as<double>(), st, part, cnt, true, s);
+            });
+            tm.time(3, it - 1, s, [&] {
+                launch_update_xr(pr, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
+                                 d.y.get(), d.z.get(), st, part, cnt, s);
+            });
+            precondition(false, it);
         }
         enqueued += J;
         hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
@@ -425,6 +562,7 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
     hip_check(hipStreamSynchronize(s), "solve");
     if (h.status == 5) hip_check(hipMemsetAsync(d_x, 0, sizeof(double) * n, s), "zero x");
+    tm.collect(h.iter - 1);
     check_sweep_error();
     res.iterations = h.iter;
     res.residual = h.residual;
@@ -461,12 +599,10 @@ void gpu_solver::apply_device(const double* d_r, double* d_z, hipStream_t s) con
     double* part = d.part.as<double>();
     unsigned* cnt = d.counter.as<unsigned>();
     int* err = d.err.as<int>();
-    const int* fl = opts_.schedule == sweep_schedule::levels ? d.f_levels.data() : nullptr;
-    const int* bl = opts_.schedule == sweep_schedule::levels ? d.b_levels.data() : nullptr;
     launch_sum_shift(n, d_r, st, part, cnt, center, s);
     launch_apply_scatter(d.prec, n, m, d.perm.as<unsigned>(), d_r, d.rp.as<double>(), d.y.get(), d.z.get(), s);
-    launch_sweep(d.fwd, d.prec, d.fp16, d.rp.get(), d.y.get(), st, err, fl, static_cast<int>(d.f_levels.size()) - 1, d.grid, s);
-    launch_sweep(d.bck, d.prec, d.fp16, d.y.get(), d.z.get(), st, err, bl, static_cast<int>(d.b_levels.size()) - 1, d.grid, s);
+    launch_sweep(d.fwd, d.prec, d.fp16, d.rp.get(), d.y.get(), st, err, d.f_sched, s);
+    launch_sweep(d.bck, d.prec, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s);
     if (center) launch_sum_out(d.prec, n, d.z.get(), st, part, cnt, s);
     launch_apply_gather(d.prec, n, d.perm.as<unsigned>(), d.z.get(), d_z, st, center, s);
 }
@@ -488,10 +624,8 @@ void gpu_solver::sweep_device(bool backward, const double* d_rhs, double* d_out,
     const int m = static_cast<int>(T_.m);
     if (m == 0) return;
     launch_fill_pending(vec_prec::fp64, d_out, m, s);
-    const bool lv = opts_.schedule == sweep_schedule::levels;
-    const auto& lvl = backward ? d.b_levels : d.f_levels;
     launch_sweep(backward ? d.bck : d.fwd, vec_prec::fp64, d.fp16, d_rhs, d_out, nullptr, d.err.as<int>(),
-                 lv ? lvl.data() : nullptr, static_cast<int>(lvl.size()) - 1, d.grid, s);
+                 backward ? d.b_sched : d.f_sched, s);
 }
 
 solve_result solve(const csc_view& L, const std::vector<double>& b, const solve_options& opts) {

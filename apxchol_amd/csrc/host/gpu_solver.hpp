@@ -14,7 +14,7 @@
 namespace apxchol_amd {
 
 enum class sweep_precision { fp64 = 0, fp32 = 1 };   // vector / accumulation width of the two sweeps
-enum class sweep_schedule { dataflow = 0, levels = 1 };
+enum class sweep_schedule { dataflow = 0, levels = 1, hybrid = 2 };
 
 // solve.h:13-25 plus the device knobs.
 struct solve_options {
@@ -25,9 +25,12 @@ struct solve_options {
     factor_options factor_opts;
     sweep_precision precision = sweep_precision::fp64;
     bool fp16_storage = false;
-    sweep_schedule schedule = sweep_schedule::dataflow;
+    sweep_schedule schedule = sweep_schedule::hybrid;
+    int tail_rows = 0;            // hybrid: levels below this many rows go to the one-workgroup tail (0: default)
     double drop_rel = -1.0;       // < 0: APXCHOL_FACTOR_DROP / the 1e-4 default
     int iters_per_check = 4;      // iterations enqueued between completion tests
+    int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 8)
+    int sweep_blocks = 0;         // dataflow grid (0: APXCHOL_AMD_SWEEP_BLOCKS or the resident grid)
 };
 
 // solve.h:27-42. status: 0 running/none, 1 converged, 2 pAp <= 0 breakdown,
@@ -74,6 +77,14 @@ public:
     void sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const;
 
     void reset_apply_count() const { apply_count_ = 0; }
+
+    // Per-kernel-class timing with HIP events on the launching stream
+    // (classes: 0 forward sweep, 1 back sweep, 2 SpMV, 3 vector passes).
+    // Only launches that did work are counted: those of iterations after
+    // convergence return at once and are excluded.
+    void set_timing(bool on) const;
+    // ms[4], launches[4] accumulated since the last call; resets them.
+    void take_timing(double* ms, std::int64_t* launches) const;
     const factorization& factor() const { return F_; }
     const trsv_factor& trsv() const { return T_; }
     const solver_stats& stats() const { return stats_; }
@@ -81,6 +92,7 @@ public:
 
 private:
     struct device_state;
+    struct timer;
     void setup(const csc_view& L);
     void check_sweep_error() const;
 
@@ -92,6 +104,7 @@ private:
     int center_k_ = 10;
     mutable std::uint64_t apply_count_ = 0;
     std::unique_ptr<device_state> dev_;
+    std::unique_ptr<timer> timer_;
 };
 
 // One-shot solve (solve.h:136-138).

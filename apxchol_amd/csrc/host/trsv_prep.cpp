@@ -163,7 +163,10 @@ trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel) {
         for (node_t j = 0; j < m; ++j) T.diag[j] = c_val[c_ptr[j]];
     }
 
-    // Back sweep: the columns without their diagonal.
+    // Back sweep: the columns without their diagonal, rows DESCENDING -- i.e.
+    // ascending sweep position (the back sweep walks rows m-1 .. 0), as the
+    // forward rows are: in both directions a row's entries run from its
+    // earliest dependency to its latest.
     const std::uint64_t off = c_row.size() - m;
     T.b_ptr.assign(static_cast<std::size_t>(m) + 1, 0);
     T.b_idx.resize(off);
@@ -171,7 +174,7 @@ trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel) {
     {
         std::uint64_t o = 0;
         for (node_t j = 0; j < m; ++j) {
-            for (std::uint64_t p = c_ptr[j] + 1; p < c_ptr[j + 1]; ++p, ++o) {
+            for (std::uint64_t p = c_ptr[j + 1]; p-- > c_ptr[j] + 1; ++o) {
                 T.b_idx[o] = static_cast<std::int32_t>(c_row[p]);
                 if (fp16) T.b_h[o] = c_h[p]; else T.b_val[o] = c_val[p];
             }

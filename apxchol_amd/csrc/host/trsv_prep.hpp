@@ -26,7 +26,7 @@ struct drop_stats {
 // column are dropped, solver/preconditioner.h:228-231) after the drop, as the
 // two sweeps read it. Row lists exclude the diagonal, which sits in `diag`.
 //   forward sweep (L y = x):   row i of L,   entries (j < i, L_ij), j ascending
-//   back sweep    (L^T z = y): column j of L, entries (i > j, L_ij), i ascending
+//   back sweep    (L^T z = y): column j of L, entries (i > j, L_ij), i descending
 // fp16 storage (per-column scaled, lowprec.h / cuda_host.h:16-34): values are
 // fp16(L_ij / s_j), diag[j] = fp32(L_jj / s_j) + the column's rounding
 // residual, inv_scale[j] = fp32(1 / s_j); fp32 storage: values L_ij, diag

@@ -31,23 +31,52 @@ int dataflow_grid_for() {
 }
 
 template <class TI, class TO, bool F16>
-void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* st, int* err, const int* lb, int nl,
-             int grid, hipStream_t s) {
+void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* st, int* err, const sweep_sched& sc,
+             hipStream_t s) {
     const TI* in = static_cast<const TI*>(rhs);
     TO* o = static_cast<TO*>(out);
-    if (!lb) {
-        sweep_dataflow<TI, TO, F16><<<grid, kBlock, 0, s>>>(d, in, o, st, grid * kWavesPerBlock, err);
-        check_launch("sweep_dataflow");
-        return;
-    }
-    for (int l = 0; l < nl; ++l) {
-        const int b0 = lb[l], b1 = lb[l + 1];
-        if (b1 <= b0) continue;
+    auto level_launch = [&](int l) {
+        const int b0 = sc.level_first[l], b1 = sc.level_first[l + 1];
+        if (b1 <= b0) return;
         int g = (b1 - b0 + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (g > grid) g = grid;
+        if (g > sc.grid) g = sc.grid;
         sweep_levels<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b0, b1, in, o, st, err);
+    };
+    if (sc.kind == 0) {
+        sweep_dataflow<TI, TO, F16><<<sc.grid, kBlock, 0, s>>>(d, in, o, st, sc.grid * kWavesPerBlock, err);
+        check_launch("sweep_dataflow");
+    } else if (sc.kind == 1) {
+        for (int l = 0; l < sc.nlevels; ++l) level_launch(l);
+        check_launch("sweep_levels");
+    } else {
+        for (int k = 0; k < sc.nseg; ++k) {
+            if (sc.seg_tail[k]) {
+                const int p0 = sc.seg_p0[k], p1 = sc.seg_p1[k];
+                const size_t lds = static_cast<size_t>(p1 - p0) * sizeof(TO);
+                if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: tail run exceeds LDS");
+                static const bool attr = [] {
+                    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TI, TO, F16>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
+                    return true;
+                }();
+                (void)attr;
+                const int bf = sc.level_first[sc.seg_l0[k]], bl = sc.level_first[sc.seg_l1[k]];
+                TO* part = nullptr;
+                if (p0 > 0) {
+                    if (static_cast<size_t>(bl - bf) * kWave * sizeof(TO) > sc.partial_bytes)
+                        throw std::runtime_error("apxchol_amd: coupling buffer too small");
+                    part = static_cast<TO*>(sc.partial);
+                    const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
+                    sweep_couple<TO, F16><<<g, kBlock, 0, s>>>(d, bf, bl, p0, o, part, st);
+                }
+                sweep_tail<TI, TO, F16><<<1, kTailBlock, lds, s>>>(d, sc.level_first_dev, sc.seg_l0[k],
+                                                                  sc.seg_l1[k], p0, in, o, part, st, err);
+            } else {
+                for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
+            }
+        }
+        check_launch("sweep_hybrid");
     }
-    check_launch("sweep_levels");
 }
 
 }  // namespace
@@ -57,18 +86,22 @@ int sweep_dataflow_grid(vec_prec p, bool fp16) {
     return fp16 ? dataflow_grid_for<float, float, true>() : dataflow_grid_for<float, float, false>();
 }
 
+int sweep_tail_max_rows(vec_prec p) {
+    return kTailLdsBytes / static_cast<int>(p == vec_prec::fp64 ? sizeof(double) : sizeof(float));
+}
+
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
-                  int* err, const int* lb, int nl, int grid, hipStream_t s) {
+                  int* err, const sweep_sched& sc, hipStream_t s) {
     if (d.nb == 0) return;
     if (p == vec_prec::fp64) {
-        if (fp16) sweep_t<double, double, true>(d, rhs, out, st, err, lb, nl, grid, s);
-        else sweep_t<double, double, false>(d, rhs, out, st, err, lb, nl, grid, s);
+        if (fp16) sweep_t<double, double, true>(d, rhs, out, st, err, sc, s);
+        else sweep_t<double, double, false>(d, rhs, out, st, err, sc, s);
     } else if (!d.backward) {
-        if (fp16) sweep_t<double, float, true>(d, rhs, out, st, err, lb, nl, grid, s);
-        else sweep_t<double, float, false>(d, rhs, out, st, err, lb, nl, grid, s);
+        if (fp16) sweep_t<double, float, true>(d, rhs, out, st, err, sc, s);
+        else sweep_t<double, float, false>(d, rhs, out, st, err, sc, s);
     } else {
-        if (fp16) sweep_t<float, float, true>(d, rhs, out, st, err, lb, nl, grid, s);
-        else sweep_t<float, float, false>(d, rhs, out, st, err, lb, nl, grid, s);
+        if (fp16) sweep_t<float, float, true>(d, rhs, out, st, err, sc, s);
+        else sweep_t<float, float, false>(d, rhs, out, st, err, sc, s);
     }
 }
 

@@ -11,12 +11,36 @@ enum class vec_prec { fp64 = 0, fp32 = 1 };
 // Persistent grid (workgroups) of the dataflow sweep for this device.
 int sweep_dataflow_grid(vec_prec p, bool fp16);
 
+// Rows a one-workgroup tail run may hold (its outputs live in LDS).
+int sweep_tail_max_rows(vec_prec p);
+
+// How one sweep direction is scheduled.
+//   dataflow: one persistent launch of `grid` workgroups (polling);
+//   levels:   one launch per level;
+//   hybrid:   per segment, either one launch of a single big level
+//             (tail == 0) or one single-workgroup launch walking a run of
+//             small levels (tail == 1).
+struct sweep_sched {
+    int kind = 0;                          // 0 dataflow, 1 levels, 2 hybrid
+    const int* level_first = nullptr;      // host: first batch of each level (levels+1)
+    const int* level_first_dev = nullptr;  // device copy (tail kernel)
+    int nlevels = 0;
+    const int* seg_l0 = nullptr;           // hybrid segments (host)
+    const int* seg_l1 = nullptr;
+    const int* seg_tail = nullptr;
+    const int* seg_p0 = nullptr;           // tail segments: sweep positions [p0, p1)
+    const int* seg_p1 = nullptr;
+    int nseg = 0;
+    int grid = 0;                          // dataflow grid / cap of a level launch
+    void* partial = nullptr;               // coupling buffer of the tail runs (device)
+    size_t partial_bytes = 0;
+};
+
 // One sweep. rhs: fp64 r (forward) or the forward output (back, fp64 or fp32
-// as `p`); out: fp64 or fp32 as `p`, every slot pending on entry.
-// level_batches: per level, the first batch (levels+1 entries) for the
-// level-set schedule; null selects the dataflow schedule.
+// as `p`); out: fp64 or fp32 as `p` (every slot pending on entry for the
+// dataflow schedule).
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
-                  int* err, const int* level_batches, int nlevels, int dataflow_grid, hipStream_t s);
+                  int* err, const sweep_sched& sc, hipStream_t s);
 
 void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s);
 

@@ -30,12 +30,12 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
     for (int row = tid / G; row < n; row += stride) {
         const unsigned b = ptr[row], e = ptr[row + 1];
         double s = 0.0;
-        for (unsigned p = b + sub; p < e; p += G) s += static_cast<double>(val[p]) * x[idx[p]];
+        for (unsigned p = b + sub; p < e; p += G) s = fma(static_cast<double>(val[p]), x[idx[p]], s);
 #pragma unroll
         for (int off = G / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, G);
         if (sub == 0) {
             y[row] = s;
-            acc += x[row] * s;
+            acc = fma(x[row], s, acc);
         }
     }
     if (!want_dot) return;
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void norm2_b(int n, const double* __restric
                                                   unsigned* counter) {
     __shared__ double sh[kWavesPerBlock * 2];
     double q = 0.0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) q += b[i] * b[i];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) q = fma(b[i], b[i], q);
     double v[1] = {q};
     if (grid_sum<1>(v, part, counter, sh)) st->bnorm = sqrt(v[0]);
 }
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void init_residual(int n, double* __restric
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         double ri = r[i];
         if (ax0) { ri -= ax0[i]; r[i] = ri; }
-        q += ri * ri;
+        q = fma(ri, ri, q);
         s += ri;
     }
     double v[2] = {q, s};
@@ -125,10 +125,10 @@ __global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __rest
     B* zb = reinterpret_cast<B*>(z);
     double q = 0.0, s = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * ap[i];
+        x[i] = fma(alpha, p[i], x[i]);
+        const double ri = fma(-alpha, ap[i], r[i]);
         r[i] = ri;
-        q += ri * ri;
+        q = fma(ri, ri, q);
         s += ri;
         if (i < m) { yb[i] = flag_bits<TO>::pending; zb[i] = flag_bits<TO>::pending; }
         else z[i] = static_cast<TO>(0);
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void dot_center(int n, const double* __rest
     double d = 0.0, s = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const double o = static_cast<double>(out[i]);
-        d += r[i] * o;
+        d = fma(r[i], o, d);
         s += o;
     }
     double v[2] = {d, s};
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void update_p(int n, double* __restrict__ p
     const double mu = st->mu, beta = st->beta;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const double zi = static_cast<double>(out[i]) - mu;
-        p[i] = first ? zi : zi + beta * p[i];
+        p[i] = first ? zi : fma(beta, p[i], zi);
     }
 }
 

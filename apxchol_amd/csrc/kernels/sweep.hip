@@ -1,5 +1,8 @@
 // Triangular sweeps of the preconditioner z = L^-T L^-1 r on gfx950.
 //
+// Compiled with -ffp-contract=off: every fused multiply-add is written out
+// (fma), so the schedules below run literally the same arithmetic per row.
+//
 // One routine, sweep_batch, processes a BATCH of consecutive rows (in sweep
 // order) with one wavefront: every row owns an aligned group of G lanes
 // (G = pow2ceil(len / kEpl), 1..64), each lane keeps its kEpl (index, value)
@@ -58,6 +61,44 @@ __device__ __forceinline__ void load_entry(const sweep_desc& d, unsigned p, int&
     }
 }
 
+// One polling round over a lane's kEpl gathered slots: all loads are issued
+// back to back and waited for once; a slot whose value has arrived leaves the
+// pending mask. A lane's valid entries are a prefix of its slots, and the
+// slots past it repeat the first entry's address, so no lane ever polls an
+// address it does not need (no shared hot spot).
+template <class TO>
+__device__ __forceinline__ void poll(const TO* out, const int (&jj)[kEpl], TO (&xv)[kEpl], unsigned& pend) {
+    typename flag_bits<TO>::type bits[kEpl];
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) bits[k] = load_flag(out + jj[k]);
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) {
+        const bool arrived = ((pend >> k) & 1u) && bits[k] != flag_bits<TO>::pending;
+        if (arrived) xv[k] = from_bits<TO>(bits[k]);
+        pend &= arrived ? ~(1u << k) : ~0u;
+    }
+}
+
+// Gather without waiting: plain (cacheable) loads of complete values; slots
+// outside the lane's valid prefix stay 0 (their address may hold a value not
+// computed yet, which must not reach the sum even times a zero weight).
+template <class TO>
+__device__ __forceinline__ void gather(const TO* out, const int (&jj)[kEpl], TO (&xv)[kEpl], unsigned valid) {
+    TO g[kEpl];
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) xv[k] = ((valid >> k) & 1u) ? g[k] : static_cast<TO>(0);
+}
+
+// Publish a row: an agent-scope atomic store when other waves poll for it,
+// a plain store when readers are ordered behind a launch or barrier.
+template <class TO, bool WAIT>
+__device__ __forceinline__ void publish(TO* p, TO v) {
+    if constexpr (WAIT) store_flag(p, v);
+    else *p = v;
+}
+
 // Right-hand side of `row`: forward reads r - shift (the Laplacian input
 // centring), back reads y scaled by inv_scale^2 on the fp16 storage.
 template <class TI, class AT>
@@ -73,7 +114,11 @@ __device__ __forceinline__ AT row_rhs(const sweep_desc& d, const TI* rhs, int ro
 // Process batch b. Returns false if a dependency never arrived within the
 // spin limit (the row is then written from whatever was gathered and the
 // caller raises the error flag).
-template <class TI, class TO, bool F16>
+// WAIT: poll the gathered slots until they leave the sentinel (dataflow
+// schedule); otherwise every dependency is known to be complete (an earlier
+// launch, or an earlier level behind a workgroup barrier) and one plain load
+// per slot suffices. The arithmetic is the same either way.
+template <class TI, class TO, bool F16, bool WAIT>
 __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ rhs, TO* out, double shift,
                             const int* err) {
     using AT = TO;
@@ -107,29 +152,31 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
 #pragma unroll
             for (int k = 0; k < kEpl; ++k) {
                 const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
-                jj[k] = 0; vv[k] = 0.0f; xv[k] = 0;
-                if (q < len) { load_entry<F16>(d, rb + q, jj[k], vv[k]); pend |= 1u << k; }
+                const bool valid = q < len;
+                load_entry<F16>(d, valid ? rb + q : rb + c0, jj[k], vv[k]);
+                if (!valid) vv[k] = 0.0f;
+                xv[k] = 0;
+                pend |= valid ? (1u << k) : 0u;
             }
-            unsigned spins = 0;
-            for (;;) {
-#pragma unroll
-                for (int k = 0; k < kEpl; ++k)
-                    if (pend & (1u << k)) {
-                        const auto bits = load_flag(out + jj[k]);
-                        if (bits != flag_bits<TO>::pending) { xv[k] = from_bits<TO>(bits); pend &= ~(1u << k); }
-                    }
-                if (__ballot(pend != 0) == 0) break;
-                if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) { ok = false; break; }
-                __builtin_amdgcn_s_sleep(1);
+            if constexpr (WAIT) {
+                unsigned spins = 0;
+                for (;;) {
+                    if (pend) poll<TO>(out, jj, xv, pend);
+                    if (__ballot(pend != 0) == 0) break;
+                    if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) { ok = false; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            } else {
+                gather<TO>(out, jj, xv, pend);
             }
 #pragma unroll
-            for (int k = 0; k < kEpl; ++k) part += static_cast<AT>(vv[k]) * xv[k];
+            for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, kWave);
         if (lane == 0) {
             const AT r = row_rhs<TI, AT>(d, rhs, row, shift);
-            store_flag(out + row, (r - part) / static_cast<AT>(d.diag[row]));
+            publish<TO, WAIT>(out + row, (r - part) / static_cast<AT>(d.diag[row]));
         }
         return ok;
     }
@@ -141,8 +188,11 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
 #pragma unroll
     for (int k = 0; k < kEpl; ++k) {
         const unsigned q = static_cast<unsigned>(sub + k * G);
-        jj[k] = 0; vv[k] = 0.0f; xv[k] = 0;
-        if (active && q < len) { load_entry<F16>(d, rb + q, jj[k], vv[k]); pend |= 1u << k; }
+        const bool valid = active && q < len;
+        load_entry<F16>(d, valid ? rb + q : (active && len > 0 ? rb : 0u), jj[k], vv[k]);
+        if (!valid) vv[k] = 0.0f;
+        xv[k] = 0;
+        pend |= valid ? (1u << k) : 0u;
     }
     AT r = 0;
     AT dg = 1;
@@ -150,36 +200,43 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         r = row_rhs<TI, AT>(d, rhs, row, shift);
         dg = static_cast<AT>(d.diag[row]);
     }
+    if constexpr (!WAIT) {
+        // Dependencies complete: gather once, then the same per-row sum.
+        if (active) gather<TO>(out, jj, xv, pend);
+        AT part = 0;
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const AT o = __shfl_xor(part, off, kWave);
+            if (off < G) part += o;
+        }
+        if (active && sub == 0) publish<TO, WAIT>(out + row, (r - part) / dg);
+        return true;
+    }
     bool done = !active;
     unsigned spins = 0;
     for (;;) {
-        if (!done) {
-#pragma unroll
-            for (int k = 0; k < kEpl; ++k)
-                if (pend & (1u << k)) {
-                    const auto bits = load_flag(out + jj[k]);
-                    if (bits != flag_bits<TO>::pending) { xv[k] = from_bits<TO>(bits); pend &= ~(1u << k); }
-                }
-        }
+        if (pend) poll<TO>(out, jj, xv, pend);
         const unsigned long long ready = __ballot(pend == 0);
         const bool go = !done && (ready & gmask) == gmask;
         AT part = 0;
         if (go)
 #pragma unroll
-            for (int k = 0; k < kEpl; ++k) part += static_cast<AT>(vv[k]) * xv[k];
+            for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             const AT o = __shfl_xor(part, off, kWave);
             if (off < G) part += o;
         }
         if (go) {
-            if (sub == 0) store_flag(out + row, (r - part) / dg);
+            if (sub == 0) publish<TO, WAIT>(out + row, (r - part) / dg);
             done = true;
         }
         if (__ballot(!done) == 0) break;
         if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) {
             ok = false;
-            if (!done && sub == 0) store_flag(out + row, static_cast<AT>(0));
+            if (!done && sub == 0) publish<TO, WAIT>(out + row, static_cast<AT>(0));
             break;
         }
         if (!go) __builtin_amdgcn_s_sleep(1);
@@ -194,14 +251,20 @@ __global__ __launch_bounds__(kBlock) void sweep_dataflow(sweep_desc d, const TI*
     const double shift = (st && !d.backward) ? st->shift : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     bool ok = true;
+    // A wave claims d.chunk consecutive batches per ticket and runs them in
+    // order, so it only ever waits on batches it already finished or on
+    // smaller tickets held by running waves.
     for (;;) {
-        int b = 0;
-        if (lane == 0) b = atomicAdd(&d.ctrl[0], 1);
-        b = __shfl(b, 0, kWave);
-        if (b >= d.nb) break;
-        const bool bok = sweep_batch<TI, TO, F16>(d, b, rhs, out, shift, err);
-        if (!bok && lane == 0) atomicOr(err, 1);
-        ok = ok && bok;
+        int b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&d.ctrl[0], d.chunk);
+        b0 = __shfl(b0, 0, kWave);
+        if (b0 >= d.nb) break;
+        const int b1 = min(b0 + d.chunk, d.nb);
+        for (int b = b0; b < b1; ++b) {
+            const bool bok = sweep_batch<TI, TO, F16, true>(d, b, rhs, out, shift, err);
+            if (!bok && lane == 0) atomicOr(err, 1);
+            ok = ok && bok;
+        }
     }
     if (lane == 0) {
         const int prev = atomicAdd(&d.ctrl[1], 1);
@@ -220,8 +283,198 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
     const int waves = static_cast<int>(gridDim.x) * kWavesPerBlock;
     const int w = static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
     bool ok = true;
-    for (int b = b0 + w; b < b1; b += waves) ok = sweep_batch<TI, TO, F16>(d, b, rhs, out, shift, err) && ok;
-    if ((threadIdx.x & (kWave - 1)) == 0 && !ok) atomicOr(err, 1);
+    for (int b = b0 + w; b < b1; b += waves) sweep_batch<TI, TO, F16, false>(d, b, rhs, out, shift, err);
+}
+
+// A run of small levels [l0, l1) -- sweep positions [p0, p1) -- walked by ONE
+// workgroup of kTailBlock threads. The run's outputs are kept in LDS as well as
+// written out, so a dependency inside the run is an LDS read after the level
+// barrier; a dependency before the run is complete when the kernel starts.
+// Each wave prefetches its next batch (entries, right-hand side, diagonal and
+// every dependency from before the run) before the barrier that precedes it,
+// so a level costs one LDS gather, the row sums and one barrier. The per-row
+// arithmetic is sweep_batch's (same slot order, fma chain and butterfly), so
+// the result is bit-identical to the other schedules.
+inline constexpr int kTailBlock = 1024;
+inline constexpr int kTailLdsBytes = 160 * 1024;
+
+template <class TO>
+struct tail_batch {
+    int b = -1;
+    int row = 0, pos = 0, G = 1, sub = 0, start = 0;
+    unsigned len = 0, rb = 0;
+    bool active = false;
+    unsigned lds = 0;  // slots inside the run (jj = LDS index); other slots are in `part0`
+    int jj[kEpl];
+    float vv[kEpl];
+    TO part0 = 0;      // the lane's chain over its entries before the run (coupling kernel)
+    TO r = 0, dg = 1;
+};
+
+// Lane -> row mapping of batch b (as in sweep_batch).
+struct lane_row {
+    int row, pos, G, sub;
+    unsigned len, rb;
+    bool has_row, active;
+};
+__device__ __forceinline__ lane_row map_lane(const sweep_desc& d, int b) {
+    const int lane = threadIdx.x & (kWave - 1);
+    lane_row x;
+    const unsigned q0 = d.batch_pos[b], q1 = d.batch_pos[b + 1];
+    const unsigned long long mask = d.batch_mask[b];
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long mu = mask & upto;
+    const int rib = __popcll(mu) - 1;
+    x.sub = lane - (63 - __clzll(mu));
+    const unsigned pos = q0 + static_cast<unsigned>(rib);
+    x.has_row = pos < q1;
+    x.pos = static_cast<int>(pos);
+    x.row = x.has_row ? (d.backward ? d.m - 1 - static_cast<int>(pos) : static_cast<int>(pos)) : 0;
+    x.rb = x.has_row ? d.ptr[x.row] : 0u;
+    x.len = x.has_row ? d.ptr[x.row + 1] - x.rb : 0u;
+    x.G = lanes_for(x.len);
+    x.active = x.has_row && x.sub < x.G;
+    return x;
+}
+
+// Coupling of a run: for every lane of every batch of the run, its fma chain
+// over the entries that point before the run (complete when this kernel runs),
+// in sweep_batch's slot order. Entries of a row are in ascending sweep
+// position, so these are a prefix of every lane's chain.
+template <class TO, bool F16>
+__global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0, const TO* out,
+                                                       TO* partial, const pcg_state* st) {
+    if (st && st->done) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int b = bfirst + static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    if (b >= blast) return;
+    const lane_row x = map_lane(d, b);
+    TO part = 0;
+    const bool longrow = x.len > static_cast<unsigned>(kWave * kEpl);
+    const int G = longrow ? kWave : x.G;
+    const int sub = longrow ? lane : x.sub;
+    const bool act = longrow ? true : x.active;
+    for (unsigned c0 = 0; c0 < (longrow ? x.len : 1u); c0 += kWave * kEpl) {
+        int jj[kEpl];
+        float vv[kEpl];
+        TO g[kEpl];
+        unsigned use = 0;
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) {
+            const unsigned q = (longrow ? c0 : 0u) + static_cast<unsigned>(sub + k * G);
+            const bool v = act && q < x.len;
+            load_entry<F16>(d, v ? x.rb + q : (x.len > 0 ? x.rb : 0u), jj[k], vv[k]);
+            const int pj = d.backward ? d.m - 1 - jj[k] : jj[k];
+            use |= (v && pj < p0) ? (1u << k) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k)
+            if ((use >> k) & 1u) part = fma(static_cast<TO>(vv[k]), g[k], part);
+    }
+    partial[static_cast<size_t>(b - bfirst) * kWave + lane] = part;
+}
+
+template <class TI, class TO, bool F16>
+__device__ __forceinline__ void tail_load(const sweep_desc& d, int b, int bfirst, int p0, const TI* __restrict__ rhs,
+                                          const TO* partial, double shift, tail_batch<TO>& t) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const lane_row x = map_lane(d, b);
+    t.b = b;
+    t.row = x.row; t.pos = x.pos; t.G = x.G; t.sub = x.sub; t.len = x.len; t.rb = x.rb; t.active = x.active;
+    t.lds = 0;
+    t.part0 = partial ? partial[static_cast<size_t>(b - bfirst) * kWave + lane] : static_cast<TO>(0);
+    if (t.active && t.sub == 0) {
+        t.r = row_rhs<TI, TO>(d, rhs, t.row, shift);
+        t.dg = static_cast<TO>(d.diag[t.row]);
+    }
+    if (t.len > static_cast<unsigned>(kWave * kEpl)) return;   // long row: walked in tail_finish
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) {
+        const unsigned q = static_cast<unsigned>(t.sub + k * t.G);
+        const bool v = t.active && q < t.len;
+        load_entry<F16>(d, v ? t.rb + q : (t.active && t.len > 0 ? t.rb : 0u), t.jj[k], t.vv[k]);
+        const int pj = d.backward ? d.m - 1 - t.jj[k] : t.jj[k];
+        const bool in_run = v && pj >= p0;
+        t.lds |= in_run ? (1u << k) : 0u;
+        t.jj[k] = in_run ? pj - p0 : 0;
+    }
+}
+
+template <class TI, class TO, bool F16>
+__device__ __forceinline__ void tail_finish(const sweep_desc& d, int p0, TO* out, TO* lds, tail_batch<TO>& t) {
+    using AT = TO;
+    const int lane = threadIdx.x & (kWave - 1);
+    AT part = t.part0;
+    if (t.len > static_cast<unsigned>(kWave * kEpl)) {
+        // Long row alone in its batch: chunks of 64 * kEpl entries; only the
+        // entries inside the run remain (the rest are in part0).
+        for (unsigned c0 = 0; c0 < t.len; c0 += kWave * kEpl) {
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) {
+                const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
+                const bool v = q < t.len;
+                int j;
+                float w;
+                load_entry<F16>(d, v ? t.rb + q : t.rb + c0, j, w);
+                const int pj = d.backward ? d.m - 1 - j : j;
+                const bool in_run = v && pj >= p0;
+                const AT xv = lds[in_run ? pj - p0 : 0];
+                if (in_run) part = fma(static_cast<AT>(w), xv, part);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, kWave);
+        if (lane == 0) {
+            const AT v = (t.r - part) / t.dg;
+            out[t.row] = v;
+            lds[t.pos - p0] = v;
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k)
+        if ((t.lds >> k) & 1u) part = fma(static_cast<AT>(t.vv[k]), lds[t.jj[k]], part);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const AT o = __shfl_xor(part, off, kWave);
+        if (off < t.G) part += o;
+    }
+    if (t.active && t.sub == 0) {
+        const AT v = (t.r - part) / t.dg;
+        out[t.row] = v;
+        lds[t.pos - p0] = v;
+    }
+}
+
+template <class TI, class TO, bool F16>
+__global__ __launch_bounds__(kTailBlock) void sweep_tail(sweep_desc d, const int* __restrict__ level_first, int l0,
+                                                         int l1, int p0, const TI* __restrict__ rhs, TO* out,
+                                                         const TO* partial, const pcg_state* st, int* err) {
+    if (st && st->done) return;
+    extern __shared__ unsigned char tail_lds[];
+    TO* lds = reinterpret_cast<TO*>(tail_lds);
+    const double shift = (st && !d.backward) ? st->shift : 0.0;
+    constexpr int waves = kTailBlock / kWave;
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    const int bfirst = level_first[l0];
+    tail_batch<TO> t;
+    if (bfirst + w < level_first[l0 + 1]) tail_load<TI, TO, F16>(d, bfirst + w, bfirst, p0, rhs, partial, shift, t);
+    for (int l = l0; l < l1; ++l) {
+        const int b0 = level_first[l], b1 = level_first[l + 1];
+        for (int b = b0 + w; b < b1; b += waves) {
+            if (t.b != b) tail_load<TI, TO, F16>(d, b, bfirst, p0, rhs, partial, shift, t);
+            tail_finish<TI, TO, F16>(d, p0, out, lds, t);
+            t.b = -1;
+            if (b + waves < b1) tail_load<TI, TO, F16>(d, b + waves, bfirst, p0, rhs, partial, shift, t);
+        }
+        if (l + 1 < l1) {
+            const int nb = level_first[l + 1] + w;
+            if (nb < level_first[l + 2] && t.b != nb) tail_load<TI, TO, F16>(d, nb, bfirst, p0, rhs, partial, shift, t);
+        }
+        __syncthreads();
+    }
 }
 
 // Fill a vector with the pending sentinel (the state every sweep output must

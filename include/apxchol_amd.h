@@ -37,10 +37,14 @@ typedef struct {
     uint32_t seed;           /* factor seed (reference default 42) */
     int32_t precision;       /* sweep vectors: 0 = fp64, 1 = fp32 (factor values are fp32 or fp16) */
     int32_t fp16_storage;    /* 1 = per-column-scaled fp16 factor values */
-    int32_t schedule;        /* 0 = sync-free dataflow sweep, 1 = level-set launches */
+    int32_t schedule;        /* 0 = sync-free dataflow, 1 = level-set launches, 2 = hybrid (default):
+                                big levels launched, runs of small levels walked by one workgroup */
     double drop_rel;         /* < 0: APXCHOL_FACTOR_DROP or 1e-4; 0 disables the drop */
     int32_t keep_factor;     /* keep the host factor values for apx_solver_export_factor */
     int32_t iters_per_check; /* PCG iterations enqueued between completion tests (default 4) */
+    int32_t sweep_chunk;     /* batches claimed per dataflow ticket (0 = default) */
+    int32_t sweep_blocks;    /* dataflow workgroups / level-launch cap (0 = the device's resident grid) */
+    int32_t tail_rows;       /* hybrid: levels with fewer rows go to the one-workgroup tail (0 = default) */
 } apx_options;
 
 /* Fill `o` with the defaults above. */
@@ -124,6 +128,14 @@ apx_status apx_solver_reset_apply_count(apx_solver_t s);
  * device boundary of dataflow_solve (include/apxchol/solver/sptrsv/cuda_dataflow.h:185-208)
  * and omp_sptrsv::forward_solve / transpose_solve (omp.h:1048-1058). d_rhs, d_out: m doubles. */
 apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rhs, double* d_out, void* stream);
+
+/* Kernel timing with HIP events on the launching stream (no reference
+ * counterpart; the reference's APXCHOL_PCG_TRACE per-component ms/iter,
+ * src/solve.cpp:578-653, is the closest). Classes: 0 forward sweep, 1 back
+ * sweep, 2 SpMV, 3 vector passes; launches of iterations after convergence
+ * are excluded. apx_solver_take_timing returns and resets the totals. */
+apx_status apx_solver_set_timing(apx_solver_t s, int32_t on);
+apx_status apx_solver_take_timing(apx_solver_t s, double* ms /* 4 */, int64_t* launches /* 4 */);
 
 /* Host copies of the factor kept with keep_factor = 1 (factorization::L, perm, rounds). */
 apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* inner, float* vals, uint32_t* perm);

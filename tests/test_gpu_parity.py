@@ -38,7 +38,7 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("schedule", ["dataflow", "levels"])
+@pytest.mark.parametrize("schedule", ["hybrid", "dataflow", "levels"])
 def test_sweeps_fp64_match_oracle(gpu, name, schedule):
     import torch
     A, F, fac = case(name)
@@ -73,7 +73,7 @@ def test_schedules_bit_identical(gpu, name):
     A, F, fac = case(name)
     print(f"[case] schedules {name}", flush=True)
     outs = []
-    for schedule in ("dataflow", "levels"):
+    for schedule in ("hybrid", "dataflow", "levels"):
         s = ax.Solver(A, factor=F, schedule=schedule)
         m = s.info["m"]
         x = torch.tensor(np.random.default_rng(3).standard_normal(m), device=gpu)
@@ -85,6 +85,7 @@ def test_schedules_bit_identical(gpu, name):
         outs.append(z.cpu().numpy().view(np.int64).copy())
         s.close()
     assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
 
 
 @pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "aniso3d_16", "rmat_12"])
