@@ -62,4 +62,47 @@ struct pcg_state {
     int pad_i[2];
 };
 
+// Static layout of the TAIL RUNS: maximal runs of small levels that one
+// workgroup walks with the run's outputs in LDS. Everything a lane needs for a
+// batch of a run sits at an address computed from (batch, lane) alone, so the
+// next batches' loads are independent and stay in flight across the level
+// barriers. Tail batches of a direction are numbered consecutively, run by run.
+//   lane[t*64 + lane]                what the lane does in tail batch t
+//   ent[(t*kTailPairs + k)*64 + lane] its first 2*kTailPairs entries inside
+//                                    the run, in pairs (padding: zero slot, 0)
+//   ovf[t]                           first overflow slot row of batch t
+//   ovf_ent[(ovf[t] + s)*64 + lane]  entry 2*kTailPairs+s of a lane (rows
+//                                    longer than 64 * 2 * kTailPairs)
+// A lane's entries inside the run are the suffix of its sweep_batch chain (in
+// chain order); `idx` is the dependency's position minus the run's first.
+inline constexpr int kTailPairs = 4;
+inline constexpr int kTailBlock = 1024;                 // threads of the tail workgroup
+inline constexpr int kTailWaves = kTailBlock / kWave;   // max batches of a tail level
+struct alignas(16) tail_lane {
+    std::int32_t row;    // output row the lane writes, or -1
+    std::int32_t slot;   // the row's LDS slot (position minus the run's first)
+    std::int32_t gc;     // G | (entries inside the run) << 8
+    float diag;          // the row's diagonal (1 for lanes that do not write)
+};
+struct alignas(16) tail_pair {
+    std::int32_t i0;
+    float v0;
+    std::int32_t i1;
+    float v1;
+};
+struct tail_desc {
+    const tail_lane* lane = nullptr;
+    const tail_pair* ent = nullptr;
+    const std::int32_t* ovf = nullptr;
+    const entry32* ovf_ent = nullptr;
+};
+// Per (tail batch, lane), written by the coupling kernel before a run: the
+// lane's fma chain over the entries before the run, and the row's right-hand
+// side (writer lanes).
+template <class T>
+struct alignas(2 * sizeof(T)) tail_val {
+    T part0;
+    T r;
+};
+
 }  // namespace apxchol_amd

@@ -185,6 +185,123 @@ void validate_sweep_layout(const trsv_factor& T, const batch_table& bt, bool bac
         const std::uint32_t lp0 = backward ? m - T.level_ptr[L - l] : T.level_ptr[l];
         if (bt.level_first[l] < static_cast<int>(nb) && bt.pos[bt.level_first[l]] != lp0) fail("level start");
     }
+    // Every dependency lies in an earlier level (what the level-set and tail
+    // schedules rely on).
+    std::vector<std::uint32_t> level_of(m);
+    for (std::size_t l = 0; l < L; ++l)
+        for (int b = bt.level_first[l]; b < bt.level_first[l + 1]; ++b)
+            for (std::uint32_t t = bt.pos[b]; t < bt.pos[b + 1]; ++t) level_of[t] = static_cast<std::uint32_t>(l);
+    for (std::uint32_t t = 0; t < m; ++t) {
+        const std::uint32_t row = backward ? m - 1 - t : t;
+        for (std::uint32_t p = ptr[row]; p < ptr[row + 1]; ++p) {
+            const std::uint32_t pj = backward ? m - 1 - static_cast<std::uint32_t>(idx[p]) : static_cast<std::uint32_t>(idx[p]);
+            if (level_of[pj] >= level_of[t]) fail("dependency inside its own level");
+        }
+    }
+}
+
+namespace {
+
+// What one lane of batch b does (the kernels' lane mapping).
+struct host_lane {
+    std::uint32_t pos = 0, row = 0, rb = 0, len = 0;
+    int G = 1, sub = 0;
+    bool has_row = false, active = false;
+};
+host_lane map_lane_host(const trsv_factor& T, const batch_table& bt, bool backward, int b, int lane) {
+    const auto& ptr = backward ? T.b_ptr : T.f_ptr;
+    host_lane x;
+    const std::uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const std::uint64_t mu = bt.mask[b] & upto;
+    const int rib = __builtin_popcountll(mu) - 1;
+    x.sub = lane - (63 - __builtin_clzll(mu));
+    x.pos = bt.pos[b] + static_cast<std::uint32_t>(rib);
+    x.has_row = x.pos < bt.pos[b + 1];
+    if (x.has_row) {
+        x.row = backward ? T.m - 1 - x.pos : x.pos;
+        x.rb = ptr[x.row];
+        x.len = ptr[x.row + 1] - x.rb;
+    }
+    x.G = lanes_for_host(x.len);
+    x.active = x.has_row && x.sub < x.G;
+    return x;
+}
+
+}  // namespace
+
+int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl, tail_tables& tt) {
+    const auto& idx = backward ? T.b_idx : T.f_idx;
+    const auto& val = backward ? T.b_val : T.f_val;
+    const auto& hv = backward ? T.b_h : T.f_h;
+    auto fail = [](const std::string& w) { throw std::runtime_error("apxchol_amd: invalid tail run: " + w); };
+    if (bf < 0 || bl <= bf || static_cast<std::size_t>(bl) > bt.mask.size()) fail("batch range");
+    const std::uint32_t p0 = bt.pos[bf];
+    // Padding entries read the zero slot past the run's outputs.
+    const std::int32_t zslot = static_cast<std::int32_t>(bt.pos[bl] - p0);
+    const int base = tt.batches;
+    std::vector<entry32> chain[kWave];
+    for (int b = bf; b < bl; ++b) {
+        int maxc = 0;
+        for (int lane = 0; lane < kWave; ++lane) {
+            const host_lane x = map_lane_host(T, bt, backward, b, lane);
+            const bool longrow = x.len > static_cast<std::uint32_t>(kWave * kEpl);
+            auto& ch = chain[lane];
+            ch.clear();
+            bool inside = false;
+            auto take = [&](std::uint32_t q) {
+                const std::uint32_t p = x.rb + q;
+                const std::uint32_t j = static_cast<std::uint32_t>(idx[p]);
+                const std::uint32_t pj = backward ? T.m - 1 - j : j;
+                if (pj < p0) {
+                    if (inside) fail("a lane's entries before the run follow entries inside it");
+                    return;
+                }
+                if (pj >= x.pos) fail("dependency not before its row");
+                inside = true;
+                ch.push_back({static_cast<std::int32_t>(pj - p0), T.fp16 ? fp16_widen(hv[p]) : val[p]});
+            };
+            if (longrow) {
+                for (std::uint32_t q = static_cast<std::uint32_t>(lane); q < x.len; q += kWave) take(q);
+            } else if (x.active) {
+                for (int k = 0; k < kEpl; ++k) {
+                    const std::uint32_t q = static_cast<std::uint32_t>(x.sub + k * x.G);
+                    if (q < x.len) take(q);
+                }
+            }
+            const bool writer = x.active && x.sub == 0;
+            const int c = static_cast<int>(ch.size());
+            if (c >= (1 << 22)) fail("row too long");
+            tt.lane.push_back({writer ? static_cast<std::int32_t>(x.row) : -1,
+                               writer ? static_cast<std::int32_t>(x.pos - p0) : 0, x.G | (c << 8),
+                               writer ? T.diag[x.row] : 1.0f});
+            maxc = std::max(maxc, c);
+        }
+        for (int k = 0; k < kTailPairs; ++k)
+            for (int lane = 0; lane < kWave; ++lane) {
+                const auto& ch = chain[lane];
+                const std::size_t a = 2 * static_cast<std::size_t>(k), c = a + 1;
+                tail_pair e{zslot, 0.0f, zslot, 0.0f};
+                if (a < ch.size()) { e.i0 = ch[a].idx; e.v0 = ch[a].val; }
+                if (c < ch.size()) { e.i1 = ch[c].idx; e.v1 = ch[c].val; }
+                tt.ent.push_back(e);
+            }
+        constexpr int slots = 2 * kTailPairs;
+        if (maxc > slots) {
+            const std::size_t first = tt.ovf_ent.size() / kWave;
+            if (first > static_cast<std::size_t>(INT32_MAX)) fail("overflow table too large");
+            tt.ovf.push_back(static_cast<std::int32_t>(first));
+            for (int s = 0; s < maxc - slots; ++s)
+                for (int lane = 0; lane < kWave; ++lane) {
+                    const auto& ch = chain[lane];
+                    const std::size_t q = static_cast<std::size_t>(slots + s);
+                    tt.ovf_ent.push_back(q < ch.size() ? ch[q] : entry32{zslot, 0.0f});
+                }
+        } else {
+            tt.ovf.push_back(0);
+        }
+    }
+    tt.batches += bl - bf;
+    return base;
 }
 
 struct gpu_solver::device_state {
@@ -194,8 +311,9 @@ struct gpu_solver::device_state {
     dbuf f_ptr, f_ent, f_idx, f_h, b_ptr, b_ent, b_idx, b_h, diag, inv_scale;
     dbuf f_pos, f_mask, b_pos, b_mask;
     std::vector<int> f_levels, b_levels;
-    dbuf f_levels_dev, b_levels_dev, partial;
-    std::vector<int> f_seg[5], b_seg[5];   // hybrid segments: l0, l1, tail flag, p0, p1
+    dbuf f_levels_dev, b_levels_dev, tail_val;
+    dbuf f_tlane, f_tent, f_tovf, f_tovf_ent, b_tlane, b_tent, b_tovf, b_tovf_ent;
+    std::vector<int> f_seg[6], b_seg[6];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch
     sweep_sched f_sched, b_sched;
     dbuf perm, op_ptr, op_idx, op_val;
     bool op32 = false;
@@ -397,32 +515,52 @@ void gpu_solver::setup(const csc_view& L) {
     // exceeded the spin limit (reported, not hung), so it is kept well below.
     d.grid = opts_.sweep_blocks > 0 ? opts_.sweep_blocks : env_int("APXCHOL_AMD_SWEEP_BLOCKS", 0);
     if (d.grid <= 0) d.grid = std::max(1, sweep_dataflow_grid(d.prec, d.fp16) / 4);
-    // Hybrid segments: a level of at least tail_rows rows is launched on its
-    // own; every maximal run of smaller levels becomes one tail segment.
+    // Hybrid segments: a level with at least tail_rows rows or more batches
+    // than the tail workgroup has waves is launched on its own; every maximal
+    // run of the other (small) levels becomes one tail segment.
     const int tail_rows = std::min(opts_.tail_rows > 0 ? opts_.tail_rows : env_int("APXCHOL_AMD_TAIL_ROWS", 1024),
                                    sweep_tail_max_rows(d.prec));
     // A tail run's outputs live in LDS: a run longer than that holds is split.
     const int tail_cap = sweep_tail_max_rows(d.prec);
-    auto make_sched = [&](sweep_sched& sc, const batch_table& bt, const std::vector<int>& lf, dbuf& lf_dev,
-                          std::vector<int> (&seg)[5]) {
+    auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
+                          dbuf& lf_dev, std::vector<int> (&seg)[6], dbuf* const (&tbuf)[4]) {
         lf_dev.upload(lf);
+        tail_tables tt;
         const int L = static_cast<int>(lf.size()) - 1;
         const auto pos = [&](int q) { return static_cast<int>(bt.pos[lf[q]]); };
         const auto rows = [&](int q) { return pos(q + 1) - pos(q); };
+        // A tail level has at most one batch per wave of the tail workgroup.
+        const auto small = [&](int q) { return rows(q) < tail_rows && lf[q + 1] - lf[q] <= kTailWaves; };
         for (int l = 0; l < L;) {
-            if (rows(l) >= tail_rows) {
+            if (!small(l)) {
                 seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
-                seg[3].push_back(pos(l)); seg[4].push_back(pos(l + 1));
+                seg[3].push_back(pos(l)); seg[4].push_back(pos(l + 1)); seg[5].push_back(-1);
                 ++l;
                 continue;
             }
             int e = l;
-            while (e < L && rows(e) < tail_rows && pos(e + 1) - pos(l) <= tail_cap) ++e;
+            while (e < L && small(e) && pos(e + 1) - pos(l) <= tail_cap) ++e;
             if (e == l) ++e;   // a single small level always fits (rows < tail_rows <= cap)
             seg[0].push_back(l); seg[1].push_back(e); seg[2].push_back(1);
             seg[3].push_back(pos(l)); seg[4].push_back(pos(e));
+            seg[5].push_back(opts_.schedule == sweep_schedule::hybrid ? append_tail_run(T_, bt, backward, lf[l], lf[e], tt)
+                                                                      : -1);
             l = e;
         }
+        const std::size_t tb = static_cast<std::size_t>(tt.batches);
+        if (tt.lane.size() != tb * kWave || tt.ent.size() != tb * kTailPairs * kWave || tt.ovf.size() != tb ||
+            tt.ovf_ent.size() % kWave != 0)
+            throw std::runtime_error("apxchol_amd: inconsistent tail layout");
+        tbuf[0]->upload(tt.lane);
+        tbuf[1]->upload(tt.ent);
+        tbuf[2]->upload(tt.ovf);
+        tbuf[3]->upload(tt.ovf_ent);
+        sc.tail.lane = tbuf[0]->as<tail_lane>();
+        sc.tail.ent = tbuf[1]->as<tail_pair>();
+        sc.tail.ovf = tbuf[2]->as<std::int32_t>();
+        sc.tail.ovf_ent = tbuf[3]->as<entry32>();
+        sc.tail_batches = tt.batches;
+        sc.seg_base = seg[5].data();
         sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
                                    : opts_.schedule == sweep_schedule::levels ? 1 : 2);
         sc.level_first = lf.data();
@@ -436,19 +574,17 @@ void gpu_solver::setup(const csc_view& L) {
         sc.nseg = static_cast<int>(seg[0].size());
         sc.grid = d.grid;
     };
-    make_sched(d.f_sched, fb, d.f_levels, d.f_levels_dev, d.f_seg);
-    make_sched(d.b_sched, bb, d.b_levels, d.b_levels_dev, d.b_seg);
-    // Coupling buffer: one value per lane of the largest tail run's batches.
-    std::size_t max_tail_batches = 1;
-    for (const auto* sc : {&d.f_sched, &d.b_sched})
-        for (int k = 0; k < sc->nseg; ++k)
-            if (sc->seg_tail[k])
-                max_tail_batches = std::max<std::size_t>(
-                    max_tail_batches, static_cast<std::size_t>(sc->level_first[sc->seg_l1[k]] - sc->level_first[sc->seg_l0[k]]));
-    d.partial.alloc(max_tail_batches * kWave * sizeof(double));
+    dbuf* f_tb[4] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent};
+    dbuf* b_tb[4] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent};
+    make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb);
+    make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb);
+    // Coupling output, shared by the two directions (their sweeps never overlap).
+    const std::size_t tail_batches = std::max<std::size_t>(
+        1, static_cast<std::size_t>(std::max(d.f_sched.tail_batches, d.b_sched.tail_batches)));
+    d.tail_val.alloc(tail_batches * kWave * sizeof(tail_val<double>));
     for (auto* sc : {&d.f_sched, &d.b_sched}) {
-        sc->partial = d.partial.get();
-        sc->partial_bytes = d.partial.bytes();
+        sc->tail_val = d.tail_val.get();
+        sc->tail_val_bytes = d.tail_val.bytes();
     }
 
     stats_.n = n_;

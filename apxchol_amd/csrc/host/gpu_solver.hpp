@@ -8,6 +8,7 @@
 #include <memory>
 #include <vector>
 
+#include "../device.hpp"
 #include "factor.hpp"
 #include "trsv_prep.hpp"
 
@@ -118,5 +119,18 @@ struct batch_table {
 };
 batch_table build_batches(const trsv_factor& T, bool backward);
 void validate_sweep_layout(const trsv_factor& T, const batch_table& bt, bool backward);
+
+// Static tail layout of one direction (device.hpp's tail_desc), run by run.
+struct tail_tables {
+    std::vector<tail_lane> lane;
+    std::vector<tail_pair> ent;
+    std::vector<std::int32_t> ovf;
+    std::vector<entry32> ovf_ent;
+    int batches = 0;
+};
+// Appends the run of batches [bf, bl) (whole levels) and returns its first
+// tail batch. Throws if a lane's chain does not have the shape the tail
+// kernel relies on (entries before the run form a prefix of the chain).
+int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl, tail_tables& tt);
 
 }  // namespace apxchol_amd

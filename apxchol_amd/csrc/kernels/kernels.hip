@@ -52,25 +52,25 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         for (int k = 0; k < sc.nseg; ++k) {
             if (sc.seg_tail[k]) {
                 const int p0 = sc.seg_p0[k], p1 = sc.seg_p1[k];
-                const size_t lds = static_cast<size_t>(p1 - p0) * sizeof(TO);
+                const size_t lds = static_cast<size_t>(p1 - p0 + 1) * sizeof(TO);
                 if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: tail run exceeds LDS");
                 static const bool attr = [] {
-                    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TI, TO, F16>),
+                    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TO>),
                                               hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
                     return true;
                 }();
                 (void)attr;
                 const int bf = sc.level_first[sc.seg_l0[k]], bl = sc.level_first[sc.seg_l1[k]];
-                TO* part = nullptr;
-                if (p0 > 0) {
-                    if (static_cast<size_t>(bl - bf) * kWave * sizeof(TO) > sc.partial_bytes)
-                        throw std::runtime_error("apxchol_amd: coupling buffer too small");
-                    part = static_cast<TO*>(sc.partial);
-                    const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
-                    sweep_couple<TO, F16><<<g, kBlock, 0, s>>>(d, bf, bl, p0, o, part, st);
-                }
-                sweep_tail<TI, TO, F16><<<1, kTailBlock, lds, s>>>(d, sc.level_first_dev, sc.seg_l0[k],
-                                                                  sc.seg_l1[k], p0, in, o, part, st, err);
+                const int base = sc.seg_base[k];
+                if (bl <= bf || base < 0 || base + (bl - bf) > sc.tail_batches ||
+                    static_cast<size_t>(sc.tail_batches) * kWave * sizeof(tail_val<TO>) > sc.tail_val_bytes)
+                    throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
+                tail_val<TO>* tv = static_cast<tail_val<TO>*>(sc.tail_val);
+                const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
+                sweep_couple<TI, TO, F16><<<g, kBlock, 0, s>>>(d, bf, bl, p0, in, o, tv + static_cast<size_t>(base) * kWave, st);
+                sweep_tail<TO><<<1, kTailBlock, lds, s>>>(sc.tail, base, sc.level_first_dev, sc.seg_l0[k], sc.seg_l1[k],
+                                                          p1 - p0, d.backward ? d.m - 1 - p0 : p0, d.backward, o,
+                                                          tv, st);
             } else {
                 for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
             }
@@ -87,7 +87,7 @@ int sweep_dataflow_grid(vec_prec p, bool fp16) {
 }
 
 int sweep_tail_max_rows(vec_prec p) {
-    return kTailLdsBytes / static_cast<int>(p == vec_prec::fp64 ? sizeof(double) : sizeof(float));
+    return kTailLdsBytes / static_cast<int>(p == vec_prec::fp64 ? sizeof(double) : sizeof(float)) - 1;
 }
 
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,

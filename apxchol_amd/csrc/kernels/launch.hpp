@@ -30,10 +30,13 @@ struct sweep_sched {
     const int* seg_tail = nullptr;
     const int* seg_p0 = nullptr;           // tail segments: sweep positions [p0, p1)
     const int* seg_p1 = nullptr;
+    const int* seg_base = nullptr;         // tail segments: first tail batch
     int nseg = 0;
     int grid = 0;                          // dataflow grid / cap of a level launch
-    void* partial = nullptr;               // coupling buffer of the tail runs (device)
-    size_t partial_bytes = 0;
+    tail_desc tail;                        // static layout of the tail runs (device)
+    int tail_batches = 0;                  // tail batches of this direction
+    void* tail_val = nullptr;              // coupling output, tail_val<T> per (tail batch, lane) (device)
+    size_t tail_val_bytes = 0;
 };
 
 // One sweep. rhs: fp64 r (forward) or the forward output (back, fp64 or fp32

@@ -289,27 +289,21 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
 // A run of small levels [l0, l1) -- sweep positions [p0, p1) -- walked by ONE
 // workgroup of kTailBlock threads. The run's outputs are kept in LDS as well as
 // written out, so a dependency inside the run is an LDS read after the level
-// barrier; a dependency before the run is complete when the kernel starts.
-// Each wave prefetches its next batch (entries, right-hand side, diagonal and
-// every dependency from before the run) before the barrier that precedes it,
-// so a level costs one LDS gather, the row sums and one barrier. The per-row
-// arithmetic is sweep_batch's (same slot order, fma chain and butterfly), so
-// the result is bit-identical to the other schedules.
-inline constexpr int kTailBlock = 1024;
+// barrier. Dependencies before the run are folded in beforehand by
+// sweep_couple, which also stages the rows' right-hand sides; the rest of a
+// batch comes from the static tail layout (tail_desc), whose addresses depend
+// on (batch, lane) only. A wave therefore keeps its next two batches' loads in
+// flight while it works, across the level barriers, which wait for LDS
+// traffic only. The per-row arithmetic is sweep_batch's (same chain order, fma
+// and butterfly), so the result is bit-identical to the other schedules.
 inline constexpr int kTailLdsBytes = 160 * 1024;
 
-template <class TO>
-struct tail_batch {
-    int b = -1;
-    int row = 0, pos = 0, G = 1, sub = 0, start = 0;
-    unsigned len = 0, rb = 0;
-    bool active = false;
-    unsigned lds = 0;  // slots inside the run (jj = LDS index); other slots are in `part0`
-    int jj[kEpl];
-    float vv[kEpl];
-    TO part0 = 0;      // the lane's chain over its entries before the run (coupling kernel)
-    TO r = 0, dg = 1;
-};
+// Workgroup barrier ordering LDS only: outstanding global loads stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Lane -> row mapping of batch b (as in sweep_batch).
 struct lane_row {
@@ -337,14 +331,18 @@ __device__ __forceinline__ lane_row map_lane(const sweep_desc& d, int b) {
     return x;
 }
 
-// Coupling of a run: for every lane of every batch of the run, its fma chain
-// over the entries that point before the run (complete when this kernel runs),
-// in sweep_batch's slot order. Entries of a row are in ascending sweep
-// position, so these are a prefix of every lane's chain.
-template <class TO, bool F16>
-__global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0, const TO* out,
-                                                       TO* partial, const pcg_state* st) {
+// Coupling of a run (batches [bfirst, blast), positions from p0): for every
+// lane of every batch, its fma chain over the entries that point before the
+// run (complete when this kernel runs), in sweep_batch's slot order, and the
+// row's right-hand side for the lane that writes it. Entries of a row are in
+// ascending sweep position, so these are a prefix of every lane's chain.
+// tv points at the run's first tail batch.
+template <class TI, class TO, bool F16>
+__global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
+                                                       const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
+                                                       const pcg_state* st) {
     if (st && st->done) return;
+    const double shift = (st && !d.backward) ? st->shift : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     const int b = bfirst + static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
     if (b >= blast) return;
@@ -354,127 +352,141 @@ __global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst,
     const int G = longrow ? kWave : x.G;
     const int sub = longrow ? lane : x.sub;
     const bool act = longrow ? true : x.active;
-    for (unsigned c0 = 0; c0 < (longrow ? x.len : 1u); c0 += kWave * kEpl) {
-        int jj[kEpl];
-        float vv[kEpl];
-        TO g[kEpl];
-        unsigned use = 0;
-#pragma unroll
-        for (int k = 0; k < kEpl; ++k) {
-            const unsigned q = (longrow ? c0 : 0u) + static_cast<unsigned>(sub + k * G);
-            const bool v = act && q < x.len;
-            load_entry<F16>(d, v ? x.rb + q : (x.len > 0 ? x.rb : 0u), jj[k], vv[k]);
-            const int pj = d.backward ? d.m - 1 - jj[k] : jj[k];
-            use |= (v && pj < p0) ? (1u << k) : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
-#pragma unroll
-        for (int k = 0; k < kEpl; ++k)
-            if ((use >> k) & 1u) part = fma(static_cast<TO>(vv[k]), g[k], part);
-    }
-    partial[static_cast<size_t>(b - bfirst) * kWave + lane] = part;
-}
-
-template <class TI, class TO, bool F16>
-__device__ __forceinline__ void tail_load(const sweep_desc& d, int b, int bfirst, int p0, const TI* __restrict__ rhs,
-                                          const TO* partial, double shift, tail_batch<TO>& t) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const lane_row x = map_lane(d, b);
-    t.b = b;
-    t.row = x.row; t.pos = x.pos; t.G = x.G; t.sub = x.sub; t.len = x.len; t.rb = x.rb; t.active = x.active;
-    t.lds = 0;
-    t.part0 = partial ? partial[static_cast<size_t>(b - bfirst) * kWave + lane] : static_cast<TO>(0);
-    if (t.active && t.sub == 0) {
-        t.r = row_rhs<TI, TO>(d, rhs, t.row, shift);
-        t.dg = static_cast<TO>(d.diag[t.row]);
-    }
-    if (t.len > static_cast<unsigned>(kWave * kEpl)) return;   // long row: walked in tail_finish
-#pragma unroll
-    for (int k = 0; k < kEpl; ++k) {
-        const unsigned q = static_cast<unsigned>(t.sub + k * t.G);
-        const bool v = t.active && q < t.len;
-        load_entry<F16>(d, v ? t.rb + q : (t.active && t.len > 0 ? t.rb : 0u), t.jj[k], t.vv[k]);
-        const int pj = d.backward ? d.m - 1 - t.jj[k] : t.jj[k];
-        const bool in_run = v && pj >= p0;
-        t.lds |= in_run ? (1u << k) : 0u;
-        t.jj[k] = in_run ? pj - p0 : 0;
-    }
-}
-
-template <class TI, class TO, bool F16>
-__device__ __forceinline__ void tail_finish(const sweep_desc& d, int p0, TO* out, TO* lds, tail_batch<TO>& t) {
-    using AT = TO;
-    const int lane = threadIdx.x & (kWave - 1);
-    AT part = t.part0;
-    if (t.len > static_cast<unsigned>(kWave * kEpl)) {
-        // Long row alone in its batch: chunks of 64 * kEpl entries; only the
-        // entries inside the run remain (the rest are in part0).
-        for (unsigned c0 = 0; c0 < t.len; c0 += kWave * kEpl) {
+    if (p0 > 0) {
+        for (unsigned c0 = 0; c0 < (longrow ? x.len : 1u); c0 += kWave * kEpl) {
+            int jj[kEpl];
+            float vv[kEpl];
+            TO g[kEpl];
+            unsigned use = 0;
 #pragma unroll
             for (int k = 0; k < kEpl; ++k) {
-                const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
-                const bool v = q < t.len;
-                int j;
-                float w;
-                load_entry<F16>(d, v ? t.rb + q : t.rb + c0, j, w);
-                const int pj = d.backward ? d.m - 1 - j : j;
-                const bool in_run = v && pj >= p0;
-                const AT xv = lds[in_run ? pj - p0 : 0];
-                if (in_run) part = fma(static_cast<AT>(w), xv, part);
+                const unsigned q = (longrow ? c0 : 0u) + static_cast<unsigned>(sub + k * G);
+                const bool v = act && q < x.len;
+                load_entry<F16>(d, v ? x.rb + q : (x.len > 0 ? x.rb : 0u), jj[k], vv[k]);
+                const int pj = d.backward ? d.m - 1 - jj[k] : jj[k];
+                use |= (v && pj < p0) ? (1u << k) : 0u;
             }
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k)
+                if ((use >> k) & 1u) part = fma(static_cast<TO>(vv[k]), g[k], part);
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, kWave);
-        if (lane == 0) {
-            const AT v = (t.r - part) / t.dg;
-            out[t.row] = v;
-            lds[t.pos - p0] = v;
-        }
-        return;
     }
-#pragma unroll
-    for (int k = 0; k < kEpl; ++k)
-        if ((t.lds >> k) & 1u) part = fma(static_cast<AT>(t.vv[k]), lds[t.jj[k]], part);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const AT o = __shfl_xor(part, off, kWave);
-        if (off < t.G) part += o;
-    }
-    if (t.active && t.sub == 0) {
-        const AT v = (t.r - part) / t.dg;
-        out[t.row] = v;
-        lds[t.pos - p0] = v;
-    }
+    TO r = 0;
+    if (x.active && x.sub == 0) r = row_rhs<TI, TO>(d, rhs, x.row, shift);
+    tail_val<TO> o;
+    o.part0 = part;
+    o.r = r;
+    tv[static_cast<size_t>(b - bfirst) * kWave + lane] = o;
 }
 
-template <class TI, class TO, bool F16>
-__global__ __launch_bounds__(kTailBlock) void sweep_tail(sweep_desc d, const int* __restrict__ level_first, int l0,
-                                                         int l1, int p0, const TI* __restrict__ rhs, TO* out,
-                                                         const TO* partial, const pcg_state* st, int* err) {
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+using vec2 = T __attribute__((ext_vector_type(2)));
+
+// By value: a bit_cast of an ext-vector element lvalue is not reliable here
+// (clang read element 0 for .y).
+__device__ __forceinline__ float as_float(int v) { return __builtin_bit_cast(float, v); }
+
+// One lane's share of one tail batch, as whole 16-byte vectors: each load
+// lands in one register tuple that is consumed in place.
+template <class TO>
+struct tail_item {
+    i32x4 ln;                // tail_lane
+    i32x4 e0, e1, e2, e3;    // tail_pair x kTailPairs
+    vec2<TO> v;              // tail_val
+};
+
+__device__ __forceinline__ size_t tail_at(int t, int k) {
+    return (static_cast<size_t>(t) * kTailPairs + static_cast<size_t>(k)) * kWave + (threadIdx.x & (kWave - 1));
+}
+
+template <class TO>
+__device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<TO>* tv, int t, tail_item<TO>& it) {
+    const size_t i = static_cast<size_t>(t) * kWave + (threadIdx.x & (kWave - 1));
+    const i32x4* ent = reinterpret_cast<const i32x4*>(td.ent);
+    it.ln = reinterpret_cast<const i32x4*>(td.lane)[i];
+    it.e0 = ent[tail_at(t, 0)];
+    it.e1 = ent[tail_at(t, 1)];
+    it.e2 = ent[tail_at(t, 2)];
+    it.e3 = ent[tail_at(t, 3)];
+    it.v = reinterpret_cast<const vec2<TO>*>(tv)[i];
+}
+
+template <class TO>
+__device__ __forceinline__ TO tail_pair_fma(const i32x4 e, const TO* lds, TO part) {
+    const TO x0 = lds[e.x], x1 = lds[e.z];
+    part = fma(static_cast<TO>(as_float(e.y)), x0, part);
+    return fma(static_cast<TO>(as_float(e.w)), x1, part);
+}
+
+// Row sums of tail batch t from the item and the run's LDS values. Padding
+// entries are (zero slot, 0): fma(0, 0, part) is sweep_batch's empty slot, so
+// every lane runs the same fmas.
+template <class TO>
+__device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail_item<TO>& it, TO* lds) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = it.ln.z >> 8;
+    const int G = it.ln.z & 0xff;
+    TO part = it.v.x;
+    part = tail_pair_fma(it.e0, lds, part);
+    part = tail_pair_fma(it.e1, lds, part);
+    part = tail_pair_fma(it.e2, lds, part);
+    part = tail_pair_fma(it.e3, lds, part);
+    if (__builtin_expect(c > 2 * kTailPairs, 0)) {
+        // A row longer than 64 * 2 * kTailPairs: the lane's further entries.
+        const size_t o = static_cast<size_t>(td.ovf[t]);
+        for (int s = 0; s < c - 2 * kTailPairs; ++s) {
+            const entry32 e = td.ovf_ent[(o + static_cast<size_t>(s)) * kWave + lane];
+            part = fma(static_cast<TO>(e.val), lds[e.idx], part);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const TO o = __shfl_xor(part, off, kWave);
+        if (off < G) part += o;
+    }
+    if (it.ln.x >= 0) lds[it.ln.y] = (it.v.y - part) / static_cast<TO>(as_float(it.ln.w));
+}
+
+// base: the run's first tail batch; levels [l0, l1) of the run, each of at
+// most kTailWaves batches; zslot: the run's row count (its LDS holds
+// zslot + 1 values); output row of slot s: row0 + s (forward) or row0 - s
+// (back). Outputs reach HBM in one coalesced copy at the end.
+template <class TO>
+__global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, const int* __restrict__ level_first,
+                                                         int l0, int l1, int zslot, int row0, int backward, TO* out,
+                                                         const tail_val<TO>* tv, const pcg_state* st) {
     if (st && st->done) return;
     extern __shared__ unsigned char tail_lds[];
     TO* lds = reinterpret_cast<TO*>(tail_lds);
-    const double shift = (st && !d.backward) ? st->shift : 0.0;
-    constexpr int waves = kTailBlock / kWave;
-    const int w = static_cast<int>(threadIdx.x) / kWave;
     const int bfirst = level_first[l0];
-    tail_batch<TO> t;
-    if (bfirst + w < level_first[l0 + 1]) tail_load<TI, TO, F16>(d, bfirst + w, bfirst, p0, rhs, partial, shift, t);
+    const int last = base + level_first[l1] - bfirst - 1;
+    // The zero slot past the run's outputs, read by padding entries; every
+    // wave clears it before its own first read (all writers store 0).
+    lds[zslot] = TO(0);
+    // Wave w takes batch w of every level. The item of level l+1 is fetched
+    // before level l is computed, so its loads are in flight during that
+    // level; every fetch is issued (past the run's end it re-reads the last
+    // batch), so the compiler's memory counters see the same loads on every
+    // path. (readfirstlane: the wave index is wave-uniform, so the level
+    // bookkeeping stays scalar.)
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    int lb = 0, le = level_first[l0 + 1] - bfirst;
+    tail_item<TO> cur;
+    tail_fetch<TO>(td, tv, min(base + w, last), cur);
     for (int l = l0; l < l1; ++l) {
-        const int b0 = level_first[l], b1 = level_first[l + 1];
-        for (int b = b0 + w; b < b1; b += waves) {
-            if (t.b != b) tail_load<TI, TO, F16>(d, b, bfirst, p0, rhs, partial, shift, t);
-            tail_finish<TI, TO, F16>(d, p0, out, lds, t);
-            t.b = -1;
-            if (b + waves < b1) tail_load<TI, TO, F16>(d, b + waves, bfirst, p0, rhs, partial, shift, t);
-        }
-        if (l + 1 < l1) {
-            const int nb = level_first[l + 1] + w;
-            if (nb < level_first[l + 2] && t.b != nb) tail_load<TI, TO, F16>(d, nb, bfirst, p0, rhs, partial, shift, t);
-        }
-        __syncthreads();
+        const int nb = le, ne = level_first[l + 2 <= l1 ? l + 2 : l1] - bfirst;
+        tail_item<TO> nxt;
+        tail_fetch<TO>(td, tv, min(base + nb + w, last), nxt);
+        if (lb + w < le) tail_step<TO>(td, base + lb + w, cur, lds);
+        lds_barrier();
+        cur = nxt;
+        lb = nb;
+        le = ne;
     }
+    for (int q = static_cast<int>(threadIdx.x); q < zslot; q += kTailBlock) out[backward ? row0 - q : row0 + q] = lds[q];
 }
 
 // Fill a vector with the pending sentinel (the state every sweep output must
