@@ -42,7 +42,7 @@ class Options(ctypes.Structure):
     _fields_ = [("tol", F64), ("max_iter", I32), ("stagnation_window", I32), ("seed", U32),
                 ("precision", I32), ("fp16_storage", I32), ("schedule", I32), ("drop_rel", F64),
                 ("keep_factor", I32), ("iters_per_check", I32), ("sweep_chunk", I32), ("sweep_blocks", I32),
-                ("tail_rows", I32)]
+                ("tail_rows", I32), ("graph", I32)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -142,7 +142,7 @@ def _csc(A):
 
 def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
              schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=4, sweep_chunk=0,
-             sweep_blocks=0, tail_rows=0) -> Options:
+             sweep_blocks=0, tail_rows=0, graph=True) -> Options:
     o = Options()
     load_library().apx_options_default(ctypes.byref(o))
     o.tol = float(tol)
@@ -158,6 +158,7 @@ def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp
     o.sweep_chunk = int(sweep_chunk)
     o.sweep_blocks = int(sweep_blocks)
     o.tail_rows = int(tail_rows)
+    o.graph = 1 if graph else 0
     return o
 
 
@@ -301,18 +302,21 @@ class Solver:
         _check(lib.apx_solver_apply(self._h, _ptr(r), _ptr(z), s))
         return z
 
+    TIMING_CLASSES = ("fwd_sweep", "bck_sweep", "spmv", "vector", "sweep_levels", "sweep_tail", "sweep_couple")
+
     def set_timing(self, on: bool = True):
-        """HIP-event timing of kernel classes (forward sweep, back sweep, SpMV,
-        vector passes) on the launching stream."""
+        """HIP-event timing on the launching stream of whole sweeps, SpMV and
+        vector passes, and of the single sweep kernels (sweep_levels,
+        sweep_tail, sweep_couple). Solves launch eagerly while it is on."""
         _check(load_library().apx_solver_set_timing(self._h, int(on)))
 
     def take_timing(self):
         """{class: (total_ms, launches)} since the last call (then reset)."""
-        ms = np.zeros(4)
-        cnt = np.zeros(4, np.int64)
+        k = len(self.TIMING_CLASSES)
+        ms = np.zeros(k)
+        cnt = np.zeros(k, np.int64)
         _check(load_library().apx_solver_take_timing(self._h, ms.ctypes.data, cnt.ctypes.data))
-        names = ("fwd_sweep", "bck_sweep", "spmv", "vector")
-        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(names)}
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(self.TIMING_CLASSES)}
 
     def reset_apply_count(self):
         _check(load_library().apx_solver_reset_apply_count(self._h))

@@ -77,6 +77,7 @@ solve_options to_opts(const apx_options* o) {
     so.iters_per_check = o->iters_per_check > 0 ? o->iters_per_check : 4;
     so.sweep_chunk = o->sweep_chunk;
     so.sweep_blocks = o->sweep_blocks;
+    so.graph = o->graph != 0;
     return so;
 }
 
@@ -112,6 +113,7 @@ void apx_options_default(apx_options* o) {
     o->sweep_chunk = 0;
     o->sweep_blocks = 0;
     o->tail_rows = 0;
+    o->graph = 1;
 }
 
 const char* apx_last_error(void) { return g_err.c_str(); }

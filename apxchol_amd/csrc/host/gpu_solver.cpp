@@ -323,7 +323,15 @@ struct gpu_solver::device_state {
     pcg_state* host_st = nullptr;
     int* host_err = nullptr;
     sweep_desc fwd, bck;
+    // One PCG iteration captured as a graph on a private stream (default
+    // priority) and replayed on the caller's stream.
+    hipStream_t cap = nullptr;
+    hipGraph_t iter_graph = nullptr;
+    hipGraphExec_t iter_exec = nullptr;
     ~device_state() {
+        if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
+        if (iter_graph) (void)hipGraphDestroy(iter_graph);
+        if (cap) (void)hipStreamDestroy(cap);
         if (host_st) (void)hipHostFree(host_st);
         if (host_err) (void)hipHostFree(host_err);
     }
@@ -333,13 +341,37 @@ struct gpu_solver::device_state {
 // record carries the PCG step it belongs to, so launches after convergence
 // (which return at once) can be dropped when the solve ends.
 struct gpu_solver::timer {
+    static constexpr int K = gpu_solver::kTimingClasses;
     bool on = false;
     struct rec { int cls; int step; hipEvent_t a, b; };
     std::vector<hipEvent_t> pool;
     std::size_t used = 0;
     std::vector<rec> recs;
-    double ms[4] = {0, 0, 0, 0};
-    std::int64_t launches[4] = {0, 0, 0, 0};
+    double ms[K] = {};
+    std::int64_t launches[K] = {};
+    // Per-kernel records of a sweep (launch_hook): the step being enqueued
+    // and the event opened by the last phase-0 call.
+    int step = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t open = nullptr;
+    static void hook_fn(void* ctx, int kind, int phase) {
+        auto* t = static_cast<timer*>(ctx);
+        if (!t->on) return;
+        if (phase == 0) {
+            t->open = t->get();
+            hip_check(hipEventRecord(t->open, t->stream), "hipEventRecord");
+            return;
+        }
+        const hipEvent_t b = t->get();
+        hip_check(hipEventRecord(b, t->stream), "hipEventRecord");
+        const int cls = kind == 1 ? 5 : kind == 2 ? 6 : 4;
+        t->recs.push_back({cls, t->step, t->open, b});
+    }
+    launch_hook hook(int st, hipStream_t s) {
+        step = st;
+        stream = s;
+        return launch_hook{on ? &hook_fn : nullptr, this};
+    }
     ~timer() { for (auto e : pool) (void)hipEventDestroy(e); }
     hipEvent_t get() {
         if (used == pool.size()) {
@@ -375,7 +407,7 @@ struct gpu_solver::timer {
 void gpu_solver::set_timing(bool on) const { timer_->on = on; }
 
 void gpu_solver::take_timing(double* ms, std::int64_t* launches) const {
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kTimingClasses; ++k) {
         ms[k] = timer_->ms[k];
         launches[k] = timer_->launches[k];
         timer_->ms[k] = 0.0;
@@ -624,6 +656,51 @@ void gpu_solver::check_sweep_error() const {
     }
 }
 
+// Step k: the preconditioner application after iteration k (k = 0 is the one
+// before the loop); the SpMV / update of iteration k carry step k - 1, so a
+// solve that stops at iteration `it` keeps steps 0 .. it-1.
+void gpu_solver::enqueue_precondition(bool first, int step, hipStream_t s) const {
+    auto& d = *dev_;
+    timer& tm = *timer_;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    pcg_state* st = d.st.as<pcg_state>();
+    double* part = d.part.as<double>();
+    unsigned* cnt = d.counter.as<unsigned>();
+    int* err = d.err.as<int>();
+    const vec_prec pr = d.prec;
+    if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
+    tm.time(0, step, s, [&] {
+        launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, d.f_sched, s, tm.hook(step, s));
+    });
+    tm.time(1, step, s, [&] {
+        launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s, tm.hook(step, s));
+    });
+    tm.time(3, step, s, [&] {
+        launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
+        launch_update_p(pr, n, d.p.as<double>(), d.z.get(), st, first, s);
+    });
+}
+
+void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
+    auto& d = *dev_;
+    timer& tm = *timer_;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    pcg_state* st = d.st.as<pcg_state>();
+    double* part = d.part.as<double>();
+    unsigned* cnt = d.counter.as<unsigned>();
+    tm.time(2, it - 1, s, [&] {
+        launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
+                    d.p.as<double>(), d.ap.as<double>(), st, part, cnt, true, s);
+    });
+    tm.time(3, it - 1, s, [&] {
+        launch_update_xr(d.prec, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
+                         d.y.get(), d.z.get(), st, part, cnt, s);
+    });
+    enqueue_precondition(false, it, s);
+}
+
 solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol, int max_iter, const double* d_x0,
                                       hipStream_t s) const {
     if (tol < 0.0) tol = opts_.tol;
@@ -660,33 +737,31 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
         launch_init_residual(n, d.r.as<double>(), nullptr, st, part, cnt, s);
     }
     timer& tm = *timer_;
-    // Step k: the preconditioner application after iteration k (k = 0 is the
-    // one before the loop); the SpMV / update of iteration k carry step k - 1,
-    // so a solve that stops at iteration `it` keeps steps 0 .. it-1.
-    auto precondition = [&](bool first, int step) {
-        if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
-        tm.time(0, step, s, [&] { launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, d.f_sched, s); });
-        tm.time(1, step, s, [&] { launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s); });
-        tm.time(3, step, s, [&] {
-            launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
-            launch_update_p(pr, n, d.p.as<double>(), d.z.get(), st, first, s);
-        });
-    };
-    precondition(true, 0);
+    enqueue_precondition(true, 0, s);
     const int J = std::max(1, opts_.iters_per_check);
+    // Graph replay of the iteration (every argument of its kernels is a
+    // pointer into this solver's device state or a size, so one capture
+    // serves every solve); eager launches while timing.
+    const bool graph = opts_.graph && !tm.on;
+    if (graph && !d.iter_exec) {
+        if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            enqueue_iteration(0, d.cap);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(d.cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(d.cap, &d.iter_graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&d.iter_exec, d.iter_graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
     int enqueued = 0;
     for (;;) {
         for (int j = 0; j < J; ++j) {
-            const int it = enqueued + j + 1;
-            tm.time(2, it - 1, s, [&] {
-                launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
-                            d.p.as<double>(), d.ap.as<double>(), st, part, cnt, true, s);
-            });
-            tm.time(3, it - 1, s, [&] {
-                launch_update_xr(pr, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
-                                 d.y.get(), d.z.get(), st, part, cnt, s);
-            });
-            precondition(false, it);
+            if (graph) hip_check(hipGraphLaunch(d.iter_exec, s), "hipGraphLaunch");
+            else enqueue_iteration(enqueued + j + 1, s);
         }
         enqueued += J;
         hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");

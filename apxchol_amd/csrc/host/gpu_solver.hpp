@@ -32,6 +32,7 @@ struct solve_options {
     int iters_per_check = 4;      // iterations enqueued between completion tests
     int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 8)
     int sweep_blocks = 0;         // dataflow grid (0: APXCHOL_AMD_SWEEP_BLOCKS or the resident grid)
+    bool graph = true;            // replay each PCG iteration as one captured HIP graph
 };
 
 // solve.h:27-42. status: 0 running/none, 1 converged, 2 pAp <= 0 breakdown,
@@ -80,11 +81,13 @@ public:
     void reset_apply_count() const { apply_count_ = 0; }
 
     // Per-kernel-class timing with HIP events on the launching stream
-    // (classes: 0 forward sweep, 1 back sweep, 2 SpMV, 3 vector passes).
-    // Only launches that did work are counted: those of iterations after
-    // convergence return at once and are excluded.
+    // (classes: 0 forward sweep, 1 back sweep, 2 SpMV, 3 vector passes, then
+    // single kernels 4 sweep_levels, 5 sweep_tail, 6 sweep_couple). Solves run
+    // eagerly while timing is on. Only launches that did work are counted:
+    // those of iterations after convergence return at once and are excluded.
+    static constexpr int kTimingClasses = 7;
     void set_timing(bool on) const;
-    // ms[4], launches[4] accumulated since the last call; resets them.
+    // ms[kTimingClasses], launches[kTimingClasses] since the last call; resets them.
     void take_timing(double* ms, std::int64_t* launches) const;
     const factorization& factor() const { return F_; }
     const trsv_factor& trsv() const { return T_; }
@@ -96,6 +99,10 @@ private:
     struct timer;
     void setup(const csc_view& L);
     void check_sweep_error() const;
+    // Enqueue PCG iteration `it` (SpMV, x/r update, both sweeps, the
+    // preconditioned dot and the p update); step labels the timing records.
+    void enqueue_iteration(int step, hipStream_t s) const;
+    void enqueue_precondition(bool first, int step, hipStream_t s) const;
 
     solve_options opts_;
     factorization F_;

@@ -30,20 +30,27 @@ int dataflow_grid_for() {
     return per_cu * cus;
 }
 
+// Workgroups of one level launch: one wave per batch, up to this many.
+inline constexpr int kLevelMaxBlocks = 16384;
+
 template <class TI, class TO, bool F16>
 void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* st, int* err, const sweep_sched& sc,
-             hipStream_t s) {
+             hipStream_t s, const launch_hook& hook) {
     const TI* in = static_cast<const TI*>(rhs);
     TO* o = static_cast<TO*>(out);
     auto level_launch = [&](int l) {
         const int b0 = sc.level_first[l], b1 = sc.level_first[l + 1];
         if (b1 <= b0) return;
         int g = (b1 - b0 + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (g > sc.grid) g = sc.grid;
+        if (g > kLevelMaxBlocks) g = kLevelMaxBlocks;
+        hook(0, 0);
         sweep_levels<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b0, b1, in, o, st, err);
+        hook(0, 1);
     };
     if (sc.kind == 0) {
+        hook(3, 0);
         sweep_dataflow<TI, TO, F16><<<sc.grid, kBlock, 0, s>>>(d, in, o, st, sc.grid * kWavesPerBlock, err);
+        hook(3, 1);
         check_launch("sweep_dataflow");
     } else if (sc.kind == 1) {
         for (int l = 0; l < sc.nlevels; ++l) level_launch(l);
@@ -67,10 +74,14 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                     throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
                 tail_val<TO>* tv = static_cast<tail_val<TO>*>(sc.tail_val);
                 const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
+                hook(2, 0);
                 sweep_couple<TI, TO, F16><<<g, kBlock, 0, s>>>(d, bf, bl, p0, in, o, tv + static_cast<size_t>(base) * kWave, st);
+                hook(2, 1);
+                hook(1, 0);
                 sweep_tail<TO><<<1, kTailBlock, lds, s>>>(sc.tail, base, sc.level_first_dev, sc.seg_l0[k], sc.seg_l1[k],
                                                           p1 - p0, d.backward ? d.m - 1 - p0 : p0, d.backward, o,
                                                           tv, st);
+                hook(1, 1);
             } else {
                 for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
             }
@@ -91,17 +102,17 @@ int sweep_tail_max_rows(vec_prec p) {
 }
 
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
-                  int* err, const sweep_sched& sc, hipStream_t s) {
+                  int* err, const sweep_sched& sc, hipStream_t s, const launch_hook& hook) {
     if (d.nb == 0) return;
     if (p == vec_prec::fp64) {
-        if (fp16) sweep_t<double, double, true>(d, rhs, out, st, err, sc, s);
-        else sweep_t<double, double, false>(d, rhs, out, st, err, sc, s);
+        if (fp16) sweep_t<double, double, true>(d, rhs, out, st, err, sc, s, hook);
+        else sweep_t<double, double, false>(d, rhs, out, st, err, sc, s, hook);
     } else if (!d.backward) {
-        if (fp16) sweep_t<double, float, true>(d, rhs, out, st, err, sc, s);
-        else sweep_t<double, float, false>(d, rhs, out, st, err, sc, s);
+        if (fp16) sweep_t<double, float, true>(d, rhs, out, st, err, sc, s, hook);
+        else sweep_t<double, float, false>(d, rhs, out, st, err, sc, s, hook);
     } else {
-        if (fp16) sweep_t<float, float, true>(d, rhs, out, st, err, sc, s);
-        else sweep_t<float, float, false>(d, rhs, out, st, err, sc, s);
+        if (fp16) sweep_t<float, float, true>(d, rhs, out, st, err, sc, s, hook);
+        else sweep_t<float, float, false>(d, rhs, out, st, err, sc, s, hook);
     }
 }
 

@@ -39,11 +39,19 @@ struct sweep_sched {
     size_t tail_val_bytes = 0;
 };
 
+// Called around every kernel launch of a sweep (timing): kind 0 sweep_levels,
+// 1 sweep_tail, 2 sweep_couple, 3 sweep_dataflow; phase 0 before, 1 after.
+struct launch_hook {
+    void (*fn)(void* ctx, int kind, int phase) = nullptr;
+    void* ctx = nullptr;
+    void operator()(int kind, int phase) const { if (fn) fn(ctx, kind, phase); }
+};
+
 // One sweep. rhs: fp64 r (forward) or the forward output (back, fp64 or fp32
 // as `p`); out: fp64 or fp32 as `p` (every slot pending on entry for the
 // dataflow schedule).
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
-                  int* err, const sweep_sched& sc, hipStream_t s);
+                  int* err, const sweep_sched& sc, hipStream_t s, const launch_hook& hook = {});
 
 void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s);
 

@@ -45,6 +45,8 @@ typedef struct {
     int32_t sweep_chunk;     /* batches claimed per dataflow ticket (0 = default) */
     int32_t sweep_blocks;    /* dataflow workgroups / level-launch cap (0 = the device's resident grid) */
     int32_t tail_rows;       /* hybrid: levels with fewer rows go to the one-workgroup tail (0 = default) */
+    int32_t graph;           /* 1 (default): each PCG iteration replays one captured HIP graph;
+                                0: eager launches (also used while kernel timing is on) */
 } apx_options;
 
 /* Fill `o` with the defaults above. */
@@ -131,11 +133,16 @@ apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rh
 
 /* Kernel timing with HIP events on the launching stream (no reference
  * counterpart; the reference's APXCHOL_PCG_TRACE per-component ms/iter,
- * src/solve.cpp:578-653, is the closest). Classes: 0 forward sweep, 1 back
- * sweep, 2 SpMV, 3 vector passes; launches of iterations after convergence
- * are excluded. apx_solver_take_timing returns and resets the totals. */
+ * src/solve.cpp:578-653, is the closest). While timing is on, solves launch
+ * eagerly. Classes (APX_TIMING_CLASSES): 0 forward sweep, 1 back sweep, 2 SpMV,
+ * 3 vector passes (whole sweeps / passes, gaps between their kernels included);
+ * then single kernels: 4 sweep_levels, 5 sweep_tail, 6 sweep_couple (both
+ * directions). Launches of iterations after convergence are excluded.
+ * apx_solver_take_timing returns and resets the totals. */
+#define APX_TIMING_CLASSES 7
 apx_status apx_solver_set_timing(apx_solver_t s, int32_t on);
-apx_status apx_solver_take_timing(apx_solver_t s, double* ms /* 4 */, int64_t* launches /* 4 */);
+apx_status apx_solver_take_timing(apx_solver_t s, double* ms /* APX_TIMING_CLASSES */,
+                                  int64_t* launches /* APX_TIMING_CLASSES */);
 
 /* Host copies of the factor kept with keep_factor = 1 (factorization::L, perm, rounds). */
 apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* inner, float* vals, uint32_t* perm);

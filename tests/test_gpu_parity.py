@@ -177,3 +177,38 @@ def test_solve_device_vectors_and_warm_start(gpu):
     assert np.array_equal(x1.view(np.int64), r3.x.cpu().numpy().view(np.int64))
     assert r3.iterations == r1.iterations
     s.close()
+
+
+@pytest.mark.parametrize("name", ["grid2d_256", "sddm3d_16", "rmat_16"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replay_matches_eager(gpu, name, precision):
+    """The captured iteration graph, replayed many times per solve and across
+    solves with eager work and synchronizations in between, gives the eager
+    launches' bits exactly (x, iterations, residual)."""
+    import torch
+    A, F, fac = case(name)
+    print(f"[case] graph vs eager {name} {precision}", flush=True)
+    b = mg.project_range(A, np.random.default_rng(3).standard_normal(A.shape[0])) if not F.sddm else \
+        np.random.default_rng(3).standard_normal(A.shape[0])
+    bd = torch.tensor(b, device=gpu)
+    eager = ax.Solver(A, factor=F, precision=precision, graph=False)
+    ref = eager.solve(bd)
+    x_ref = ref.x.cpu().numpy()
+    eager.close()
+    g = ax.Solver(A, factor=F, precision=precision, graph=True)
+    for k in range(3):
+        r = g.solve(bd)
+        torch.cuda.synchronize()
+        junk = (bd * 2.0).sum().item()        # eager work between replays
+        assert np.isfinite(junk)
+        assert r.iterations == ref.iterations and r.status == ref.status
+        assert r.residual == ref.residual
+        assert np.array_equal(x_ref.view(np.int64), r.x.cpu().numpy().view(np.int64)), f"solve {k}"
+    # Timing forces eager launches in the same solver: still the same bits.
+    g.set_timing(True)
+    r = g.solve(bd)
+    g.set_timing(False)
+    tm = g.take_timing()
+    assert np.array_equal(x_ref.view(np.int64), r.x.cpu().numpy().view(np.int64))
+    assert tm["sweep_levels"][1] + tm["sweep_tail"][1] > 0
+    g.close()
