@@ -61,6 +61,8 @@ class SolverInfo(ctypes.Structure):
 _SIGS = {
     "apx_last_error": ([], ctypes.c_char_p),
     "apx_options_default": ([P], None),
+    "apx_fp16_round": ([ctypes.c_float], ctypes.c_uint16),
+    "apx_fp16_widen": ([ctypes.c_uint16], ctypes.c_float),
     "apx_factorize": ([I64, P, P, P, U32, P], ctypes.c_int),
     "apx_factor_info": ([P, P], ctypes.c_int),
     "apx_factor_export": ([P, P, P, P, P, P], ctypes.c_int),

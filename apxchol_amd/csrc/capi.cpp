@@ -118,6 +118,9 @@ void apx_options_default(apx_options* o) {
 
 const char* apx_last_error(void) { return g_err.c_str(); }
 
+uint16_t apx_fp16_round(float x) { return fp16_round(x); }
+float apx_fp16_widen(uint16_t h) { return fp16_widen(h); }
+
 apx_status apx_factorize(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, uint32_t seed,
                          apx_factor_t* out) {
     return guarded([&] {

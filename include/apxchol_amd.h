@@ -88,6 +88,14 @@ apx_status apx_factor_export(apx_factor_t f, uint64_t* outer, uint32_t* inner, f
                              uint64_t* round_is);
 void apx_factor_destroy(apx_factor_t f);
 
+/* ---- fp16 storage conversions (no GPU) -----------------------------------
+ * fp16_t(float) and fp16_t::to_float (include/apxchol/lowprec.h:111-201):
+ * IEEE binary16 round-to-nearest-even with subnormals, overflow to infinity
+ * and NaN kept; the widening is exact. (The factor storage additionally
+ * flushes stored subnormals to signed zero, omp.h:353-366.) */
+uint16_t apx_fp16_round(float x);
+float apx_fp16_widen(uint16_t h);
+
 /* ---- host SpTRSV preparation (no GPU) ------------------------------------
  * omp_sptrsv::setup / cuda_sptrsv::setup host part (include/apxchol/solver/sptrsv/omp.h:505-861,
  * cuda.h:233-459, cuda_host.h:62-232): L11, compacting drop, fp16 narrowing,
