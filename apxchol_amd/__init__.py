@@ -55,7 +55,8 @@ class SolverInfo(ctypes.Structure):
                 ("stored_nnz", ctypes.c_uint64), ("offdiag", ctypes.c_uint64), ("dropped", ctypes.c_uint64),
                 ("rounds", I64), ("levels", I64), ("fwd_batches", I64), ("bck_batches", I64), ("op_nnz", I64),
                 ("op_fp32", I32), ("spmv_lanes", I32), ("dataflow_grid", I32), ("fp16_storage", I32),
-                ("device_bytes", ctypes.c_uint64)]
+                ("device_bytes", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64 * 2),
+                ("tail_bytes", ctypes.c_uint64 * 2), ("level_launches", I64 * 2), ("tail_runs", I64 * 2)]
 
 
 _SIGS = {
@@ -253,7 +254,8 @@ class Solver:
         self._h = h
         inf = SolverInfo()
         _check(lib.apx_solver_info_get(h, ctypes.byref(inf)))
-        self.info = {k: getattr(inf, k) for k, _ in SolverInfo._fields_}
+        self.info = {k: (list(v) if isinstance(v, ctypes.Array) else v)
+                     for k, v in ((k, getattr(inf, k)) for k, _ in SolverInfo._fields_)}
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -236,6 +236,12 @@ apx_status apx_solver_info_get(apx_solver_t s, apx_solver_info* info) {
         info->op_fp32 = st.op_fp32 ? 1 : 0; info->spmv_lanes = st.spmv_lanes; info->dataflow_grid = st.dataflow_grid;
         info->fp16_storage = s->s->trsv().fp16 ? 1 : 0;
         info->device_bytes = st.device_bytes;
+        for (int k = 0; k < 2; ++k) {
+            info->level_bytes[k] = st.level_bytes[k];
+            info->tail_bytes[k] = st.tail_bytes[k];
+            info->level_launches[k] = st.level_launches[k];
+            info->tail_runs[k] = st.tail_runs[k];
+        }
     });
 }
 

@@ -349,27 +349,20 @@ struct gpu_solver::timer {
     std::vector<rec> recs;
     double ms[K] = {};
     std::int64_t launches[K] = {};
-    // Per-kernel records of a sweep (launch_hook): the step being enqueued
-    // and the event opened by the last phase-0 call.
+    // Single-kernel records (launch_hook): the launch carries the event pair,
+    // which then holds the kernel's own start and end.
     int step = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t open = nullptr;
-    static void hook_fn(void* ctx, int kind, int phase) {
+    static bool hook_fn(void* ctx, int kind, hipEvent_t* ev) {
         auto* t = static_cast<timer*>(ctx);
-        if (!t->on) return;
-        if (phase == 0) {
-            t->open = t->get();
-            hip_check(hipEventRecord(t->open, t->stream), "hipEventRecord");
-            return;
-        }
-        const hipEvent_t b = t->get();
-        hip_check(hipEventRecord(b, t->stream), "hipEventRecord");
-        const int cls = kind == 1 ? 5 : kind == 2 ? 6 : 4;
-        t->recs.push_back({cls, t->step, t->open, b});
+        if (!t->on) return false;
+        ev[0] = t->get();
+        ev[1] = t->get();
+        const int cls = kind == 1 ? 5 : kind == 2 ? 6 : kind == 4 ? 2 : 4;
+        t->recs.push_back({cls, t->step, ev[0], ev[1]});
+        return true;
     }
-    launch_hook hook(int st, hipStream_t s) {
+    launch_hook hook(int st) {
         step = st;
-        stream = s;
         return launch_hook{on ? &hook_fn : nullptr, this};
     }
     ~timer() { for (auto e : pool) (void)hipEventDestroy(e); }
@@ -595,6 +588,32 @@ void gpu_solver::setup(const csc_view& L) {
         sc.seg_base = seg[5].data();
         sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
                                    : opts_.schedule == sweep_schedule::levels ? 1 : 2);
+        // Algorithmic bytes per level (see solver_stats).
+        if (sc.kind != 0) {
+            const auto& ptr = backward ? T_.b_ptr : T_.f_ptr;
+            const std::uint64_t vf = T_.fp16 ? 2 : 4;
+            const std::uint64_t vin = backward ? (d.prec == vec_prec::fp32 ? 4 : 8) : 8;
+            const std::uint64_t vout = d.prec == vec_prec::fp32 ? 4 : 8;
+            const std::uint64_t row_b = 8 + vin + vout + ((T_.fp16 && backward) ? 4 : 0);
+            auto level_b = [&](int q) {
+                std::uint64_t off = 0;
+                for (int t = pos(q); t < pos(q + 1); ++t) {
+                    const int row = backward ? m - 1 - t : t;
+                    off += ptr[row + 1] - ptr[row];
+                }
+                return off * (4 + vf) + static_cast<std::uint64_t>(rows(q)) * row_b +
+                       12ull * static_cast<std::uint64_t>(lf[q + 1] - lf[q]);
+            };
+            const int di = backward ? 1 : 0;
+            for (std::size_t k = 0; k < seg[0].size(); ++k) {
+                const bool tail = sc.kind == 2 && seg[2][k];
+                for (int q = seg[0][k]; q < seg[1][k]; ++q) {
+                    if (tail) stats_.tail_bytes[di] += level_b(q);
+                    else { stats_.level_bytes[di] += level_b(q); ++stats_.level_launches[di]; }
+                }
+                if (tail) ++stats_.tail_runs[di];
+            }
+        }
         sc.level_first = lf.data();
         sc.level_first_dev = lf_dev.as<int>();
         sc.nlevels = L;
@@ -671,10 +690,10 @@ void gpu_solver::enqueue_precondition(bool first, int step, hipStream_t s) const
     const vec_prec pr = d.prec;
     if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
     tm.time(0, step, s, [&] {
-        launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, d.f_sched, s, tm.hook(step, s));
+        launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, d.f_sched, s, tm.hook(step));
     });
     tm.time(1, step, s, [&] {
-        launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s, tm.hook(step, s));
+        launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s, tm.hook(step));
     });
     tm.time(3, step, s, [&] {
         launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
@@ -690,10 +709,8 @@ void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
     pcg_state* st = d.st.as<pcg_state>();
     double* part = d.part.as<double>();
     unsigned* cnt = d.counter.as<unsigned>();
-    tm.time(2, it - 1, s, [&] {
-        launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
-                    d.p.as<double>(), d.ap.as<double>(), st, part, cnt, true, s);
-    });
+    launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes, d.p.as<double>(),
+                d.ap.as<double>(), st, part, cnt, true, s, tm.hook(it - 1));
     tm.time(3, it - 1, s, [&] {
         launch_update_xr(d.prec, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
                          d.y.get(), d.z.get(), st, part, cnt, s);

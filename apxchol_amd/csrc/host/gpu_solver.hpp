@@ -55,6 +55,13 @@ struct solver_stats {
     bool op_fp32 = false;
     int spmv_lanes = 0, dataflow_grid = 0;
     std::uint64_t device_bytes = 0;
+    // Per direction (0 forward, 1 back) of the hybrid / level schedule: the
+    // algorithmic HBM bytes of one sweep's level launches and tail runs, and
+    // their kernel counts. Bytes of a level: off-diagonals x (4 index + value
+    // width) + rows x (4 pointer + 4 diagonal + input + output width [+ 4
+    // inverse scale on the fp16 back sweep]) + 12 per batch (batch table).
+    std::uint64_t level_bytes[2] = {0, 0}, tail_bytes[2] = {0, 0};
+    std::int64_t level_launches[2] = {0, 0}, tail_runs[2] = {0, 0};
 };
 
 class gpu_solver {

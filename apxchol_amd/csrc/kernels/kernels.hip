@@ -2,6 +2,8 @@
 // instantiate once).
 #include "launch.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <stdexcept>
 #include <string>
 
@@ -11,6 +13,15 @@
 namespace apxchol_amd {
 
 namespace {
+
+// Launch `kernel`, carrying the hook's timing events when it hands some out.
+template <class K, class... A>
+void launch_timed(const launch_hook& hook, int kind, K kernel, dim3 grid, dim3 block, unsigned lds, hipStream_t s,
+                  A... args) {
+    hipEvent_t ev[2];
+    if (hook.events(kind, ev)) hipExtLaunchKernelGGL(kernel, grid, block, lds, s, ev[0], ev[1], 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
 
 void check_launch(const char* what) {
     const hipError_t e = hipGetLastError();
@@ -43,14 +54,11 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         if (b1 <= b0) return;
         int g = (b1 - b0 + kWavesPerBlock - 1) / kWavesPerBlock;
         if (g > kLevelMaxBlocks) g = kLevelMaxBlocks;
-        hook(0, 0);
-        sweep_levels<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b0, b1, in, o, st, err);
-        hook(0, 1);
+        launch_timed(hook, 0, sweep_levels<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, b0, b1, in, o, st, err);
     };
     if (sc.kind == 0) {
-        hook(3, 0);
-        sweep_dataflow<TI, TO, F16><<<sc.grid, kBlock, 0, s>>>(d, in, o, st, sc.grid * kWavesPerBlock, err);
-        hook(3, 1);
+        launch_timed(hook, 3, sweep_dataflow<TI, TO, F16>, dim3(sc.grid), dim3(kBlock), 0, s, d, in, o, st,
+                     sc.grid * kWavesPerBlock, err);
         check_launch("sweep_dataflow");
     } else if (sc.kind == 1) {
         for (int l = 0; l < sc.nlevels; ++l) level_launch(l);
@@ -74,14 +82,12 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                     throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
                 tail_val<TO>* tv = static_cast<tail_val<TO>*>(sc.tail_val);
                 const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
-                hook(2, 0);
-                sweep_couple<TI, TO, F16><<<g, kBlock, 0, s>>>(d, bf, bl, p0, in, o, tv + static_cast<size_t>(base) * kWave, st);
-                hook(2, 1);
-                hook(1, 0);
-                sweep_tail<TO><<<1, kTailBlock, lds, s>>>(sc.tail, base, sc.level_first_dev, sc.seg_l0[k], sc.seg_l1[k],
-                                                          p1 - p0, d.backward ? d.m - 1 - p0 : p0, d.backward, o,
-                                                          tv, st);
-                hook(1, 1);
+                launch_timed(hook, 2, sweep_couple<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, bf, bl, p0, in,
+                             static_cast<const TO*>(o), tv + static_cast<size_t>(base) * kWave, st);
+                launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
+                             base, sc.level_first_dev, sc.seg_l0[k], sc.seg_l1[k], p1 - p0,
+                             d.backward ? d.m - 1 - p0 : p0, static_cast<int>(d.backward), o,
+                             static_cast<const tail_val<TO>*>(tv), st);
             } else {
                 for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
             }
@@ -125,25 +131,30 @@ void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_s
 
 template <class VT>
 static void spmv_t(int n, const unsigned* ptr, const int* idx, const VT* val, int lanes, const double* x, double* y,
-                   pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s) {
+                   pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
+                   const launch_hook& hook) {
     const int g = vec_blocks(static_cast<long long>(n) * lanes);
     const int w = want_dot ? 1 : 0;
+    auto go = [&](auto kernel) {
+        launch_timed(hook, 4, kernel, dim3(g), dim3(kBlock), 0, s, n, ptr, idx, val, x, y, st, part, counter, w);
+    };
     switch (lanes) {
-        case 1: spmv_pAp<VT, 1><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
-        case 2: spmv_pAp<VT, 2><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
-        case 4: spmv_pAp<VT, 4><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
-        case 8: spmv_pAp<VT, 8><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
-        case 16: spmv_pAp<VT, 16><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
-        case 32: spmv_pAp<VT, 32><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
-        default: spmv_pAp<VT, 64><<<g, kBlock, 0, s>>>(n, ptr, idx, val, x, y, st, part, counter, w); break;
+        case 1: go(spmv_pAp<VT, 1>); break;
+        case 2: go(spmv_pAp<VT, 2>); break;
+        case 4: go(spmv_pAp<VT, 4>); break;
+        case 8: go(spmv_pAp<VT, 8>); break;
+        case 16: go(spmv_pAp<VT, 16>); break;
+        case 32: go(spmv_pAp<VT, 32>); break;
+        default: go(spmv_pAp<VT, 64>); break;
     }
     check_launch("spmv_pAp");
 }
 
 void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
-                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s) {
-    if (val32) spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, st, part, counter, want_dot, s);
-    else spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, st, part, counter, want_dot, s);
+                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
+                 const launch_hook& hook) {
+    if (val32) spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, st, part, counter, want_dot, s, hook);
+    else spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, st, part, counter, want_dot, s, hook);
 }
 
 void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,

@@ -39,12 +39,14 @@ struct sweep_sched {
     size_t tail_val_bytes = 0;
 };
 
-// Called around every kernel launch of a sweep (timing): kind 0 sweep_levels,
-// 1 sweep_tail, 2 sweep_couple, 3 sweep_dataflow; phase 0 before, 1 after.
+// Kernel timing: for kernel `kind` (0 sweep_levels, 1 sweep_tail,
+// 2 sweep_couple, 3 sweep_dataflow, 4 spmv_pAp) the hook may hand out a
+// start / stop event pair, which the launch carries (hipExtLaunchKernelGGL:
+// the events take the kernel's own start and end times).
 struct launch_hook {
-    void (*fn)(void* ctx, int kind, int phase) = nullptr;
+    bool (*fn)(void* ctx, int kind, hipEvent_t* ev) = nullptr;
     void* ctx = nullptr;
-    void operator()(int kind, int phase) const { if (fn) fn(ctx, kind, phase); }
+    bool events(int kind, hipEvent_t* ev) const { return fn && fn(ctx, kind, ev); }
 };
 
 // One sweep. rhs: fp64 r (forward) or the forward output (back, fp64 or fp32
@@ -56,7 +58,8 @@ void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, v
 void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s);
 
 void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
-                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s);
+                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
+                 const launch_hook& hook = {});
 
 void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,
                          hipStream_t s);

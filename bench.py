@@ -10,8 +10,14 @@ fp64-sweep path is timed beside it. Every solve is graded by the true residual
 ||b - A x|| / ||b|| recomputed in fp64 on the host, and the first solution is
 compared with the CPU oracle's fp64 solve of the same system.
 
-The roofline object covers the kernel with the largest total time, measured
-live with HIP events around its launches during the timed solves.
+The timed solves replay each PCG iteration as a captured HIP graph. After
+them, a timing pass of --timing-solves solves launches eagerly with HIP events
+around every kernel on the solve's stream; the roofline object covers the
+single kernel (sweep_levels, sweep_tail or the SpMV) with the largest total
+time there: algorithmic bytes per launch (from the solver's layout: per level
+off-diagonals x (4 + value width) + rows x (8 + input + output width) + 12 per
+batch; SpMV nnz x (4 + value width) + 4(n+1) + 16n) over its average launch
+duration.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -28,18 +34,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
-
-
-def sweep_bytes(info: dict, backward: bool, vec_bytes: int, fp16: bool) -> float:
-    """Algorithmic HBM bytes of one sweep: every stored off-diagonal once
-    (index + value), the row pointers, the diagonal, the batch table, the
-    right-hand side read, the output written once and gathered once."""
-    m, off = info["m"], info["offdiag"]
-    nb = info["bck_batches"] if backward else info["fwd_batches"]
-    ent = 6 if fp16 else 8
-    rhs = (vec_bytes if backward else 8)
-    extra = (4 * m) if (fp16 and backward) else 0
-    return off * ent + 4 * (m + 1) + 4 * m + 12 * nb + m * rhs + 2 * m * vec_bytes + extra
 
 
 def spmv_bytes(info: dict) -> float:
@@ -64,6 +58,9 @@ def main() -> int:
     ap.add_argument("--iters-per-check", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--timing-solves", type=int, default=2)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run 3 plain device copies of 256 MiB (calibration of the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve path is single-GPU (--gpus {args.gpus} is not supported)",
@@ -97,9 +94,6 @@ def main() -> int:
         t_setup = time.perf_counter() - t0
         for _ in range(args.warmup):
             s.solve(bd, out=xd)
-        torch.cuda.synchronize()
-        s.set_timing(True)
-        s.take_timing()
         lat, iters, res = [], [], []
         torch.cuda.synchronize()
         t_start = time.perf_counter()
@@ -111,10 +105,18 @@ def main() -> int:
             res.append(r.residual)
         torch.cuda.synchronize()
         total = time.perf_counter() - t_start
-        timing = s.take_timing()
-        s.set_timing(False)
         x = xd.cpu().numpy()
         true_res = float(np.linalg.norm(b - A @ x) / bnorm)
+        # Kernel timing pass (HIP events around every kernel on the solve's
+        # stream; the solves launch eagerly while it is on).
+        s.set_timing(True)
+        s.take_timing()
+        torch.cuda.synchronize()
+        for _ in range(args.timing_solves):
+            s.solve(bd, out=xd)
+        torch.cuda.synchronize()
+        timing = s.take_timing()
+        s.set_timing(False)
         info = dict(s.info)
         s.close()
         return dict(total=total, lat=lat, iters=iters, rec_res=res, x=x, true_res=true_res, timing=timing,
@@ -128,20 +130,41 @@ def main() -> int:
         print(f"[bench] fp64:  {args.steps / plain['total']:.2f} solves/s iters={plain['iters'][0]} "
               f"true_res={plain['true_res']:.2e}", file=sys.stderr, flush=True)
 
-    # Roofline of the dominant kernel of the mixed solve (HIP events, live).
+    # Roofline of the dominant single kernel of the mixed solve, from the
+    # event-timed pass: algorithmic bytes per launch / average launch time.
     info = mixed["info"]
-    fp16 = bool(info["fp16_storage"])
     tm = mixed["timing"]
-    kb = {"fwd_sweep": sweep_bytes(info, False, 4, fp16), "bck_sweep": sweep_bytes(info, True, 4, fp16),
-          "spmv": spmv_bytes(info)}
-    dom = max(kb, key=lambda k: tm[k][0])
-    ms_total, launches = tm[dom]
-    avg_s = ms_total / 1e3 / max(launches, 1)
-    achieved = kb[dom] / avg_s / 1e9
-    per_kernel = {k: {"avg_us": tm[k][0] * 1e3 / max(tm[k][1], 1), "launches": tm[k][1],
-                      "bytes": kb[k], "gbps": kb[k] / (tm[k][0] / 1e3 / max(tm[k][1], 1)) / 1e9 if tm[k][1] else 0.0}
-                  for k in kb}
-    per_kernel["vector"] = {"avg_us": tm["vector"][0] * 1e3 / max(tm["vector"][1], 1), "launches": tm["vector"][1]}
+    lb, ll = info["level_bytes"], info["level_launches"]
+    tb, tr = info["tail_bytes"], info["tail_runs"]
+    per_launch_bytes = {
+        "sweep_levels": (lb[0] + lb[1]) / max(ll[0] + ll[1], 1),
+        "sweep_tail": (tb[0] + tb[1]) / max(tr[0] + tr[1], 1),
+        "spmv": spmv_bytes(info),
+    }
+    per_kernel = {}
+    for k, bpl in per_launch_bytes.items():
+        ms, cnt = tm[k]
+        avg = ms / 1e3 / cnt if cnt else float("nan")
+        per_kernel[k] = {"avg_us": avg * 1e6, "launches": cnt, "total_ms": ms, "bytes_per_launch": bpl,
+                         "gbps": bpl / avg / 1e9 if cnt else 0.0}
+    for k in ("sweep_couple", "vector", "fwd_sweep", "bck_sweep"):
+        ms, cnt = tm[k]
+        per_kernel[k] = {"avg_us": ms * 1e3 / max(cnt, 1), "launches": cnt, "total_ms": ms}
+    dom = max(per_launch_bytes, key=lambda k: tm[k][0])
+    achieved = per_kernel[dom]["gbps"]
+    # HBM bytes per launch of that kernel from the PMC passes recorded with
+    # the profiles (FETCH_SIZE / WRITE_SIZE, copy-calibrated), when present.
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    if tf.exists():
+        tj = json.loads(tf.read_text())
+        if tj.get("kernel") == dom:
+            traffic = tj["bytes_per_launch"]
+    # Whole-sweep SpTRSV rate (level launches + tail runs, gaps included).
+    sweep_gbps = {}
+    for di, k in ((0, "fwd_sweep"), (1, "bck_sweep")):
+        ms, cnt = tm[k]
+        sweep_gbps[k] = (lb[di] + tb[di]) / (ms / 1e3 / max(cnt, 1)) / 1e9 if cnt else 0.0
 
     cpu = None
     x_cpu_err = None
@@ -177,8 +200,9 @@ def main() -> int:
         "config": {"workload": args.workload, "n": n, "nnz": int(A.nnz), "tol": 1e-8, "factor_seed": 0,
                    "factor_nnz": F.nnz, "levels": info["levels"], "rhs": "N(0,1) seed 0"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "algorithmic_bytes": kb[dom],
-                     "avg_launch_us": avg_s * 1e6},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": per_kernel[dom]["bytes_per_launch"],
+                     "avg_launch_us": per_kernel[dom]["avg_us"]},
         "cpu_baseline": cpu,
         "detail": {
             "mixed": {"solves_per_s": value, "p50_ms": 1e3 * percentile(mixed["lat"], 50),
@@ -191,11 +215,19 @@ def main() -> int:
                 "true_residual": plain["true_res"], "setup_s": plain["setup"]},
             "mixed_over_fp64": None if not plain else value / (args.steps / plain["total"]),
             "kernels_mixed": per_kernel,
+            "sptrsv_sweep_gbps_mixed": sweep_gbps,
             "host_factorization_s": t_fact,
             "solver_info": info,
         },
     }
     print(json.dumps(out))
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] copy probe: 3 x 256 MiB device copies", file=sys.stderr, flush=True)
     return 0
 
 

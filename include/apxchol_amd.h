@@ -69,6 +69,12 @@ typedef struct {
     int64_t rounds, levels, fwd_batches, bck_batches, op_nnz;
     int32_t op_fp32, spmv_lanes, dataflow_grid, fp16_storage;
     uint64_t device_bytes;
+    /* Per direction [0 forward, 1 back], level / hybrid schedules: algorithmic
+     * HBM bytes of one sweep's level launches and tail runs and their kernel
+     * counts (off-diagonals x (4 + value width) + rows x (8 + input + output
+     * width) + 12 per batch). */
+    uint64_t level_bytes[2], tail_bytes[2];
+    int64_t level_launches[2], tail_runs[2];
 } apx_solver_info;
 
 const char* apx_last_error(void);
