@@ -62,6 +62,9 @@ class SolverInfo(ctypes.Structure):
 _SIGS = {
     "apx_last_error": ([], ctypes.c_char_p),
     "apx_options_default": ([P], None),
+    "apx_solver_solve_batch": ([P, I32, P, P, F64, I32, P, P], ctypes.c_int),
+    "apx_solver_apply_batch": ([P, I32, P, P, P], ctypes.c_int),
+    "apx_solver_solve_batch_host": ([P, I32, P, P, F64, I32, P], ctypes.c_int),
     "apx_fp16_round": ([ctypes.c_float], ctypes.c_uint16),
     "apx_fp16_widen": ([ctypes.c_uint16], ctypes.c_float),
     "apx_factorize": ([I64, P, P, P, U32, P], ctypes.c_int),
@@ -292,6 +295,42 @@ class Solver:
             _check(lib.apx_solver_solve(self._h, _ptr(b), _ptr(x), _ptr(x0), tol, maxiter, ctypes.byref(info), s))
         return SolveResult(x, int(info.iterations), float(info.residual), int(info.status),
                            int(info.lumped_offdiag), float(info.solve_vram_mb))
+
+    def solve_batch(self, B, tol: float | None = None, maxiter: int | None = None, out=None,
+                    stream=None) -> tuple:
+        """k right-hand sides (rows of B, shape (k, n)) in one blocked PCG.
+        Returns (X, [SolveResult per column]) -- X of B's kind (numpy array on
+        the host, or a torch tensor on the device)."""
+        lib = load_library()
+        tol = -1.0 if tol is None else float(tol)
+        maxiter = -1 if maxiter is None else int(maxiter)
+        if isinstance(B, np.ndarray):
+            B = np.ascontiguousarray(B, dtype=np.float64)
+            k = B.shape[0]
+            X = np.zeros_like(B) if out is None else out
+            infos = (SolveInfo * max(k, 1))()
+            _check(lib.apx_solver_solve_batch_host(self._h, k, B.ctypes.data, X.ctypes.data, tol, maxiter, infos))
+        else:
+            import torch
+            B = B.contiguous()
+            k = B.shape[0]
+            X = torch.empty_like(B) if out is None else out
+            infos = (SolveInfo * max(k, 1))()
+            s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+            _check(lib.apx_solver_solve_batch(self._h, k, _ptr(B), _ptr(X), tol, maxiter, infos, s))
+        res = [SolveResult(None, int(i.iterations), float(i.residual), int(i.status), int(i.lumped_offdiag),
+                           float(i.solve_vram_mb)) for i in list(infos)[:k]]
+        return X, res
+
+    def apply_batch(self, R, out=None, stream=None):
+        """Preconditioner applied to the k rows of the device tensor R (k, n)
+        at once (one application of the centring schedule)."""
+        import torch
+        R = R.contiguous()
+        Z = torch.empty_like(R) if out is None else out
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().apx_solver_apply_batch(self._h, R.shape[0], _ptr(R), _ptr(Z), s))
+        return Z
 
     def apply(self, r, out=None, stream=None):
         lib = load_library()

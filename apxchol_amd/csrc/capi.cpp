@@ -268,6 +268,42 @@ apx_status apx_solver_solve_host(apx_solver_t s, const double* b, double* x, con
     });
 }
 
+apx_status apx_solver_solve_batch(apx_solver_t s, int32_t k, const double* d_B, double* d_X, double tol,
+                                  int32_t max_iter, apx_solve_info* infos, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        if (k < 0) throw std::invalid_argument("k must be >= 0");
+        if (k > 0 && s->s->rows() > 0) { need(d_B, "B"); need(d_X, "X"); }
+        const auto r = s->s->solve_batch_device(k, d_B, d_X, tol, max_iter, static_cast<hipStream_t>(stream));
+        if (infos)
+            for (int32_t c = 0; c < k; ++c) fill_info(r[c], infos + c);
+    });
+}
+
+apx_status apx_solver_solve_batch_host(apx_solver_t s, int32_t k, const double* B, double* X, double tol,
+                                       int32_t max_iter, apx_solve_info* infos) {
+    return guarded([&] {
+        need(s, "solver");
+        if (k < 0) throw std::invalid_argument("k must be >= 0");
+        const std::size_t total = static_cast<std::size_t>(k) * static_cast<std::size_t>(s->s->rows());
+        if (total > 0) { need(B, "B"); need(X, "X"); }
+        std::vector<double> bv(B, B + total), xv;
+        const auto r = s->s->solve_batch(k, bv, xv, tol, max_iter);
+        if (total > 0) std::memcpy(X, xv.data(), total * sizeof(double));
+        if (infos)
+            for (int32_t c = 0; c < k; ++c) fill_info(r[c], infos + c);
+    });
+}
+
+apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, double* d_Z, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        if (k < 0) throw std::invalid_argument("k must be >= 0");
+        if (k > 0 && s->s->rows() > 0) { need(d_R, "R"); need(d_Z, "Z"); }
+        s->s->apply_batch_device(k, d_R, d_Z, static_cast<hipStream_t>(stream));
+    });
+}
+
 apx_status apx_solver_apply(apx_solver_t s, const double* d_r, double* d_z, void* stream) {
     return guarded([&] {
         need(s, "solver");

@@ -62,6 +62,13 @@ struct pcg_state {
     int pad_i[2];
 };
 
+// Bookkeeping of a blocked (k right-hand sides) solve: the number of columns
+// still iterating; every blocked kernel returns at once when it is zero.
+struct batch_state {
+    int active;
+    int pad[15];
+};
+
 // Static layout of the TAIL RUNS: maximal runs of small levels that one
 // workgroup walks with the run's outputs in LDS. Everything a lane needs for a
 // batch of a run sits at an address computed from (batch, lane) alone, so the

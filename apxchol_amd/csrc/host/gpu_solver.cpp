@@ -61,6 +61,9 @@ int lanes_for_host(std::uint32_t len) {
     return p > 64 ? 64 : static_cast<int>(p);
 }
 
+// Workgroup cap of the vector kernels (their partial buffers are sized by it).
+constexpr int kMaxBlocksHost = 2048;
+
 int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
     if (!e || !*e) return dflt;
@@ -328,6 +331,30 @@ struct gpu_solver::device_state {
     hipStream_t cap = nullptr;
     hipGraph_t iter_graph = nullptr;
     hipGraphExec_t iter_exec = nullptr;
+    // Blocked solves: first sweep position of every level per direction, and
+    // the block buffers (allocated on first use for the widest block seen).
+    std::vector<int> f_lpos, b_lpos;
+    struct block_bufs {
+        int kp = 0;
+        dbuf r, x, p, ap, y, z, st, bs, part, counter;
+        pcg_state* host_st = nullptr;
+        batch_state* host_bs = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int graph_kp = 0;
+        void drop_graph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            graph_kp = 0;
+        }
+        ~block_bufs() {
+            drop_graph();
+            if (host_st) (void)hipHostFree(host_st);
+            if (host_bs) (void)hipHostFree(host_bs);
+        }
+    } blk;
     ~device_state() {
         if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
         if (iter_graph) (void)hipGraphDestroy(iter_graph);
@@ -629,6 +656,8 @@ void gpu_solver::setup(const csc_view& L) {
     dbuf* b_tb[4] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent};
     make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb);
     make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb);
+    for (const int l : fb.level_first) d.f_lpos.push_back(static_cast<int>(fb.pos[l]));
+    for (const int l : bb.level_first) d.b_lpos.push_back(static_cast<int>(bb.pos[l]));
     // Coupling output, shared by the two directions (their sweeps never overlap).
     const std::size_t tail_batches = std::max<std::size_t>(
         1, static_cast<std::size_t>(std::max(d.f_sched.tail_batches, d.b_sched.tail_batches)));
@@ -797,6 +826,209 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     res.status = h.status;
     std::size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess && tot >= fr) res.solve_vram_mb = static_cast<double>(tot - fr) / (1024.0 * 1024.0);
+    return res;
+}
+
+// ---- blocked solves -----------------------------------------------------
+
+// Block width for k columns (a power of two <= 64), with the block buffers
+// grown to it.
+int gpu_solver::prepare_block(int k) const {
+    auto& b = dev_->blk;
+    int kp = 1;
+    while (kp < k) kp <<= 1;
+    if (kp > 64) throw std::invalid_argument("gpu_solver: a block holds at most 64 columns");
+    if (b.kp >= kp) return kp;
+    b.drop_graph();
+    const std::size_t vb = static_cast<std::size_t>(std::max<std::int64_t>(n_, 1)) * kp * sizeof(double);
+    b.r.alloc(vb); b.x.alloc(vb); b.p.alloc(vb); b.ap.alloc(vb); b.y.alloc(vb); b.z.alloc(vb);
+    b.st.alloc(static_cast<std::size_t>(kp) * sizeof(pcg_state));
+    b.bs.alloc(sizeof(batch_state));
+    b.part.alloc(static_cast<std::size_t>(kMaxBlocksHost) * 2 * kp * sizeof(double));
+    b.counter.alloc(64);
+    b.counter.zero();
+    if (b.host_st) (void)hipHostFree(b.host_st);
+    b.host_st = nullptr;
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&b.host_st), static_cast<std::size_t>(kp) * sizeof(pcg_state),
+                            hipHostMallocDefault), "hipHostMalloc");
+    if (!b.host_bs)
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&b.host_bs), sizeof(batch_state), hipHostMallocDefault),
+                  "hipHostMalloc");
+    b.kp = kp;
+    return kp;
+}
+
+void gpu_solver::apply_block(int k, const double* d_R, double* d_Z, bool center, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    const int kp = prepare_block(k);
+    for (int c = 0; c < kp; ++c) {
+        std::memset(&b.host_st[c], 0, sizeof(pcg_state));
+        b.host_st[c].n = n;
+    }
+    std::memset(b.host_bs, 0, sizeof(batch_state));
+    b.host_bs->active = kp;
+    pcg_state* st = b.st.as<pcg_state>();
+    batch_state* bs = b.bs.as<batch_state>();
+    double* part = b.part.as<double>();
+    unsigned* cnt = b.counter.as<unsigned>();
+    hip_check(hipMemcpyAsync(st, b.host_st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyHostToDevice, s),
+              "state upload");
+    hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
+    launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_R, b.r.as<double>(), b.x.as<double>(), s);
+    launch_zero_tail_k(d.prec, n, m, kp, b.z.get(), s);
+    if (center) {
+        launch_sum_cols_k(vec_prec::fp64, n, kp, b.r.get(), st, part, cnt, s);
+        launch_shift_from_sum_k(kp, st, true, s);
+    }
+    const int nlf = static_cast<int>(d.f_lpos.size()) - 1, nlb = static_cast<int>(d.b_lpos.size()) - 1;
+    launch_sweep_k(d.fwd, d.prec, d.fp16, kp, b.r.get(), b.y.get(), st, bs, d.f_lpos.data(), nlf, s);
+    launch_sweep_k(d.bck, d.prec, d.fp16, kp, b.y.get(), b.z.get(), st, bs, d.b_lpos.data(), nlb, s);
+    if (center) launch_sum_cols_k(d.prec, n, kp, b.z.get(), st, part, cnt, s);
+    launch_gather_k(d.prec, n, k, kp, d.perm.as<unsigned>(), b.z.get(), d_Z, st, center, s);
+}
+
+void gpu_solver::apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const {
+    if (k < 0) throw std::invalid_argument("gpu_solver::apply_batch: negative column count");
+    if (n_ == 0 || k == 0) return;
+    // One application of the center-k schedule for all columns.
+    const bool center = !F_.sddm && (++apply_count_ % static_cast<std::uint64_t>(center_k_) == 0);
+    for (int c0 = 0; c0 < k; c0 += 64)
+        apply_block(std::min(64, k - c0), d_R + static_cast<std::size_t>(c0) * n_,
+                    d_Z + static_cast<std::size_t>(c0) * n_, center, s);
+}
+
+void gpu_solver::enqueue_precondition_k(bool first, int kp, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int n = static_cast<int>(n_);
+    pcg_state* st = b.st.as<pcg_state>();
+    batch_state* bs = b.bs.as<batch_state>();
+    const int nlf = static_cast<int>(d.f_lpos.size()) - 1, nlb = static_cast<int>(d.b_lpos.size()) - 1;
+    launch_sweep_k(d.fwd, d.prec, d.fp16, kp, b.r.get(), b.y.get(), st, bs, d.f_lpos.data(), nlf, s);
+    launch_sweep_k(d.bck, d.prec, d.fp16, kp, b.y.get(), b.z.get(), st, bs, d.b_lpos.data(), nlb, s);
+    launch_dot_center_k(d.prec, n, kp, b.r.as<double>(), b.z.get(), st, bs, b.part.as<double>(),
+                        b.counter.as<unsigned>(), first, s);
+    launch_update_p_k(d.prec, n, kp, b.p.as<double>(), b.z.get(), st, bs, first, s);
+}
+
+void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int n = static_cast<int>(n_);
+    pcg_state* st = b.st.as<pcg_state>();
+    batch_state* bs = b.bs.as<batch_state>();
+    launch_spmm_k(n, kp, d.lanes, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32,
+                  b.p.as<double>(), b.ap.as<double>(), st, bs, b.part.as<double>(), b.counter.as<unsigned>(), true, s);
+    launch_update_xr_k(n, kp, b.x.as<double>(), b.p.as<double>(), b.r.as<double>(), b.ap.as<double>(), st, bs,
+                       b.part.as<double>(), b.counter.as<unsigned>(), s);
+    launch_refresh_active_k(kp, st, bs, s);
+    enqueue_precondition_k(false, kp, s);
+}
+
+std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
+                                                  hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    const int kp = prepare_block(k);
+    for (int c = 0; c < kp; ++c) {
+        pcg_state& h = b.host_st[c];
+        std::memset(&h, 0, sizeof h);
+        h.tol = tol;
+        h.max_iter = max_iter;
+        h.stag_window = opts_.stagnation_window;
+        h.center_k = center_k_;
+        h.laplacian = F_.sddm ? 0 : 1;
+        h.n = n;
+        h.prev_check = 1.0;
+    }
+    std::memset(b.host_bs, 0, sizeof(batch_state));
+    b.host_bs->active = kp;
+    pcg_state* st = b.st.as<pcg_state>();
+    batch_state* bs = b.bs.as<batch_state>();
+    hip_check(hipMemcpyAsync(st, b.host_st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyHostToDevice, s),
+              "state upload");
+    hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
+    launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_B, b.r.as<double>(), b.x.as<double>(), s);
+    launch_zero_tail_k(d.prec, n, m, kp, b.z.get(), s);
+    launch_init_residual_k(n, kp, b.r.as<double>(), st, bs, b.part.as<double>(), b.counter.as<unsigned>(), s);
+    launch_refresh_active_k(kp, st, bs, s);
+    enqueue_precondition_k(true, kp, s);
+    const int J = std::max(1, opts_.iters_per_check);
+    if (opts_.graph && b.graph_kp != kp) {
+        b.drop_graph();
+        if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            enqueue_iteration_k(kp, d.cap);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(d.cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(d.cap, &b.graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        b.graph_kp = kp;
+    }
+    int enqueued = 0;
+    for (;;) {
+        for (int j = 0; j < J; ++j) {
+            if (opts_.graph) hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+            else enqueue_iteration_k(kp, s);
+        }
+        enqueued += J;
+        hip_check(hipMemcpyAsync(b.host_bs, bs, sizeof(batch_state), hipMemcpyDeviceToHost, s), "state read");
+        hip_check(hipStreamSynchronize(s), "solve");
+        if (b.host_bs->active == 0 || enqueued > max_iter + J) break;
+    }
+    if (!F_.sddm)
+        launch_sum_cols_k(vec_prec::fp64, n, kp, b.x.get(), st, b.part.as<double>(), b.counter.as<unsigned>(), s);
+    launch_gather_k(vec_prec::fp64, n, k, kp, d.perm.as<unsigned>(), b.x.get(), d_X, st, !F_.sddm, s);
+    hip_check(hipMemcpyAsync(b.host_st, st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyDeviceToHost, s),
+              "state read");
+    hip_check(hipStreamSynchronize(s), "solve");
+    std::vector<solve_result> res(static_cast<std::size_t>(k));
+    for (int c = 0; c < k; ++c) {
+        res[c].iterations = b.host_st[c].iter;
+        res[c].residual = b.host_st[c].residual;
+        res[c].status = b.host_st[c].status;
+        res[c].lumped_offdiag = F_.lumped_offdiag;
+    }
+    return res;
+}
+
+std::vector<solve_result> gpu_solver::solve_batch_device(int k, const double* d_B, double* d_X, double tol,
+                                                         int max_iter, hipStream_t s) const {
+    if (k < 0) throw std::invalid_argument("gpu_solver::solve_batch: negative column count");
+    if (tol < 0.0) tol = opts_.tol;
+    if (max_iter < 0) max_iter = opts_.max_iter;
+    std::vector<solve_result> res;
+    if (n_ == 0 || k == 0) { res.resize(static_cast<std::size_t>(k)); return res; }
+    for (int c0 = 0; c0 < k; c0 += 64) {
+        const int kc = std::min(64, k - c0);
+        auto part = solve_block(kc, d_B + static_cast<std::size_t>(c0) * n_, d_X + static_cast<std::size_t>(c0) * n_,
+                                tol, max_iter, s);
+        res.insert(res.end(), part.begin(), part.end());
+    }
+    check_sweep_error();
+    return res;
+}
+
+std::vector<solve_result> gpu_solver::solve_batch(int k, const std::vector<double>& B, std::vector<double>& X,
+                                                  double tol, int max_iter) const {
+    const std::size_t total = static_cast<std::size_t>(k) * static_cast<std::size_t>(n_);
+    if (k < 0 || B.size() != total) throw std::invalid_argument("gpu_solver::solve_batch: B must hold k * n values");
+    X.assign(total, 0.0);
+    if (total == 0) return solve_batch_device(k, nullptr, nullptr, tol, max_iter, nullptr);
+    dbuf db(total * sizeof(double)), dx(total * sizeof(double));
+    hip_check(hipMemcpy(db.get(), B.data(), total * sizeof(double), hipMemcpyHostToDevice), "upload B");
+    auto res = solve_batch_device(k, db.as<double>(), dx.as<double>(), tol, max_iter, nullptr);
+    hip_check(hipMemcpy(X.data(), dx.get(), total * sizeof(double), hipMemcpyDeviceToHost), "download X");
     return res;
 }
 

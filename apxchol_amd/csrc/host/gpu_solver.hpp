@@ -82,6 +82,18 @@ public:
     solve_result solve_device(const double* d_b, double* d_x, double tol, int max_iter, const double* d_x0,
                               hipStream_t s) const;
     void apply_device(const double* d_r, double* d_z, hipStream_t s) const;
+    // k right-hand sides in blocked PCG (no counterpart in the reference:
+    // k independent cpu_solver::solve(b) calls, solve.h:83-100, run as one):
+    // column c of B / X is n doubles at d_B + c * n / d_X + c * n. Every
+    // column keeps its own convergence test and stops like its own solve;
+    // the sweeps and SpMM give each column the single-vector kernels' bits.
+    std::vector<solve_result> solve_batch_device(int k, const double* d_B, double* d_X, double tol, int max_iter,
+                                                 hipStream_t s) const;
+    std::vector<solve_result> solve_batch(int k, const std::vector<double>& B, std::vector<double>& X,
+                                          double tol = -1.0, int max_iter = -1) const;
+    // k preconditioner applications at once (k calls of cpu_solver::apply,
+    // solve.h:103, counting as ONE application of the center-k schedule).
+    void apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const;
     // One sweep in factor order with fp64 vectors (rhs, out: m doubles).
     void sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const;
 
@@ -110,6 +122,12 @@ private:
     // preconditioned dot and the p update); step labels the timing records.
     void enqueue_iteration(int step, hipStream_t s) const;
     void enqueue_precondition(bool first, int step, hipStream_t s) const;
+    std::vector<solve_result> solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
+                                          hipStream_t s) const;
+    int prepare_block(int k) const;
+    void apply_block(int k, const double* d_R, double* d_Z, bool center, hipStream_t s) const;
+    void enqueue_iteration_k(int kp, hipStream_t s) const;
+    void enqueue_precondition_k(bool first, int kp, hipStream_t s) const;
 
     solve_options opts_;
     factorization F_;

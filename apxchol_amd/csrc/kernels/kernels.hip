@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "block.hip"
 #include "pcg.hip"
 #include "sweep.hip"
 
@@ -244,6 +245,135 @@ void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s) {
     if (p == vec_prec::fp64) fill_pending<double><<<g, kBlock, 0, s>>>(static_cast<double*>(a), n, nullptr);
     else fill_pending<float><<<g, kBlock, 0, s>>>(static_cast<float*>(a), n, nullptr);
     check_launch("fill_pending");
+}
+
+// ---- blocked solves ---------------------------------------------------------
+
+namespace {
+
+template <class TI, class TO, bool F16>
+void sweep_k_t(const sweep_desc& d, int kp, const void* rhs, void* out, const pcg_state* st, const batch_state* bs,
+               const int* level_pos, int nlevels, hipStream_t s) {
+    for (int l = 0; l < nlevels; ++l) {
+        const int p0 = level_pos[l], p1 = level_pos[l + 1];
+        if (p1 <= p0) continue;
+        const long long threads = static_cast<long long>(p1 - p0) * kp;
+        const int g = static_cast<int>((threads + kBlock - 1) / kBlock);
+        sweep_levels_k<TI, TO, F16><<<g, kBlock, 0, s>>>(d, p0, p1, kp, static_cast<const TI*>(rhs),
+                                                         static_cast<TO*>(out), st, bs);
+    }
+    check_launch("sweep_levels_k");
+}
+
+void check_kp(int kp) {
+    if (kp < 1 || kp > 64 || (kp & (kp - 1)) != 0) throw std::invalid_argument("apxchol_amd: block width must be a power of two <= 64");
+}
+
+}  // namespace
+
+void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, int kp, const void* rhs, void* out,
+                    const pcg_state* st, const batch_state* bs, const int* level_pos, int nlevels, hipStream_t s) {
+    check_kp(kp);
+    if (p == vec_prec::fp64) {
+        if (fp16) sweep_k_t<double, double, true>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+        else sweep_k_t<double, double, false>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+    } else if (!d.backward) {
+        if (fp16) sweep_k_t<double, float, true>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+        else sweep_k_t<double, float, false>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+    } else {
+        if (fp16) sweep_k_t<float, float, true>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+        else sweep_k_t<float, float, false>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+    }
+}
+
+void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx, const void* val, bool val32,
+                   const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
+                   bool want_dot, hipStream_t s) {
+    check_kp(kp);
+    const int g = block_grid(n, kp);
+    if (val32)
+        spmm_pAp_k<float><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const float*>(val), x, y, st, bs,
+                                               part, counter, want_dot ? 1 : 0);
+    else
+        spmm_pAp_k<double><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const double*>(val), x, y, st,
+                                                bs, part, counter, want_dot ? 1 : 0);
+    check_launch("spmm_pAp_k");
+}
+
+void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const double* B, double* r, double* x,
+                           hipStream_t s) {
+    scatter_init_k<<<vec_blocks(static_cast<long long>(n) * kp), kBlock, 0, s>>>(n, k, kp, perm, B, r, x);
+    check_launch("scatter_init_k");
+}
+
+void launch_init_residual_k(int n, int kp, const double* r, pcg_state* st, batch_state* bs, double* part,
+                            unsigned* counter, hipStream_t s) {
+    init_residual_k<<<block_grid(n, kp), kBlock, 0, s>>>(n, kp, r, st, bs, part, counter);
+    check_launch("init_residual_k");
+}
+
+void launch_update_xr_k(int n, int kp, double* x, const double* p, double* r, const double* ap, pcg_state* st,
+                        batch_state* bs, double* part, unsigned* counter, hipStream_t s) {
+    update_xr_k<<<block_grid(n, kp), kBlock, 0, s>>>(n, kp, x, p, r, ap, st, bs, part, counter);
+    check_launch("update_xr_k");
+}
+
+void launch_refresh_active_k(int kp, const pcg_state* st, batch_state* bs, hipStream_t s) {
+    refresh_active_k<<<1, 64, 0, s>>>(kp, st, bs);
+    check_launch("refresh_active_k");
+}
+
+void launch_dot_center_k(vec_prec p, int n, int kp, const double* r, const void* out, pcg_state* st, batch_state* bs,
+                         double* part, unsigned* counter, bool first, hipStream_t s) {
+    const int g = block_grid(n, kp);
+    if (p == vec_prec::fp64)
+        dot_center_k<double><<<g, kBlock, 0, s>>>(n, kp, r, static_cast<const double*>(out), st, bs, part, counter,
+                                                  first ? 1 : 0);
+    else
+        dot_center_k<float><<<g, kBlock, 0, s>>>(n, kp, r, static_cast<const float*>(out), st, bs, part, counter,
+                                                 first ? 1 : 0);
+    check_launch("dot_center_k");
+}
+
+void launch_update_p_k(vec_prec p, int n, int kp, double* pv, const void* out, const pcg_state* st, batch_state* bs,
+                       bool first, hipStream_t s) {
+    const int g = block_grid(n, kp);
+    if (p == vec_prec::fp64)
+        update_p_k<double><<<g, kBlock, 0, s>>>(n, kp, pv, static_cast<const double*>(out), st, bs, first ? 1 : 0);
+    else
+        update_p_k<float><<<g, kBlock, 0, s>>>(n, kp, pv, static_cast<const float*>(out), st, bs, first ? 1 : 0);
+    check_launch("update_p_k");
+}
+
+void launch_zero_tail_k(vec_prec p, int n, int m, int kp, void* a, hipStream_t s) {
+    if (n <= m) return;
+    const int g = vec_blocks(static_cast<long long>(n - m) * kp);
+    if (p == vec_prec::fp64) zero_tail_k<double><<<g, kBlock, 0, s>>>(n, m, kp, static_cast<double*>(a));
+    else zero_tail_k<float><<<g, kBlock, 0, s>>>(n, m, kp, static_cast<float*>(a));
+    check_launch("zero_tail_k");
+}
+
+void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, unsigned* counter,
+                       hipStream_t s) {
+    const int g = block_grid(n, kp);
+    if (p == vec_prec::fp64) sum_cols_k<double><<<g, kBlock, 0, s>>>(n, kp, static_cast<const double*>(x), st, part, counter);
+    else sum_cols_k<float><<<g, kBlock, 0, s>>>(n, kp, static_cast<const float*>(x), st, part, counter);
+    check_launch("sum_cols_k");
+}
+
+void launch_shift_from_sum_k(int kp, pcg_state* st, bool center, hipStream_t s) {
+    shift_from_sum_k<<<1, 64, 0, s>>>(kp, st, center ? 1 : 0);
+    check_launch("shift_from_sum_k");
+}
+
+void launch_gather_k(vec_prec p, int n, int k, int kp, const unsigned* perm, const void* x, double* X,
+                     const pcg_state* st, bool use_mean, hipStream_t s) {
+    const int g = vec_blocks(static_cast<long long>(n) * k);
+    if (p == vec_prec::fp64)
+        gather_k<double><<<g, kBlock, 0, s>>>(n, k, kp, perm, static_cast<const double*>(x), X, st, use_mean ? 1 : 0);
+    else
+        gather_k<float><<<g, kBlock, 0, s>>>(n, k, kp, perm, static_cast<const float*>(x), X, st, use_mean ? 1 : 0);
+    check_launch("gather_k");
 }
 
 }  // namespace apxchol_amd

@@ -85,4 +85,32 @@ void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* ou
                          bool center, hipStream_t s);
 void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s);
 
+// ---- blocked solves: kp columns (a power of two <= 64), row-major blocks ----
+
+// One blocked sweep as one launch per level; level_pos: host array of the
+// levels' first sweep positions (nlevels + 1 entries, in sweep order).
+void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, int kp, const void* rhs, void* out,
+                    const pcg_state* st, const batch_state* bs, const int* level_pos, int nlevels, hipStream_t s);
+void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx, const void* val, bool val32,
+                   const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
+                   bool want_dot, hipStream_t s);
+void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const double* B, double* r, double* x,
+                           hipStream_t s);
+void launch_init_residual_k(int n, int kp, const double* r, pcg_state* st, batch_state* bs, double* part,
+                            unsigned* counter, hipStream_t s);
+void launch_update_xr_k(int n, int kp, double* x, const double* p, double* r, const double* ap, pcg_state* st,
+                        batch_state* bs, double* part, unsigned* counter, hipStream_t s);
+void launch_refresh_active_k(int kp, const pcg_state* st, batch_state* bs, hipStream_t s);
+void launch_dot_center_k(vec_prec p, int n, int kp, const double* r, const void* out, pcg_state* st, batch_state* bs,
+                         double* part, unsigned* counter, bool first, hipStream_t s);
+void launch_update_p_k(vec_prec p, int n, int kp, double* pv, const void* out, const pcg_state* st, batch_state* bs,
+                       bool first, hipStream_t s);
+void launch_zero_tail_k(vec_prec p, int n, int m, int kp, void* a, hipStream_t s);
+// x: fp64 (p == fp64) or fp32 block.
+void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, unsigned* counter,
+                       hipStream_t s);
+void launch_shift_from_sum_k(int kp, pcg_state* st, bool center, hipStream_t s);
+void launch_gather_k(vec_prec p, int n, int k, int kp, const unsigned* perm, const void* x, double* X,
+                     const pcg_state* st, bool use_mean, hipStream_t s);
+
 }  // namespace apxchol_amd

@@ -132,6 +132,26 @@ apx_status apx_solver_solve(apx_solver_t s, const double* d_b, double* d_x, cons
 apx_status apx_solver_solve_host(apx_solver_t s, const double* b, double* x, const double* x0, double tol,
                                  int32_t max_iter, apx_solve_info* info);
 
+/* k right-hand sides in one blocked PCG. No counterpart in the reference: it
+ * stands for k calls of cpu_solver::solve(b) (solve.h:83-100) run together.
+ * Column c of d_B / d_X is n doubles at d_B + c*n / d_X + c*n (device
+ * memory). Every column keeps its own convergence test (tol, max_iter,
+ * stagnation) and stops as its own solve would; the sweeps and the SpMM give
+ * each column the bits of the single-vector kernels, the per-column dot
+ * products are reduced in another fixed order. infos: k entries, or NULL. */
+apx_status apx_solver_solve_batch(apx_solver_t s, int32_t k, const double* d_B, double* d_X, double tol,
+                                  int32_t max_iter, apx_solve_info* infos, void* stream);
+/* Host-array form of apx_solver_solve_batch (B, X: k * n doubles). */
+apx_status apx_solver_solve_batch_host(apx_solver_t s, int32_t k, const double* B, double* X, double tol,
+                                       int32_t max_iter, apx_solve_info* infos);
+
+/* k preconditioner applications z_c = M^-1 r_c at once: k calls of
+ * cpu_solver::apply (solve.h:103) counted as ONE application of the center-k
+ * schedule (no counterpart in the reference). Column c at d_R + c*n /
+ * d_Z + c*n. Non-centring applications give each column the single apply's
+ * bits. */
+apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, double* d_Z, void* stream);
+
 /* cpu_solver::apply(r) = apx_cholesky::_solve_impl (solve.h:103, preconditioner.h:344-353):
  * z = M^-1 r with the center-k schedule (every K-th application centres). */
 apx_status apx_solver_apply(apx_solver_t s, const double* d_r, double* d_z, void* stream);
