@@ -42,7 +42,7 @@ class Options(ctypes.Structure):
     _fields_ = [("tol", F64), ("max_iter", I32), ("stagnation_window", I32), ("seed", U32),
                 ("precision", I32), ("fp16_storage", I32), ("schedule", I32), ("drop_rel", F64),
                 ("keep_factor", I32), ("iters_per_check", I32), ("sweep_chunk", I32), ("sweep_blocks", I32),
-                ("tail_rows", I32), ("graph", I32)]
+                ("tail_rows", I32), ("graph", I32), ("relabel", I32)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -64,6 +64,12 @@ _SIGS = {
     "apx_options_default": ([P], None),
     "apx_solver_solve_batch": ([P, I32, P, P, F64, I32, P, P], ctypes.c_int),
     "apx_solver_apply_batch": ([P, I32, P, P, P], ctypes.c_int),
+    "apx_server_create": ([ctypes.c_uint64, I32, I32, P], ctypes.c_int),
+    "apx_server_destroy": ([P], None),
+    "apx_server_add": ([P, P, P], ctypes.c_int),
+    "apx_server_submit": ([P, I32, P, F64, I32, P], ctypes.c_int),
+    "apx_server_wait": ([P, I64, P, P, F64, P], ctypes.c_int),
+    "apx_server_get_stats": ([P, P], ctypes.c_int),
     "apx_solver_solve_batch_host": ([P, I32, P, P, F64, I32, P], ctypes.c_int),
     "apx_fp16_round": ([ctypes.c_float], ctypes.c_uint16),
     "apx_fp16_widen": ([ctypes.c_uint16], ctypes.c_float),
@@ -148,7 +154,7 @@ def _csc(A):
 
 def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
              schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=4, sweep_chunk=0,
-             sweep_blocks=0, tail_rows=0, graph=True) -> Options:
+             sweep_blocks=0, tail_rows=0, graph=True, relabel=True) -> Options:
     o = Options()
     load_library().apx_options_default(ctypes.byref(o))
     o.tol = float(tol)
@@ -165,6 +171,7 @@ def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp
     o.sweep_blocks = int(sweep_blocks)
     o.tail_rows = int(tail_rows)
     o.graph = 1 if graph else 0
+    o.relabel = 1 if relabel else 0
     return o
 
 
@@ -370,6 +377,71 @@ class Solver:
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(load_library().apx_solver_sweep(self._h, int(backward), _ptr(rhs), _ptr(out), s))
         return out
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", I64), ("completed", I64), ("factors", I64), ("p50_ms", F64), ("p99_ms", F64),
+                ("mean_ms", F64), ("solves_per_s", F64), ("memory_used", ctypes.c_uint64),
+                ("memory_budget", ctypes.c_uint64)]
+
+
+class Server:
+    """Serving loop over resident factors (continuous batching): every added
+    Solver keeps one blocked PCG of `slots` columns running on its own stream;
+    submitted right-hand sides join free columns between iteration groups and
+    leave when they finish. A solver added here is used by the server only."""
+
+    def __init__(self, memory_budget: int = 0, slots: int = 32, iters_per_check: int = 2):
+        lib = load_library()
+        h = P()
+        _check(lib.apx_server_create(int(memory_budget), int(slots), int(iters_per_check), ctypes.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def add(self, solver: "Solver") -> int:
+        fid = I32()
+        _check(load_library().apx_server_add(self._h, solver._h, ctypes.byref(fid)))
+        self._n[int(fid.value)] = solver._n
+        return int(fid.value)
+
+    def submit(self, factor: int, b, tol: float = 1e-8, maxiter: int = 200) -> int:
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (self._n[factor],):
+            raise ValueError("b has the wrong length")
+        rid = I64()
+        _check(load_library().apx_server_submit(self._h, int(factor), b.ctypes.data, float(tol), int(maxiter),
+                                                ctypes.byref(rid)))
+        self._pending = getattr(self, "_pending", {})
+        self._pending[int(rid.value)] = factor
+        return int(rid.value)
+
+    def wait(self, rid: int, timeout: float | None = None):
+        """(x, SolveResult) of a request; raises TimeoutError."""
+        factor = self._pending[rid]
+        x = np.zeros(self._n[factor])
+        info = SolveInfo()
+        ready = I32()
+        _check(load_library().apx_server_wait(self._h, int(rid), x.ctypes.data, ctypes.byref(info),
+                                              -1.0 if timeout is None else float(timeout), ctypes.byref(ready)))
+        if not ready.value:
+            raise TimeoutError(f"request {rid} not done")
+        del self._pending[rid]
+        return x, SolveResult(x, int(info.iterations), float(info.residual), int(info.status),
+                              int(info.lumped_offdiag), float(info.solve_vram_mb))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(load_library().apx_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().apx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
 
 
 def solve(A, b, tol: float = 1e-8, maxiter: int = 200, **opts) -> SolveResult:

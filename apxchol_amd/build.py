@@ -23,7 +23,7 @@ LIB = ROOT / "libapxchol_amd.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host/factorize.cpp", "host/trsv_prep.cpp", "host/gpu_solver.cpp", "capi.cpp"]
+HOST_SOURCES = ["host/factorize.cpp", "host/trsv_prep.cpp", "host/gpu_solver.cpp", "host/server.cpp", "capi.cpp"]
 KERNEL_SOURCES = ["kernels/kernels.hip"]
 
 HOST_FLAGS = ["-std=c++20", "-O3", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
@@ -59,7 +59,7 @@ def build(verbose: bool = False) -> Path:
         list(ex.map(_run, jobs))
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
-          f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+          f"-L{ROCM}/lib", "-lamdhip64", "-pthread", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -10,6 +10,7 @@
 
 #include "host/factor.hpp"
 #include "host/gpu_solver.hpp"
+#include "host/server.hpp"
 #include "host/trsv_prep.hpp"
 
 using namespace apxchol_amd;
@@ -18,7 +19,10 @@ struct apx_factor_s {
     factorization F;
 };
 struct apx_solver_s {
-    std::unique_ptr<gpu_solver> s;
+    std::shared_ptr<gpu_solver> s;   // shared with a server that serves it
+};
+struct apx_server_s {
+    std::unique_ptr<solve_server> s;
 };
 
 namespace {
@@ -78,6 +82,7 @@ solve_options to_opts(const apx_options* o) {
     so.sweep_chunk = o->sweep_chunk;
     so.sweep_blocks = o->sweep_blocks;
     so.graph = o->graph != 0;
+    so.relabel = o->relabel != 0;
     return so;
 }
 
@@ -114,6 +119,7 @@ void apx_options_default(apx_options* o) {
     o->sweep_blocks = 0;
     o->tail_rows = 0;
     o->graph = 1;
+    o->relabel = 1;
 }
 
 const char* apx_last_error(void) { return g_err.c_str(); }
@@ -197,7 +203,7 @@ apx_status apx_solver_create(int64_t n, const int64_t* colptr, const int32_t* ro
         need(out, "out");
         auto* s = new apx_solver_s;
         try {
-            s->s = std::make_unique<gpu_solver>(make_view(n, colptr, rowidx, vals), to_opts(opts));
+            s->s = std::make_shared<gpu_solver>(make_view(n, colptr, rowidx, vals), to_opts(opts));
         } catch (...) {
             delete s;
             throw;
@@ -213,7 +219,7 @@ apx_status apx_solver_create_with_factor(int64_t n, const int64_t* colptr, const
         need(f, "factor");
         auto* s = new apx_solver_s;
         try {
-            s->s = std::make_unique<gpu_solver>(make_view(n, colptr, rowidx, vals), f->F, to_opts(opts));
+            s->s = std::make_shared<gpu_solver>(make_view(n, colptr, rowidx, vals), f->F, to_opts(opts));
         } catch (...) {
             delete s;
             throw;
@@ -301,6 +307,68 @@ apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, 
         if (k < 0) throw std::invalid_argument("k must be >= 0");
         if (k > 0 && s->s->rows() > 0) { need(d_R, "R"); need(d_Z, "Z"); }
         s->s->apply_batch_device(k, d_R, d_Z, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_server_create(uint64_t memory_budget, int32_t slots, int32_t iters_per_check, apx_server_t* out) {
+    return guarded([&] {
+        need(out, "out");
+        server_options o;
+        o.memory_budget = memory_budget;
+        o.slots = slots > 0 ? slots : 32;
+        o.iters_per_check = iters_per_check > 0 ? iters_per_check : 2;
+        auto h = std::make_unique<apx_server_s>();
+        h->s = std::make_unique<solve_server>(o);
+        *out = h.release();
+    });
+}
+
+void apx_server_destroy(apx_server_t srv) { delete srv; }
+
+apx_status apx_server_add(apx_server_t srv, apx_solver_t solver, int32_t* factor_id) {
+    return guarded([&] {
+        need(srv, "server");
+        need(solver, "solver");
+        const int id = srv->s->add(solver->s);
+        if (factor_id) *factor_id = id;
+    });
+}
+
+apx_status apx_server_submit(apx_server_t srv, int32_t factor, const double* b, double tol, int32_t max_iter,
+                             int64_t* request_id) {
+    return guarded([&] {
+        need(srv, "server");
+        need(b, "b");
+        const int64_t id = srv->s->submit(factor, b, tol, max_iter);
+        if (request_id) *request_id = id;
+    });
+}
+
+apx_status apx_server_wait(apx_server_t srv, int64_t request_id, double* x, apx_solve_info* info, double timeout_s,
+                           int32_t* ready) {
+    return guarded([&] {
+        need(srv, "server");
+        solve_result r;
+        const bool ok = srv->s->wait(request_id, x, &r, timeout_s);
+        if (ready) *ready = ok ? 1 : 0;
+        if (ok) fill_info(r, info);
+    });
+}
+
+apx_status apx_server_get_stats(apx_server_t srv, apx_server_stats* st) {
+    return guarded([&] {
+        need(srv, "server");
+        need(st, "stats");
+        const server_stats s = srv->s->stats();
+        st->submitted = s.submitted;
+        st->completed = s.completed;
+        st->factors = s.factors;
+        st->p50_ms = s.p50_ms;
+        st->p99_ms = s.p99_ms;
+        st->mean_ms = s.mean_ms;
+        st->solves_per_s = s.solves_per_s;
+        st->memory_used = s.memory_used;
+        st->memory_budget = s.memory_budget;
     });
 }
 

@@ -57,9 +57,11 @@ struct pcg_state {
     double pad[4];
     int iter, done, max_iter, stag_window;
     int apply_count, center_k, laplacian, center_now;
-    int status;          // 0 running, 1 converged, 2 breakdown (pAp <= 0), 3 stagnation, 4 max_iter
+    int status;          // 0 running, 1 converged, 2 breakdown (pAp <= 0), 3 stagnation, 4 max_iter,
+                         // 5 zero right-hand side
     int n;
-    int pad_i[2];
+    int phase;           // blocked solves: 0 iterating, 1 joining (first application pending)
+    int pad_i;
 };
 
 // Bookkeeping of a blocked (k right-hand sides) solve: the number of columns

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -203,6 +204,106 @@ void validate_sweep_layout(const trsv_factor& T, const batch_table& bt, bool bac
     }
 }
 
+block_batch_table build_block_batches(const trsv_factor& T, bool backward, int kp) {
+    block_batch_table bt;
+    bt.kp = kp;
+    const std::uint32_t m = T.m;
+    const auto& ptr = backward ? T.b_ptr : T.f_ptr;
+    const std::size_t L = T.level_ptr.size() - 1;
+    std::uint64_t mask = 0;
+    int lane = 0;
+    bool open = false;
+    auto close = [&]() {
+        if (!open) return;
+        bt.mask.push_back(mask);
+        bt.col0.push_back(0);
+        open = false;
+    };
+    for (std::size_t li = 0; li < L; ++li) {
+        std::uint32_t p0, p1;
+        if (!backward) { p0 = T.level_ptr[li]; p1 = T.level_ptr[li + 1]; }
+        else { p0 = m - T.level_ptr[L - li]; p1 = m - T.level_ptr[L - li - 1]; }
+        close();
+        bt.level_first.push_back(static_cast<int>(bt.mask.size()));
+        for (std::uint32_t t = p0; t < p1; ++t) {
+            const std::uint32_t row = backward ? m - 1 - t : t;
+            const int G = lanes_for_host(ptr[row + 1] - ptr[row]);
+            const int U = G * kp;
+            if (U > kWave) {
+                close();
+                for (int c0 = 0; c0 < kp; c0 += kWave / G) {
+                    bt.pos.push_back(t);
+                    bt.mask.push_back(0);
+                    bt.col0.push_back(c0);
+                }
+                continue;
+            }
+            int a = (lane + U - 1) / U * U;
+            if (open && a + U > kWave) { close(); a = 0; }
+            if (!open) { bt.pos.push_back(t); open = true; mask = 0; a = 0; }
+            mask |= 1ull << a;
+            lane = a + U;
+        }
+    }
+    close();
+    bt.level_first.push_back(static_cast<int>(bt.mask.size()));
+    bt.pos.push_back(m);
+    return bt;
+}
+
+// The blocked table's invariants, checked before upload: every position of
+// every level is covered once per column, groups are aligned and disjoint,
+// split waves cover the columns exactly, levels start on wave boundaries.
+void validate_block_batches(const trsv_factor& T, const block_batch_table& bt, bool backward) {
+    auto fail = [](const std::string& w) { throw std::runtime_error("apxchol_amd: invalid blocked sweep table: " + w); };
+    const std::uint32_t m = T.m;
+    const auto& ptr = backward ? T.b_ptr : T.f_ptr;
+    const int kp = bt.kp;
+    if (kp < 1 || kp > kWave || (kp & (kp - 1))) fail("block width");
+    const std::size_t nw = bt.mask.size();
+    if (bt.pos.size() != nw + 1 || bt.col0.size() != nw || bt.pos[nw] != m) fail("sizes");
+    std::vector<std::uint32_t> seen(m, 0);   // columns covered per position
+    for (std::size_t w = 0; w < nw; ++w) {
+        const std::uint32_t p = bt.pos[w];
+        if (p >= m) fail("position range");
+        if (bt.mask[w] == 0) {
+            const int G = lanes_for_host(ptr[(backward ? m - 1 - p : p) + 1] - ptr[backward ? m - 1 - p : p]);
+            if (G * kp <= kWave || bt.col0[w] < 0 || bt.col0[w] % (kWave / G) || bt.col0[w] >= kp) fail("split wave");
+            seen[p] += static_cast<std::uint32_t>(kWave / G);
+            continue;
+        }
+        std::uint64_t used = 0;
+        int k = 0;
+        for (int lane = 0; lane < kWave; ++lane) {
+            if (!((bt.mask[w] >> lane) & 1ull)) continue;
+            const std::uint32_t t = p + static_cast<std::uint32_t>(k++);
+            if (t >= m) fail("row past m");
+            const std::uint32_t row = backward ? m - 1 - t : t;
+            const int U = lanes_for_host(ptr[row + 1] - ptr[row]) * kp;
+            if (U > kWave || lane % U != 0 || lane + U > kWave) fail("group alignment");
+            const std::uint64_t g = (U == 64 ? ~0ull : ((1ull << U) - 1ull)) << lane;
+            if (used & g) fail("group overlap");
+            used |= g;
+            const std::uint64_t after = bt.mask[w] >> lane >> 1;
+            const int next = after ? lane + 1 + __builtin_ctzll(after) : kWave;
+            if (next < lane + U) fail("next group inside this group");
+            seen[t] += static_cast<std::uint32_t>(kp);
+        }
+    }
+    for (std::uint32_t t = 0; t < m; ++t)
+        if (seen[t] != static_cast<std::uint32_t>(kp)) fail("coverage");
+    const std::size_t L = T.level_ptr.size() - 1;
+    if (bt.level_first.size() != L + 1 || bt.level_first[0] != 0 || bt.level_first[L] != static_cast<int>(nw))
+        fail("level bounds");
+    for (std::size_t l = 0; l < L; ++l) {
+        if (bt.level_first[l + 1] < bt.level_first[l]) fail("level order");
+        const std::uint32_t lp0 = backward ? m - T.level_ptr[L - l] : T.level_ptr[l];
+        const std::uint32_t lp1 = backward ? m - T.level_ptr[L - l - 1] : T.level_ptr[l + 1];
+        for (int w = bt.level_first[l]; w < bt.level_first[l + 1]; ++w)
+            if (bt.pos[w] < lp0 || bt.pos[w] >= lp1) fail("wave outside its level");
+    }
+}
+
 namespace {
 
 // What one lane of batch b does (the kernels' lane mapping).
@@ -322,6 +423,7 @@ struct gpu_solver::device_state {
     bool op32 = false;
     int lanes = 8;
     dbuf r, x, p, ap, rp, y, z, xout;
+    dbuf relabel;   // factor position -> storage position (empty: identity)
     dbuf part, counter, st, st_apply, ctrl, err;
     pcg_state* host_st = nullptr;
     int* host_err = nullptr;
@@ -331,17 +433,25 @@ struct gpu_solver::device_state {
     hipStream_t cap = nullptr;
     hipGraph_t iter_graph = nullptr;
     hipGraphExec_t iter_exec = nullptr;
-    // Blocked solves: first sweep position of every level per direction, and
-    // the block buffers (allocated on first use for the widest block seen).
-    std::vector<int> f_lpos, b_lpos;
+    // Blocked solves: the block buffers (allocated on first use for the
+    // widest block seen) and the sweep tables per block width.
+    struct block_tables {
+        dbuf f_pos, f_mask, f_col0, b_pos, b_mask, b_col0;
+        std::vector<int> f_lf, b_lf;
+        block_sched f, b;
+    };
     struct block_bufs {
         int kp = 0;
+        std::map<int, std::unique_ptr<block_tables>> tables;   // per block width
         dbuf r, x, p, ap, y, z, st, bs, part, counter;
         pcg_state* host_st = nullptr;
         batch_state* host_bs = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int graph_kp = 0;
+        int session_kp = 0;             // block width of a running serving session
+        unsigned char* host_adm = nullptr;   // pinned: 64 slot ids + 64 pointers
+        dbuf adm_dev;
         void drop_graph() {
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
@@ -353,6 +463,7 @@ struct gpu_solver::device_state {
             drop_graph();
             if (host_st) (void)hipHostFree(host_st);
             if (host_bs) (void)hipHostFree(host_bs);
+            if (host_adm) (void)hipHostFree(host_adm);
         }
     } blk;
     ~device_state() {
@@ -457,7 +568,18 @@ void gpu_solver::setup(const csc_view& L) {
     if (n_ > (1ll << 31) - 2) throw std::invalid_argument("gpu_solver: n exceeds the 32-bit device index range");
     center_k_ = center_k_from_env();
     T_ = prepare_trsv(F_, opts_.fp16_storage, opts_.drop_rel);
-    const permuted_operator op = build_permuted_operator(L, F_.perm);
+    // Storage order of the device vectors: the factor order, or with the
+    // within-level renumbering (default) the factor order with every level's
+    // rows sorted by original vertex, so a vertex's neighbours -- gathered by
+    // the sweeps and the SpMV -- sit close together.
+    std::vector<node_t> sperm = F_.perm;
+    relabel_.clear();
+    if (opts_.relabel && env_int("APXCHOL_AMD_RELABEL", 1) != 0) {
+        relabel_ = relabel_within_levels(T_, F_.perm);
+        for (auto& q : sperm)
+            if (q < T_.m) q = relabel_[q];
+    }
+    const permuted_operator op = build_permuted_operator(L, sperm);
     if (op.ptr.back() > static_cast<std::int64_t>(UINT32_MAX))
         throw std::invalid_argument("gpu_solver: operator nnz exceeds the 32-bit device index range");
     for (const std::int32_t c : op.idx)
@@ -500,7 +622,7 @@ void gpu_solver::setup(const csc_view& L) {
         throw std::runtime_error("apxchol_amd: invalid factor layout");
     {
         std::vector<std::uint8_t> seen(n, 0);
-        for (const node_t q : F_.perm) {
+        for (const node_t q : sperm) {
             if (q >= static_cast<node_t>(n) || seen[q]) throw std::runtime_error("apxchol_amd: perm is not a permutation");
             seen[q] = 1;
         }
@@ -512,8 +634,9 @@ void gpu_solver::setup(const csc_view& L) {
     d.f_levels = fb.level_first;
     d.b_levels = bb.level_first;
 
-    std::vector<std::uint32_t> perm(F_.perm.begin(), F_.perm.end());
+    std::vector<std::uint32_t> perm(sperm.begin(), sperm.end());
     d.perm.upload(perm);
+    if (!relabel_.empty()) d.relabel.upload(relabel_);
     std::vector<std::uint32_t> optr(op.ptr.begin(), op.ptr.end());
     d.op_ptr.upload(optr);
     d.op_idx.upload(op.idx);
@@ -656,8 +779,6 @@ void gpu_solver::setup(const csc_view& L) {
     dbuf* b_tb[4] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent};
     make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb);
     make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb);
-    for (const int l : fb.level_first) d.f_lpos.push_back(static_cast<int>(fb.pos[l]));
-    for (const int l : bb.level_first) d.b_lpos.push_back(static_cast<int>(bb.pos[l]));
     // Coupling output, shared by the two directions (their sweeps never overlap).
     const std::size_t tail_batches = std::max<std::size_t>(
         1, static_cast<std::size_t>(std::max(d.f_sched.tail_batches, d.b_sched.tail_batches)));
@@ -844,7 +965,9 @@ int gpu_solver::prepare_block(int k) const {
     b.r.alloc(vb); b.x.alloc(vb); b.p.alloc(vb); b.ap.alloc(vb); b.y.alloc(vb); b.z.alloc(vb);
     b.st.alloc(static_cast<std::size_t>(kp) * sizeof(pcg_state));
     b.bs.alloc(sizeof(batch_state));
-    b.part.alloc(static_cast<std::size_t>(kMaxBlocksHost) * 2 * kp * sizeof(double));
+    // Partials: 2 values x kp columns per workgroup; the SpMM grid may round
+    // kMaxBlocksHost up by at most 15 (a multiple of G kp / 256 <= 16).
+    b.part.alloc(static_cast<std::size_t>(kMaxBlocksHost + 16) * 2 * kp * sizeof(double));
     b.counter.alloc(64);
     b.counter.zero();
     if (b.host_st) (void)hipHostFree(b.host_st);
@@ -854,6 +977,11 @@ int gpu_solver::prepare_block(int k) const {
     if (!b.host_bs)
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&b.host_bs), sizeof(batch_state), hipHostMallocDefault),
                   "hipHostMalloc");
+    if (!b.host_adm) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&b.host_adm), 64 * sizeof(int) + 64 * sizeof(void*),
+                                hipHostMallocDefault), "hipHostMalloc");
+        b.adm_dev.alloc(64 * sizeof(int) + 64 * sizeof(void*));
+    }
     b.kp = kp;
     return kp;
 }
@@ -883,9 +1011,10 @@ void gpu_solver::apply_block(int k, const double* d_R, double* d_Z, bool center,
         launch_sum_cols_k(vec_prec::fp64, n, kp, b.r.get(), st, part, cnt, s);
         launch_shift_from_sum_k(kp, st, true, s);
     }
-    const int nlf = static_cast<int>(d.f_lpos.size()) - 1, nlb = static_cast<int>(d.b_lpos.size()) - 1;
-    launch_sweep_k(d.fwd, d.prec, d.fp16, kp, b.r.get(), b.y.get(), st, bs, d.f_lpos.data(), nlf, s);
-    launch_sweep_k(d.bck, d.prec, d.fp16, kp, b.y.get(), b.z.get(), st, bs, d.b_lpos.data(), nlb, s);
+    block_scheds(kp);
+    const auto& t = *b.tables.at(kp);
+    launch_sweep_k(d.fwd, d.prec, d.fp16, t.f, b.r.get(), b.y.get(), st, bs, s);
+    launch_sweep_k(d.bck, d.prec, d.fp16, t.b, b.y.get(), b.z.get(), st, bs, s);
     if (center) launch_sum_cols_k(d.prec, n, kp, b.z.get(), st, part, cnt, s);
     launch_gather_k(d.prec, n, k, kp, d.perm.as<unsigned>(), b.z.get(), d_Z, st, center, s);
 }
@@ -900,15 +1029,48 @@ void gpu_solver::apply_batch_device(int k, const double* d_R, double* d_Z, hipSt
                     d_Z + static_cast<std::size_t>(c0) * n_, center, s);
 }
 
+// The blocked sweep tables of block width kp (built, checked and uploaded on
+// first use).
+const block_sched* gpu_solver::block_scheds(int kp) const {
+    auto& b = dev_->blk;
+    auto it = b.tables.find(kp);
+    if (it == b.tables.end()) {
+        auto t = std::make_unique<device_state::block_tables>();
+        for (int dir = 0; dir < 2; ++dir) {
+            const bool backward = dir == 1;
+            const block_batch_table bt = build_block_batches(T_, backward, kp);
+            validate_block_batches(T_, bt, backward);
+            dbuf& dp = backward ? t->b_pos : t->f_pos;
+            dbuf& dm = backward ? t->b_mask : t->f_mask;
+            dbuf& dc = backward ? t->b_col0 : t->f_col0;
+            dp.upload(bt.pos);
+            dm.upload(bt.mask);
+            dc.upload(bt.col0);
+            std::vector<int>& lf = backward ? t->b_lf : t->f_lf;
+            lf = bt.level_first;
+            block_sched& sc = backward ? t->b : t->f;
+            sc.kp = kp;
+            sc.pos = dp.as<unsigned>();
+            sc.mask = dm.as<unsigned long long>();
+            sc.col0 = dc.as<int>();
+            sc.level_first = lf.data();
+            sc.nlevels = static_cast<int>(lf.size()) - 1;
+        }
+        it = b.tables.emplace(kp, std::move(t)).first;
+    }
+    return &it->second->f;
+}
+
 void gpu_solver::enqueue_precondition_k(bool first, int kp, hipStream_t s) const {
     auto& d = *dev_;
     auto& b = d.blk;
     const int n = static_cast<int>(n_);
     pcg_state* st = b.st.as<pcg_state>();
     batch_state* bs = b.bs.as<batch_state>();
-    const int nlf = static_cast<int>(d.f_lpos.size()) - 1, nlb = static_cast<int>(d.b_lpos.size()) - 1;
-    launch_sweep_k(d.fwd, d.prec, d.fp16, kp, b.r.get(), b.y.get(), st, bs, d.f_lpos.data(), nlf, s);
-    launch_sweep_k(d.bck, d.prec, d.fp16, kp, b.y.get(), b.z.get(), st, bs, d.b_lpos.data(), nlb, s);
+    block_scheds(kp);
+    const auto& t = *b.tables.at(kp);
+    launch_sweep_k(d.fwd, d.prec, d.fp16, t.f, b.r.get(), b.y.get(), st, bs, s);
+    launch_sweep_k(d.bck, d.prec, d.fp16, t.b, b.y.get(), b.z.get(), st, bs, s);
     launch_dot_center_k(d.prec, n, kp, b.r.as<double>(), b.z.get(), st, bs, b.part.as<double>(),
                         b.counter.as<unsigned>(), first, s);
     launch_update_p_k(d.prec, n, kp, b.p.as<double>(), b.z.get(), st, bs, first, s);
@@ -926,40 +1088,18 @@ void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s) const {
                        b.part.as<double>(), b.counter.as<unsigned>(), s);
     launch_refresh_active_k(kp, st, bs, s);
     enqueue_precondition_k(false, kp, s);
+    launch_end_iteration_k(kp, st, bs, s);
 }
 
-std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
-                                                  hipStream_t s) const {
+// The iteration graph of block width kp (captured on first use).
+void gpu_solver::replay_iterations_k(int kp, int count, hipStream_t s) const {
     auto& d = *dev_;
     auto& b = d.blk;
-    const int n = static_cast<int>(n_);
-    const int m = static_cast<int>(T_.m);
-    const int kp = prepare_block(k);
-    for (int c = 0; c < kp; ++c) {
-        pcg_state& h = b.host_st[c];
-        std::memset(&h, 0, sizeof h);
-        h.tol = tol;
-        h.max_iter = max_iter;
-        h.stag_window = opts_.stagnation_window;
-        h.center_k = center_k_;
-        h.laplacian = F_.sddm ? 0 : 1;
-        h.n = n;
-        h.prev_check = 1.0;
+    if (!opts_.graph) {
+        for (int j = 0; j < count; ++j) enqueue_iteration_k(kp, s);
+        return;
     }
-    std::memset(b.host_bs, 0, sizeof(batch_state));
-    b.host_bs->active = kp;
-    pcg_state* st = b.st.as<pcg_state>();
-    batch_state* bs = b.bs.as<batch_state>();
-    hip_check(hipMemcpyAsync(st, b.host_st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyHostToDevice, s),
-              "state upload");
-    hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
-    launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_B, b.r.as<double>(), b.x.as<double>(), s);
-    launch_zero_tail_k(d.prec, n, m, kp, b.z.get(), s);
-    launch_init_residual_k(n, kp, b.r.as<double>(), st, bs, b.part.as<double>(), b.counter.as<unsigned>(), s);
-    launch_refresh_active_k(kp, st, bs, s);
-    enqueue_precondition_k(true, kp, s);
-    const int J = std::max(1, opts_.iters_per_check);
-    if (opts_.graph && b.graph_kp != kp) {
+    if (b.graph_kp != kp) {
         b.drop_graph();
         if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -975,16 +1115,153 @@ std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, doub
         hip_check(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         b.graph_kp = kp;
     }
+    for (int j = 0; j < count; ++j) hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+}
+
+void gpu_solver::fresh_state(pcg_state& h, double tol, int max_iter) const {
+    std::memset(&h, 0, sizeof h);
+    h.tol = tol;
+    h.max_iter = max_iter;
+    h.stag_window = opts_.stagnation_window;
+    h.center_k = center_k_;
+    h.laplacian = F_.sddm ? 0 : 1;
+    h.n = static_cast<int>(n_);
+    h.prev_check = 1.0;
+    h.phase = 1;
+}
+
+// ---- serving sessions: a running block that admits and releases columns ----
+
+int gpu_solver::session_begin(int slots, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int kp = prepare_block(slots);
+    block_scheds(kp);
+    for (int c = 0; c < kp; ++c) {
+        std::memset(&b.host_st[c], 0, sizeof(pcg_state));
+        b.host_st[c].done = 1;          // empty slot
+        b.host_st[c].n = static_cast<int>(n_);
+    }
+    std::memset(b.host_bs, 0, sizeof(batch_state));
+    hip_check(hipMemcpyAsync(b.st.get(), b.host_st, static_cast<std::size_t>(kp) * sizeof(pcg_state),
+                             hipMemcpyHostToDevice, s), "state upload");
+    hip_check(hipMemcpyAsync(b.bs.get(), b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
+    launch_zero_tail_k(d.prec, static_cast<int>(n_), static_cast<int>(T_.m), kp, b.z.get(), s);
+    hip_check(hipStreamSynchronize(s), "session begin");
+    b.session_kp = kp;
+    return kp;
+}
+
+void gpu_solver::session_admit(const std::vector<session_column>& cols, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int kp = b.session_kp;
+    if (kp == 0) throw std::logic_error("gpu_solver: no session");
+    if (cols.empty()) return;
+    const int nc = static_cast<int>(cols.size());
+    if (nc > kp) throw std::invalid_argument("gpu_solver: more columns than slots");
+    // Per-slot states (only the admitted slots' structs are written: the
+    // others are live on the device) and the column / source tables.
+    int* hc = reinterpret_cast<int*>(b.host_adm);
+    const double** hs = reinterpret_cast<const double**>(b.host_adm + 64 * sizeof(int));
+    for (int j = 0; j < nc; ++j) {
+        const int c = cols[j].slot;
+        if (c < 0 || c >= kp) throw std::invalid_argument("gpu_solver: slot out of range");
+        fresh_state(b.host_st[c], cols[j].tol, cols[j].max_iter);
+        hip_check(hipMemcpyAsync(b.st.as<pcg_state>() + c, &b.host_st[c], sizeof(pcg_state), hipMemcpyHostToDevice, s),
+                  "state upload");
+        hc[j] = c;
+        hs[j] = cols[j].d_b;
+    }
+    hip_check(hipMemcpyAsync(b.adm_dev.get(), b.host_adm, 64 * sizeof(int) + 64 * sizeof(void*), hipMemcpyHostToDevice, s),
+              "admission upload");
+    const int* dcols = b.adm_dev.as<int>();
+    const double* const* dsrc = reinterpret_cast<const double* const*>(b.adm_dev.as<unsigned char>() + 64 * sizeof(int));
+    launch_admit_scatter_k(static_cast<int>(n_), kp, d.perm.as<unsigned>(), nc, dcols, dsrc, b.r.as<double>(),
+                           b.x.as<double>(), s);
+    launch_init_residual_k(static_cast<int>(n_), kp, b.r.as<double>(), b.st.as<pcg_state>(), b.bs.as<batch_state>(),
+                           b.part.as<double>(), b.counter.as<unsigned>(), s);
+    launch_refresh_active_k(kp, b.st.as<pcg_state>(), b.bs.as<batch_state>(), s);
+    // The host copies must stay valid until the stream has consumed them.
+    hip_check(hipStreamSynchronize(s), "admit");
+}
+
+void gpu_solver::session_step(int iterations, hipStream_t s) const {
+    replay_iterations_k(dev_->blk.session_kp, iterations, s);
+}
+
+void gpu_solver::session_read(std::vector<session_slot>& out, hipStream_t s) const {
+    auto& b = dev_->blk;
+    const int kp = b.session_kp;
+    hip_check(hipMemcpyAsync(b.host_st, b.st.get(), static_cast<std::size_t>(kp) * sizeof(pcg_state),
+                             hipMemcpyDeviceToHost, s), "state read");
+    hip_check(hipStreamSynchronize(s), "session read");
+    out.resize(static_cast<std::size_t>(kp));
+    for (int c = 0; c < kp; ++c) {
+        out[c].done = b.host_st[c].done != 0;
+        out[c].status = b.host_st[c].status;
+        out[c].iterations = b.host_st[c].iter;
+        out[c].residual = b.host_st[c].residual;
+    }
+}
+
+void gpu_solver::session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int kp = b.session_kp;
+    const int nc = static_cast<int>(slots.size());
+    if (nc == 0) return;
+    if (nc > kp || d_x.size() != slots.size()) throw std::invalid_argument("gpu_solver: collect tables");
+    int* hc = reinterpret_cast<int*>(b.host_adm);
+    double** hd = reinterpret_cast<double**>(b.host_adm + 64 * sizeof(int));
+    for (int j = 0; j < nc; ++j) {
+        if (slots[j] < 0 || slots[j] >= kp) throw std::invalid_argument("gpu_solver: slot out of range");
+        hc[j] = slots[j];
+        hd[j] = d_x[j];
+    }
+    hip_check(hipMemcpyAsync(b.adm_dev.get(), b.host_adm, 64 * sizeof(int) + 64 * sizeof(void*), hipMemcpyHostToDevice, s),
+              "collect upload");
+    pcg_state* st = b.st.as<pcg_state>();
+    if (!F_.sddm)
+        launch_sum_cols_k(vec_prec::fp64, static_cast<int>(n_), kp, b.x.get(), st, b.part.as<double>(),
+                          b.counter.as<unsigned>(), s);
+    launch_gather_cols_k(static_cast<int>(n_), kp, d.perm.as<unsigned>(), nc, b.adm_dev.as<int>(),
+                         reinterpret_cast<double* const*>(b.adm_dev.as<unsigned char>() + 64 * sizeof(int)),
+                         b.x.as<double>(), st, !F_.sddm, s);
+    hip_check(hipStreamSynchronize(s), "collect");
+}
+
+std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
+                                                  hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int n = static_cast<int>(n_);
+    const int m = static_cast<int>(T_.m);
+    const int kp = prepare_block(k);
+    block_scheds(kp);
+    b.session_kp = 0;   // a solve ends any serving session on this solver
+    // Every column starts joining: the first iteration is its first
+    // preconditioner application (SpMM and x / r update skipped).
+    for (int c = 0; c < kp; ++c) fresh_state(b.host_st[c], tol, max_iter);
+    std::memset(b.host_bs, 0, sizeof(batch_state));
+    b.host_bs->active = kp;
+    pcg_state* st = b.st.as<pcg_state>();
+    batch_state* bs = b.bs.as<batch_state>();
+    hip_check(hipMemcpyAsync(st, b.host_st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyHostToDevice, s),
+              "state upload");
+    hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
+    launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_B, b.r.as<double>(), b.x.as<double>(), s);
+    launch_zero_tail_k(d.prec, n, m, kp, b.z.get(), s);
+    launch_init_residual_k(n, kp, b.r.as<double>(), st, bs, b.part.as<double>(), b.counter.as<unsigned>(), s);
+    launch_refresh_active_k(kp, st, bs, s);
+    const int J = std::max(1, opts_.iters_per_check);
     int enqueued = 0;
     for (;;) {
-        for (int j = 0; j < J; ++j) {
-            if (opts_.graph) hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
-            else enqueue_iteration_k(kp, s);
-        }
+        replay_iterations_k(kp, J, s);
         enqueued += J;
         hip_check(hipMemcpyAsync(b.host_bs, bs, sizeof(batch_state), hipMemcpyDeviceToHost, s), "state read");
         hip_check(hipStreamSynchronize(s), "solve");
-        if (b.host_bs->active == 0 || enqueued > max_iter + J) break;
+        if (b.host_bs->active == 0 || enqueued > max_iter + J + 1) break;
     }
     if (!F_.sddm)
         launch_sum_cols_k(vec_prec::fp64, n, kp, b.x.get(), st, b.part.as<double>(), b.counter.as<unsigned>(), s);
@@ -1083,9 +1360,19 @@ void gpu_solver::sweep_device(bool backward, const double* d_rhs, double* d_out,
     auto& d = *dev_;
     const int m = static_cast<int>(T_.m);
     if (m == 0) return;
-    launch_fill_pending(vec_prec::fp64, d_out, m, s);
-    launch_sweep(backward ? d.bck : d.fwd, vec_prec::fp64, d.fp16, d_rhs, d_out, nullptr, d.err.as<int>(),
+    if (relabel_.empty()) {
+        launch_fill_pending(vec_prec::fp64, d_out, m, s);
+        launch_sweep(backward ? d.bck : d.fwd, vec_prec::fp64, d.fp16, d_rhs, d_out, nullptr, d.err.as<int>(),
+                     backward ? d.b_sched : d.f_sched, s);
+        return;
+    }
+    // Factor order at the boundary, storage order inside.
+    const unsigned* Q = d.relabel.as<unsigned>();
+    launch_permute(m, Q, d_rhs, d.rp.as<double>(), true, s);
+    launch_fill_pending(vec_prec::fp64, d.xout.get(), m, s);
+    launch_sweep(backward ? d.bck : d.fwd, vec_prec::fp64, d.fp16, d.rp.get(), d.xout.get(), nullptr, d.err.as<int>(),
                  backward ? d.b_sched : d.f_sched, s);
+    launch_permute(m, Q, d.xout.as<double>(), d_out, false, s);
 }
 
 solve_result solve(const csc_view& L, const std::vector<double>& b, const solve_options& opts) {

@@ -33,6 +33,7 @@ struct solve_options {
     int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 8)
     int sweep_blocks = 0;         // dataflow grid (0: APXCHOL_AMD_SWEEP_BLOCKS or the resident grid)
     bool graph = true;            // replay each PCG iteration as one captured HIP graph
+    bool relabel = true;          // sort every level's rows by original vertex (APXCHOL_AMD_RELABEL=0: off)
 };
 
 // solve.h:27-42. status: 0 running/none, 1 converged, 2 pAp <= 0 breakdown,
@@ -64,6 +65,8 @@ struct solver_stats {
     std::int64_t level_launches[2] = {0, 0}, tail_runs[2] = {0, 0};
 };
 
+struct block_sched;
+
 class gpu_solver {
 public:
     explicit gpu_solver(const csc_view& L, const solve_options& opts = {});
@@ -91,6 +94,30 @@ public:
                                                  hipStream_t s) const;
     std::vector<solve_result> solve_batch(int k, const std::vector<double>& B, std::vector<double>& X,
                                           double tol = -1.0, int max_iter = -1) const;
+    // Serving sessions: one running block of `slots` columns (rounded up to
+    // a power of two <= 64) that admits right-hand sides into free slots
+    // between iteration groups and releases finished ones. A column's
+    // arithmetic does not depend on the other columns or on its slot, so a
+    // request gets the same bits alone and in a full block of this width.
+    // Not to be mixed with other calls on this solver while a session runs.
+    struct session_column {
+        int slot;
+        const double* d_b;   // n doubles (device), caller order
+        double tol;
+        int max_iter;
+    };
+    struct session_slot {
+        bool done;
+        int status;
+        std::int64_t iterations;
+        double residual;
+    };
+    int session_begin(int slots, hipStream_t s) const;
+    void session_admit(const std::vector<session_column>& cols, hipStream_t s) const;
+    void session_step(int iterations, hipStream_t s) const;
+    void session_read(std::vector<session_slot>& out, hipStream_t s) const;
+    void session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s) const;
+
     // k preconditioner applications at once (k calls of cpu_solver::apply,
     // solve.h:103, counting as ONE application of the center-k schedule).
     void apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const;
@@ -125,11 +152,15 @@ private:
     std::vector<solve_result> solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
                                           hipStream_t s) const;
     int prepare_block(int k) const;
+    void replay_iterations_k(int kp, int count, hipStream_t s) const;
+    void fresh_state(pcg_state& h, double tol, int max_iter) const;
+    const block_sched* block_scheds(int kp) const;
     void apply_block(int k, const double* d_R, double* d_Z, bool center, hipStream_t s) const;
     void enqueue_iteration_k(int kp, hipStream_t s) const;
     void enqueue_precondition_k(bool first, int kp, hipStream_t s) const;
 
     solve_options opts_;
+    std::vector<std::uint32_t> relabel_;   // factor position -> storage position (empty: identity)
     factorization F_;
     trsv_factor T_;
     solver_stats stats_;
@@ -151,6 +182,21 @@ struct batch_table {
 };
 batch_table build_batches(const trsv_factor& T, bool backward);
 void validate_sweep_layout(const trsv_factor& T, const batch_table& bt, bool backward);
+
+// Wave table of a blocked sweep over kp columns. Row of `len` off-diagonals
+// needs U = G(len) * kp lanes (lane = column * G + chain). A wave with
+// mask != 0 packs whole rows (bit = first lane of a row's aligned U-lane
+// group, rows at consecutive positions from pos); a wave with mask == 0
+// holds columns [col0, col0 + 64 / G) of the row at pos (U > 64).
+struct block_batch_table {
+    int kp = 0;
+    std::vector<std::uint32_t> pos;
+    std::vector<std::uint64_t> mask;
+    std::vector<std::int32_t> col0;
+    std::vector<int> level_first;     // levels + 1
+};
+block_batch_table build_block_batches(const trsv_factor& T, bool backward, int kp);
+void validate_block_batches(const trsv_factor& T, const block_batch_table& bt, bool backward);
 
 // Static tail layout of one direction (device.hpp's tail_desc), run by run.
 struct tail_tables {

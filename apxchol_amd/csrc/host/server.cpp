@@ -1,0 +1,240 @@
+// solve_server: resident factors, one blocked session per factor, requests
+// admitted into free columns between iteration groups.
+#include "server.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace apxchol_amd {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("apxchol_amd server: ") + what + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+solve_server::solve_server(const server_options& opts) : opts_(opts) {
+    check(hipGetDevice(&device_), "hipGetDevice");
+    if (opts_.memory_budget == 0) {
+        std::size_t fr = 0, tot = 0;
+        check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+        opts_.memory_budget = static_cast<std::uint64_t>(static_cast<double>(fr) * 0.9);
+    }
+    if (opts_.slots < 1 || opts_.slots > 64) throw std::invalid_argument("server: slots must be in [1, 64]");
+    opts_.iters_per_check = std::max(1, opts_.iters_per_check);
+}
+
+solve_server::~solve_server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    work_cv_.notify_all();
+    for (auto& l : lanes_)
+        if (l->worker.joinable()) l->worker.join();
+    for (auto& l : lanes_) {
+        if (l->d_b) (void)hipFree(l->d_b);
+        if (l->d_x) (void)hipFree(l->d_x);
+        if (l->stream) (void)hipStreamDestroy(l->stream);
+    }
+}
+
+int solve_server::add(std::shared_ptr<gpu_solver> solver) {
+    if (!solver) throw std::invalid_argument("server: null solver");
+    int kp = 1;
+    while (kp < opts_.slots) kp <<= 1;
+    const std::int64_t n = solver->rows();
+    // Solver arrays + the block (r, x, p, Ap, y, z) + the b / x staging.
+    const std::uint64_t need = solver->stats().device_bytes +
+                               static_cast<std::uint64_t>(8) * static_cast<std::uint64_t>(n) * kp * sizeof(double);
+    auto ln = std::make_unique<lane>();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (used_ + need > opts_.memory_budget)
+            throw std::runtime_error("server: factor does not fit the memory budget (" + std::to_string(used_ + need) +
+                                     " > " + std::to_string(opts_.memory_budget) + " bytes)");
+        used_ += need;
+    }
+    ln->solver = std::move(solver);
+    ln->kp = kp;
+    ln->n = n;
+    ln->slot.assign(static_cast<std::size_t>(kp), nullptr);
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const std::size_t vb = static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * kp * sizeof(double);
+    check(hipMalloc(&ln->d_b, vb), "hipMalloc");
+    check(hipMalloc(&ln->d_x, vb), "hipMalloc");
+    ln->solver->session_begin(kp, ln->stream);
+    lane* raw = ln.get();
+    int id;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        lanes_.push_back(std::move(ln));
+        id = static_cast<int>(lanes_.size()) - 1;
+    }
+    raw->worker = std::thread([this, raw] { run(*raw); });
+    return id;
+}
+
+std::int64_t solve_server::submit(int factor, const double* b, double tol, int max_iter) {
+    auto rq = std::make_shared<request>();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (factor < 0 || factor >= static_cast<int>(lanes_.size())) throw std::invalid_argument("server: bad factor id");
+        const std::int64_t n = lanes_[factor]->n;
+        rq->b.assign(b, b + n);
+        rq->id = next_id_++;
+        rq->factor = factor;
+        rq->tol = tol;
+        rq->max_iter = max_iter;
+        rq->t_submit = std::chrono::steady_clock::now();
+        if (!any_submit_) { first_submit_ = rq->t_submit; any_submit_ = true; }
+        ++submitted_;
+        requests_[rq->id] = rq;
+        lanes_[factor]->queue.push_back(rq);
+    }
+    work_cv_.notify_all();
+    return rq->id;
+}
+
+bool solve_server::wait(std::int64_t id, double* x, solve_result* res, double timeout_s) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = requests_.find(id);
+    if (it == requests_.end()) throw std::invalid_argument("server: unknown request id");
+    auto rq = it->second;
+    auto ready = [&] { return rq->done; };
+    if (timeout_s < 0) done_cv_.wait(lk, ready);
+    else if (!done_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) return false;
+    if (x) std::copy(rq->x.begin(), rq->x.end(), x);
+    if (res) *res = rq->res;
+    requests_.erase(id);
+    return true;
+}
+
+server_stats solve_server::stats() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    server_stats s;
+    s.submitted = submitted_;
+    s.completed = static_cast<std::int64_t>(latencies_ms_.size());
+    s.factors = static_cast<std::int64_t>(lanes_.size());
+    s.memory_used = used_;
+    s.memory_budget = opts_.memory_budget;
+    if (!latencies_ms_.empty()) {
+        std::vector<double> v = latencies_ms_;
+        std::sort(v.begin(), v.end());
+        auto q = [&](double p) {
+            const std::size_t k = static_cast<std::size_t>(p * static_cast<double>(v.size() - 1) + 0.5);
+            return v[std::min(k, v.size() - 1)];
+        };
+        s.p50_ms = q(0.50);
+        s.p99_ms = q(0.99);
+        double sum = 0.0;
+        for (double t : v) sum += t;
+        s.mean_ms = sum / static_cast<double>(v.size());
+        const double wall = std::chrono::duration<double>(last_done_ - first_submit_).count();
+        s.solves_per_s = wall > 0.0 ? static_cast<double>(v.size()) / wall : 0.0;
+    }
+    return s;
+}
+
+void solve_server::run(lane& ln) {
+    (void)hipSetDevice(device_);
+    const std::size_t n = static_cast<std::size_t>(ln.n);
+    double* db = static_cast<double*>(ln.d_b);
+    double* dx = static_cast<double*>(ln.d_x);
+    std::vector<gpu_solver::session_slot> states;
+    for (;;) {
+        std::vector<std::pair<int, std::shared_ptr<request>>> admit;
+        bool busy = false;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            work_cv_.wait(lk, [&] {
+                if (stop_ || !ln.queue.empty()) return true;
+                for (auto& r : ln.slot)
+                    if (r) return true;
+                return false;
+            });
+            if (stop_) return;
+            for (int c = 0; c < ln.kp && !ln.queue.empty(); ++c)
+                if (!ln.slot[c]) {
+                    ln.slot[c] = ln.queue.front();
+                    ln.queue.pop_front();
+                    admit.emplace_back(c, ln.slot[c]);
+                }
+            for (auto& r : ln.slot)
+                if (r) busy = true;
+        }
+        try {
+            if (!admit.empty()) {
+                std::vector<gpu_solver::session_column> cols;
+                for (auto& [c, rq] : admit) {
+                    double* dst = db + static_cast<std::size_t>(c) * n;
+                    check(hipMemcpyAsync(dst, rq->b.data(), n * sizeof(double), hipMemcpyHostToDevice, ln.stream),
+                          "upload b");
+                    cols.push_back({c, dst, rq->tol, rq->max_iter});
+                }
+                ln.solver->session_admit(cols, ln.stream);
+            }
+            if (!busy) continue;
+            ln.solver->session_step(opts_.iters_per_check, ln.stream);
+            ln.solver->session_read(states, ln.stream);
+            std::vector<int> fin;
+            std::vector<double*> dst;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                for (int c = 0; c < ln.kp; ++c)
+                    if (ln.slot[c] && states[c].done) {
+                        fin.push_back(c);
+                        dst.push_back(dx + static_cast<std::size_t>(c) * n);
+                    }
+            }
+            if (fin.empty()) continue;
+            ln.solver->session_collect(fin, dst, ln.stream);
+            std::vector<std::shared_ptr<request>> finished;
+            for (std::size_t j = 0; j < fin.size(); ++j) {
+                auto rq = ln.slot[fin[j]];
+                rq->x.resize(n);
+                check(hipMemcpy(rq->x.data(), dst[j], n * sizeof(double), hipMemcpyDeviceToHost), "download x");
+                const auto& st = states[fin[j]];
+                rq->res.iterations = st.iterations;
+                rq->res.residual = st.residual;
+                rq->res.status = st.status;
+                rq->res.lumped_offdiag = ln.solver->factor().lumped_offdiag;
+                if (st.status == 5) std::fill(rq->x.begin(), rq->x.end(), 0.0);
+                finished.push_back(rq);
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                const auto now = std::chrono::steady_clock::now();
+                for (std::size_t j = 0; j < fin.size(); ++j) {
+                    auto& rq = finished[j];
+                    rq->t_done = now;
+                    rq->done = true;
+                    latencies_ms_.push_back(std::chrono::duration<double, std::milli>(now - rq->t_submit).count());
+                    last_done_ = now;
+                    ln.slot[fin[j]] = nullptr;
+                }
+            }
+            done_cv_.notify_all();
+        } catch (const std::exception&) {
+            // A device failure ends this lane: its requests fail (status -1).
+            std::lock_guard<std::mutex> lk(mu_);
+            auto fail = [&](std::shared_ptr<request>& rq) {
+                if (!rq) return;
+                rq->res.status = -1;
+                rq->done = true;
+                rq = nullptr;
+            };
+            for (auto& r : ln.slot) fail(r);
+            for (auto& r : ln.queue) fail(r);
+            ln.queue.clear();
+            done_cv_.notify_all();
+            return;
+        }
+    }
+}
+
+}  // namespace apxchol_amd

@@ -211,6 +211,59 @@ trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel) {
     return T;
 }
 
+std::vector<std::uint32_t> relabel_within_levels(trsv_factor& T, const std::vector<node_t>& perm) {
+    const std::uint32_t m = T.m;
+    std::vector<std::uint32_t> vert(perm.size());
+    for (std::size_t v = 0; v < perm.size(); ++v) vert[perm[v]] = static_cast<std::uint32_t>(v);
+    std::vector<std::uint32_t> Q(m), order;
+    for (std::size_t l = 0; l + 1 < T.level_ptr.size(); ++l) {
+        const std::uint32_t a = T.level_ptr[l], b = T.level_ptr[l + 1];
+        order.resize(b - a);
+        for (std::uint32_t q = a; q < b; ++q) order[q - a] = q;
+        std::sort(order.begin(), order.end(), [&](std::uint32_t x, std::uint32_t y) { return vert[x] < vert[y]; });
+        for (std::uint32_t k = 0; k < b - a; ++k) Q[order[k]] = a + k;
+    }
+    auto relist = [&](std::vector<std::uint32_t>& ptr, std::vector<std::int32_t>& idx, std::vector<float>& val,
+                      std::vector<std::uint16_t>& h, bool descending) {
+        std::vector<std::uint32_t> nptr(static_cast<std::size_t>(m) + 1, 0);
+        for (std::uint32_t r = 0; r < m; ++r) nptr[Q[r] + 1] = ptr[r + 1] - ptr[r];
+        for (std::uint32_t r = 0; r < m; ++r) nptr[r + 1] += nptr[r];
+        std::vector<std::int32_t> nidx(idx.size());
+        std::vector<float> nval(val.size());
+        std::vector<std::uint16_t> nh(h.size());
+        std::vector<std::pair<std::int32_t, std::uint32_t>> ent;   // (new index, old slot)
+        for (std::uint32_t r = 0; r < m; ++r) {
+            ent.clear();
+            for (std::uint32_t p = ptr[r]; p < ptr[r + 1]; ++p)
+                ent.emplace_back(static_cast<std::int32_t>(Q[static_cast<std::uint32_t>(idx[p])]), p);
+            if (descending) std::sort(ent.begin(), ent.end(), [](auto& x, auto& y) { return x.first > y.first; });
+            else std::sort(ent.begin(), ent.end(), [](auto& x, auto& y) { return x.first < y.first; });
+            std::uint32_t o = nptr[Q[r]];
+            for (const auto& [j, p] : ent) {
+                nidx[o] = j;
+                if (!val.empty()) nval[o] = val[p];
+                if (!h.empty()) nh[o] = h[p];
+                ++o;
+            }
+        }
+        ptr.swap(nptr);
+        idx.swap(nidx);
+        val.swap(nval);
+        h.swap(nh);
+    };
+    relist(T.f_ptr, T.f_idx, T.f_val, T.f_h, false);
+    relist(T.b_ptr, T.b_idx, T.b_val, T.b_h, true);
+    auto move_rows = [&](std::vector<float>& a) {
+        if (a.empty()) return;
+        std::vector<float> b(a.size());
+        for (std::uint32_t r = 0; r < m; ++r) b[Q[r]] = a[r];
+        a.swap(b);
+    };
+    move_rows(T.diag);
+    move_rows(T.inv_scale);
+    return Q;
+}
+
 permuted_operator build_permuted_operator(const csc_view& A, const std::vector<node_t>& perm) {
     permuted_operator op;
     const std::int64_t n = A.n;

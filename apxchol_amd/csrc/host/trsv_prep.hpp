@@ -51,6 +51,14 @@ struct trsv_factor {
 // rel < 0 reads APXCHOL_FACTOR_DROP like the reference (unset = 1e-4, <= 0 off).
 trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel = -1.0);
 
+// Renumbers the rows of every level by the original vertex order (a level is
+// an independent set of the elimination, so any order inside it is the same
+// factor): position q moves to Q[q], levels keep their ranges. Rewrites T's
+// row / column lists (entries re-sorted: forward rows ascending, back
+// columns descending), diag and inv_scale, and returns Q (m entries). The
+// vertex at factor position q is the v with perm[v] == q.
+std::vector<std::uint32_t> relabel_within_levels(trsv_factor& T, const std::vector<node_t>& perm);
+
 // P A P^T as a full-symmetric CSR with rows' columns ascending
 // (solver/pcg_cuda.h:304-404, solve.cpp:320-476); values stored fp32 when every
 // one round-trips fp32 exactly.

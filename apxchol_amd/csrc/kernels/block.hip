@@ -1,17 +1,14 @@
 // Blocked k-right-hand-side PCG on gfx950: the SpMM, the two sweeps and the
 // CG vector passes for k columns at once. Block vectors are row-major
 // [rows][kp] (kp = the column count rounded up to a power of two, <= 64), so
-// gathering one dependency brings all k values in one or two cache lines, and
-// one thread works on one (row, column).
+// the values one gather needs for all columns are contiguous.
 //
-// Per-column arithmetic replays the single-right-hand-side kernels exactly:
-// a row of `len` entries handled there by G lanes (lane s takes entries s,
-// s + G, ... as an fma chain from 0, then an xor butterfly over the G lanes)
-// is computed here by one thread as the same G chains combined in the same
-// tree (bit-reversed pairwise order), so every column of a blocked sweep or
-// SpMM equals the single-vector result bit for bit. The reductions are per
-// column, two-stage and fixed-order (deterministic, not the single kernels'
-// tree).
+// The sweeps and the SpMM give every (row, column) the lanes the single-
+// vector kernels give that row (lane = column * G + chain: the same
+// lane-strided fma chains and the same xor butterfly), so every column of a
+// blocked sweep or SpMM equals the single-vector result bit for bit. The
+// per-column dot products are two-stage and fixed-order (deterministic, but
+// not the single kernels' tree).
 //
 // Each column carries its own pcg_state (convergence, stagnation, centring
 // schedule), so a column stops exactly as its own solve would; a batch_state
@@ -20,115 +17,93 @@
 
 namespace apxchol_amd {
 
-__device__ __forceinline__ int bitrev(int v, int bits) {
-    return bits == 0 ? 0 : static_cast<int>(__builtin_bitreverse32(static_cast<unsigned>(v)) >> (32 - bits));
-}
-
-// Pairwise combination stack of the butterfly replay: leaves pushed in
-// bit-reversed lane order combine like the xor butterfly (off = G/2 .. 1).
-template <class T>
-struct pair_stack {
-    T v[7];
-    int top = 0;
-    __device__ __forceinline__ void push(T x, int i) {
-        // i: index of this leaf in visiting order; merge while the count of
-        // leaves pushed so far (i + 1) has trailing zeros.
-        unsigned cnt = static_cast<unsigned>(i) + 1u;
-        while ((cnt & 1u) == 0u) {
-            x = v[--top] + x;
-            cnt >>= 1;
-        }
-        v[top++] = x;
-    }
-};
-
-// The single-vector row sum of `len` entries at [rb, rb + len) over G lanes,
-// for column c of a [*][kp] block `x`: lane s's fma chain over entries
-// s, s + G, ... (ascending), then the butterfly tree.
-template <class VT, class XT, class IT>
-__device__ __forceinline__ XT replay_row(unsigned rb, unsigned len, int G, const IT* __restrict__ idx,
-                                         const VT* __restrict__ val, const XT* x, int kp, int c) {
-    int bits = 0;
-    while ((1 << bits) < G) ++bits;
-    pair_stack<XT> stk;
-    for (int i = 0; i < G; ++i) {
-        const int s = bitrev(i, bits);
-        XT part = 0;
-        for (unsigned q = static_cast<unsigned>(s); q < len; q += static_cast<unsigned>(G))
-            part = fma(static_cast<XT>(val[rb + q]), x[static_cast<size_t>(idx[rb + q]) * kp + c], part);
-        stk.push(part, i);
-    }
-    return stk.v[0];
-}
-
 // ---- sweeps ----------------------------------------------------------------
 
-// One level of a blocked sweep: rows at sweep positions [p0, p1), kp threads
-// per row. Forward input: fp64 r minus the column's shift; back input: y
-// (times inv_scale^2 on the fp16 storage). Same lane grouping as sweep_batch:
-// G = lanes_for(len), rows longer than 64 * kEpl use the whole wave (G = 64).
+// One level of a blocked sweep: waves [w0, w1) of the blocked table (see
+// block_batch_table). Lane = column * G + chain inside a row's group, so the
+// arithmetic of (row, column) is sweep_batch's for that row, and the lanes
+// of one chain index gather the row's dependency for consecutive columns
+// from one contiguous span.
 template <class TI, class TO, bool F16>
-__global__ __launch_bounds__(kBlock) void sweep_levels_k(sweep_desc d, int p0, int p1, int kp,
-                                                         const TI* __restrict__ rhs, TO* out,
-                                                         const pcg_state* st, const batch_state* bs) {
+__global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsigned* __restrict__ wpos,
+                                                      const unsigned long long* __restrict__ wmask,
+                                                      const int* __restrict__ wcol0, int w0, int w1, int kp,
+                                                      const TI* __restrict__ rhs, TO* out, const pcg_state* st,
+                                                      const batch_state* bs) {
     if (bs && bs->active == 0) return;
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int c = t & (kp - 1);
-    const int pos = p0 + t / kp;
-    if (pos >= p1) return;
-    if (st && st[c].done) return;
+    const int w = w0 + static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    if (w >= w1) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned p = wpos[w];
+    const unsigned long long mask = wmask[w];
+    int pos, sub0;
+    if (mask) {
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const unsigned long long mu = mask & upto;
+        pos = static_cast<int>(p) + __popcll(mu) - 1;
+        sub0 = lane - (63 - __clzll(mu));
+    } else {
+        pos = static_cast<int>(p);
+        sub0 = lane;
+    }
     const int row = d.backward ? d.m - 1 - pos : pos;
     const unsigned rb = d.ptr[row];
     const unsigned len = d.ptr[row + 1] - rb;
-    int G = 1;
-    {
-        unsigned g = (len + kEpl - 1) / kEpl;
-        if (g > 1) {
-            g = 1u << (32 - __builtin_clz(g - 1));
-            G = g > 64 ? 64 : static_cast<int>(g);
-        }
-    }
-    TO part;
-    if constexpr (F16) {
-        int bits = 0;
-        while ((1 << bits) < G) ++bits;
-        pair_stack<TO> stk;
-        for (int i = 0; i < G; ++i) {
-            const int s = bitrev(i, bits);
-            TO acc = 0;
-            for (unsigned q = static_cast<unsigned>(s); q < len; q += static_cast<unsigned>(G)) {
-                const float v = static_cast<float>(__builtin_bit_cast(_Float16, d.h[rb + q]));
-                acc = fma(static_cast<TO>(v), out[static_cast<size_t>(d.idx[rb + q]) * kp + c], acc);
+    const int G = lanes_for(len);
+    const int c = mask ? sub0 / G : wcol0[w] + lane / G;
+    const int s = sub0 & (G - 1);
+    const bool active = mask ? sub0 < G * kp : true;
+    TO part = 0;
+    if (__builtin_expect(len > static_cast<unsigned>(kWave * kEpl), 0)) {
+        // Long row (G = 64, one column per wave): sweep_batch's chunked walk.
+        for (unsigned c0 = 0; c0 < len; c0 += kWave * kEpl) {
+            int jj[kEpl];
+            float vv[kEpl];
+            TO xv[kEpl];
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) {
+                const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
+                const bool valid = q < len;
+                load_entry<F16>(d, valid ? rb + q : rb + c0, jj[k], vv[k]);
+                if (!valid) vv[k] = 0.0f;
+                xv[k] = valid ? out[static_cast<size_t>(jj[k]) * kp + c] : static_cast<TO>(0);
             }
-            stk.push(acc, i);
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
         }
-        part = stk.v[0];
     } else {
-        int bits = 0;
-        while ((1 << bits) < G) ++bits;
-        pair_stack<TO> stk;
-        for (int i = 0; i < G; ++i) {
-            const int s = bitrev(i, bits);
-            TO acc = 0;
-            for (unsigned q = static_cast<unsigned>(s); q < len; q += static_cast<unsigned>(G)) {
-                const entry32 e = d.ent[rb + q];
-                acc = fma(static_cast<TO>(e.val), out[static_cast<size_t>(e.idx) * kp + c], acc);
-            }
-            stk.push(acc, i);
+        int jj[kEpl];
+        float vv[kEpl];
+        TO xv[kEpl];
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) {
+            const unsigned q = static_cast<unsigned>(s + k * G);
+            const bool valid = active && q < len;
+            load_entry<F16>(d, valid ? rb + q : (len > 0 ? rb : 0u), jj[k], vv[k]);
+            if (!valid) vv[k] = 0.0f;
+            xv[k] = valid ? out[static_cast<size_t>(jj[k]) * kp + c] : static_cast<TO>(0);
         }
-        part = stk.v[0];
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
     }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const TO o = __shfl_xor(part, off, kWave);
+        if (off < G) part += o;
+    }
+    if (!active || s != 0) return;
+    const size_t o = static_cast<size_t>(row) * kp + c;
     TO r;
     if (!d.backward) {
         const double shift = st ? st[c].shift : 0.0;
-        r = static_cast<TO>(static_cast<double>(rhs[static_cast<size_t>(row) * kp + c]) - shift);
+        r = static_cast<TO>(static_cast<double>(rhs[o]) - shift);
     } else if (d.inv_scale) {
         const double sc = static_cast<double>(d.inv_scale[row]);
-        r = static_cast<TO>(static_cast<double>(rhs[static_cast<size_t>(row) * kp + c]) * (sc * sc));
+        r = static_cast<TO>(static_cast<double>(rhs[o]) * (sc * sc));
     } else {
-        r = static_cast<TO>(rhs[static_cast<size_t>(row) * kp + c]);
+        r = static_cast<TO>(rhs[o]);
     }
-    out[static_cast<size_t>(row) * kp + c] = (r - part) / static_cast<TO>(d.diag[row]);
+    out[o] = (r - part) / static_cast<TO>(d.diag[row]);
 }
 
 // ---- per-column reductions -----------------------------------------------
@@ -137,8 +112,13 @@ __global__ __launch_bounds__(kBlock) void sweep_levels_k(sweep_desc d, int p0, i
 // columns, kBlock / kp rows per block), in row order, then over blocks in
 // block order (last block). Thread c < kp of the last block returns true
 // with the totals of column c in v.
+//
+// Lane mapping: thread t of the grid serves column ((t / G) % kp) and only
+// its chain-0 lanes (t % G == 0) carry values (the others hold exact zeros,
+// which the block partial skips); G = 1 is one thread per (row, column).
 template <int N>
-__device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* part, unsigned* counter, double* sh) {
+__device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* part, unsigned* counter, double* sh,
+                                             int G = 1) {
     const int c = threadIdx.x & (kp - 1);
     const int rows = kBlock / kp;
     __shared__ double col[N * 64];
@@ -147,10 +127,18 @@ __device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* par
     __syncthreads();
     __shared__ int is_last;
     if (threadIdx.x < kp) {
+        const int U = G * kp;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            double s = sh[k * kBlock + c];
-            for (int j = 1; j < rows; ++j) s += sh[k * kBlock + j * kp + c];
+            double s = 0.0;
+            if (U <= kBlock) {
+                for (int r = 0; r < kBlock / U; ++r) s += sh[k * kBlock + r * U + c * G];
+            } else {
+                // The block spans kBlock / G consecutive columns of one row.
+                const int first = static_cast<int>((static_cast<long long>(blockIdx.x) * kBlock / G) % kp);
+                const int off = (c - first + kp) % kp;
+                if (off < kBlock / G) s = sh[k * kBlock + off * G];
+            }
             col[k * 64 + c] = s;
         }
     }
@@ -169,13 +157,26 @@ __device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* par
     __syncthreads();
     if (!is_last) return false;
     __threadfence();
+    // Last block: thread (j, c) sums the partials of blocks j, j + rows, ...
+    // of column c (all loads in flight at once), then column c's `rows`
+    // sums are added in j order.
+    const int j = threadIdx.x / kp;
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0.0;
+    for (unsigned b = static_cast<unsigned>(j); b < gridDim.x; b += static_cast<unsigned>(rows))
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            acc[k] += __hip_atomic_load(&part[(static_cast<size_t>(b) * N + k) * kp + c], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k < N; ++k) sh[k * kBlock + threadIdx.x] = acc[k];
+    __syncthreads();
     if (threadIdx.x >= kp) return false;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        double s = 0.0;
-        for (unsigned b = 0; b < gridDim.x; ++b)
-            s += __hip_atomic_load(&part[(static_cast<size_t>(b) * N + k) * kp + c], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        double s = sh[k * kBlock + c];
+        for (int i = 1; i < rows; ++i) s += sh[k * kBlock + i * kp + c];
         v[k] = s;
     }
     if (c == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -188,8 +189,12 @@ inline int block_grid(long long rows, int kp) { return vec_blocks(rows * kp); }
 
 // ---- SpMM + p.Ap -----------------------------------------------------------
 
-// Y = A P for kp columns, each row replaying spmv_pAp's G-lane order; with
-// want_dot, p.Ap per column into st[c].pAp (breakdown ends that column).
+// Y = A P for kp columns with spmv_pAp's lanes: thread t serves row
+// t / (G kp), column (t / G) % kp, chain t % G (lane-strided entries, fma
+// from 0, butterfly over the G lanes), so every column gets the single
+// SpMV's bits. With want_dot, p.Ap per column into st[c].pAp (breakdown
+// ends that column). The grid stride is a multiple of G kp (host), so a
+// thread's column is fixed.
 template <class VT>
 __global__ __launch_bounds__(kBlock) void spmm_pAp_k(int n, int kp, int G, const unsigned* __restrict__ ptr,
                                                      const int* __restrict__ idx, const VT* __restrict__ val,
@@ -198,24 +203,41 @@ __global__ __launch_bounds__(kBlock) void spmm_pAp_k(int n, int kp, int G, const
                                                      int want_dot) {
     if (bs->active == 0) return;
     __shared__ double sh[kBlock];
-    const int c = threadIdx.x & (kp - 1);
-    const bool live = !st[c].done;
-    const int rows_per_grid = gridDim.x * (kBlock / kp);
+    const long long U = static_cast<long long>(G) * kp;
+    const long long t0 = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int c = static_cast<int>((t0 / G) % kp);
+    const int s0 = static_cast<int>(t0 % G);
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     double acc = 0.0;
-    if (live) {
-        for (int row = (blockIdx.x * kBlock + threadIdx.x) / kp; row < n; row += rows_per_grid) {
-            const unsigned b = ptr[row];
-            const double s = replay_row<VT, double, int>(b, ptr[row + 1] - b, G, idx, val, x, kp, c);
-            y[static_cast<size_t>(row) * kp + c] = s;
-            acc = fma(x[static_cast<size_t>(row) * kp + c], s, acc);
+    for (long long t = t0; t / U < n; t += stride) {
+        const int row = static_cast<int>(t / U);
+        const unsigned b = ptr[row], e = ptr[row + 1];
+        double sum = 0.0;
+        for (unsigned q = b + static_cast<unsigned>(s0); q < e; q += static_cast<unsigned>(G))
+            sum = fma(static_cast<double>(val[q]), x[static_cast<size_t>(idx[q]) * kp + c], sum);
+        for (int off = G / 2; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, G);
+        if (s0 == 0) {
+            const size_t o = static_cast<size_t>(row) * kp + c;
+            y[o] = sum;
+            acc = fma(x[o], sum, acc);
         }
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    if (col_grid_sum<1>(v, kp, part, counter, sh) && live) {
+    if (col_grid_sum<1>(v, kp, part, counter, sh, G) && !st[c].done && st[c].phase == 0) {
         st[c].pAp = v[0];
         if (v[0] <= 0.0) { st[c].done = 1; st[c].status = 2; }
     }
+}
+
+// Workgroups of the SpMM: a multiple of G kp / kBlock when one row's lanes
+// span several workgroups (so the grid stride keeps every thread's column).
+inline int spmm_grid(long long n, int kp, int G) {
+    const long long U = static_cast<long long>(G) * kp;
+    long long g = vec_blocks(n * U);
+    const long long q = U > kBlock ? U / kBlock : 1;
+    g = (g + q - 1) / q * q;
+    return static_cast<int>(g);
 }
 
 // ---- vector passes ---------------------------------------------------------
@@ -251,8 +273,9 @@ __device__ __forceinline__ void count_active(int kp, const pcg_state* st, batch_
     }
 }
 
-// rr, rs per column of r; the start of each column's solve (zero right-hand
-// side, max_iter, first centring decision).
+// rr, rs per column of r; the start of each joining column's solve (zero
+// right-hand side, max_iter, first centring decision). Other columns are
+// left as they are.
 __global__ __launch_bounds__(kBlock) void init_residual_k(int n, int kp, const double* __restrict__ r, pcg_state* st,
                                                           batch_state* bs, double* part, unsigned* counter) {
     __shared__ double sh[2 * kBlock];
@@ -266,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void init_residual_k(int n, int kp, const d
     }
     double v[2] = {q, s};
     const bool last = col_grid_sum<2>(v, kp, part, counter, sh);
-    if (last) {
+    if (last && !st[c].done && st[c].phase == 1) {
         pcg_state& t = st[c];
         t.rr = v[0];
         t.rs = v[1];
@@ -290,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void update_xr_k(int n, int kp, double* __r
     if (bs->active == 0) return;
     __shared__ double sh[2 * kBlock];
     const int c = threadIdx.x & (kp - 1);
-    const bool live = !st[c].done;
+    const bool live = !st[c].done && st[c].phase == 0;
     const double alpha = live ? st[c].rz / st[c].pAp : 0.0;
     const int rows_per_grid = gridDim.x * (kBlock / kp);
     double q = 0.0, s = 0.0;
@@ -349,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void dot_center_k(int n, int kp, const doub
         const double rz = v[0] - mu * t.rs;
         t.out_sum = v[1];
         t.mu = mu;
-        if (first) {
+        if (first || t.phase == 1) {
             t.rz = rz;
             t.beta = 0.0;
         } else {
@@ -368,12 +391,20 @@ __global__ __launch_bounds__(kBlock) void update_p_k(int n, int kp, double* __re
     const int c = threadIdx.x & (kp - 1);
     if (st[c].done) return;
     const double mu = st[c].mu, beta = st[c].beta;
+    const bool fresh = first || st[c].phase == 1;
     const int rows_per_grid = gridDim.x * (kBlock / kp);
     for (int row = (blockIdx.x * kBlock + threadIdx.x) / kp; row < n; row += rows_per_grid) {
         const size_t i = static_cast<size_t>(row) * kp + c;
         const double zi = static_cast<double>(out[i]) - mu;
-        p[i] = first ? zi : fma(beta, p[i], zi);
+        p[i] = fresh ? zi : fma(beta, p[i], zi);
     }
+}
+
+// End of an iteration: joining columns have had their first application.
+__global__ void end_iteration_k(int kp, pcg_state* st, batch_state* bs) {
+    if (bs->active == 0) return;
+    const int c = threadIdx.x;
+    if (c < kp && !st[c].done && st[c].phase == 1) st[c].phase = 0;
 }
 
 // Active-column count after update_xr_k (one block, kp threads).
@@ -387,6 +418,41 @@ __global__ __launch_bounds__(kBlock) void zero_tail_k(int n, int m, int kp, TO* 
     for (long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; t < total;
          t += static_cast<long long>(gridDim.x) * kBlock)
         a[static_cast<size_t>(m) * kp + t] = static_cast<TO>(0);
+}
+
+// Admission of columns into a running block: for j < ncol, column cols[j]
+// gets r = the right-hand side srcs[j] (n doubles, caller order) permuted,
+// and x = 0.
+__global__ __launch_bounds__(kBlock) void admit_scatter_k(int n, int kp, const unsigned* __restrict__ perm, int ncol,
+                                                          const int* __restrict__ cols,
+                                                          const double* const* __restrict__ srcs,
+                                                          double* __restrict__ r, double* __restrict__ x) {
+    const long long total = static_cast<long long>(n) * ncol;
+    for (long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; t < total;
+         t += static_cast<long long>(gridDim.x) * kBlock) {
+        const int j = static_cast<int>(t / n);
+        const int v = static_cast<int>(t - static_cast<long long>(j) * n);
+        const size_t q = static_cast<size_t>(perm[v]) * kp + cols[j];
+        r[q] = srcs[j][v];
+        x[q] = 0.0;
+    }
+}
+
+// Collection of finished columns: dsts[j][v] = x[perm[v]][cols[j]] - mean
+// (the Laplacian's min-norm centring, from st[c].out_sum, when use_mean).
+__global__ __launch_bounds__(kBlock) void gather_cols_k(int n, int kp, const unsigned* __restrict__ perm, int ncol,
+                                                        const int* __restrict__ cols, double* const* __restrict__ dsts,
+                                                        const double* __restrict__ x, const pcg_state* st,
+                                                        int use_mean) {
+    const long long total = static_cast<long long>(n) * ncol;
+    for (long long t = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; t < total;
+         t += static_cast<long long>(gridDim.x) * kBlock) {
+        const int j = static_cast<int>(t / n);
+        const int v = static_cast<int>(t - static_cast<long long>(j) * n);
+        const int c = cols[j];
+        const double sh = use_mean ? st[c].out_sum / static_cast<double>(n) : 0.0;
+        dsts[j][v] = x[static_cast<size_t>(perm[v]) * kp + c] - sh;
+    }
 }
 
 // Column sums into st[c].out_sum (the Laplacian min-norm centring of the

@@ -7,9 +7,9 @@
 #include <stdexcept>
 #include <string>
 
-#include "block.hip"
 #include "pcg.hip"
 #include "sweep.hip"
+#include "block.hip"
 
 namespace apxchol_amd {
 
@@ -252,17 +252,16 @@ void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s) {
 namespace {
 
 template <class TI, class TO, bool F16>
-void sweep_k_t(const sweep_desc& d, int kp, const void* rhs, void* out, const pcg_state* st, const batch_state* bs,
-               const int* level_pos, int nlevels, hipStream_t s) {
-    for (int l = 0; l < nlevels; ++l) {
-        const int p0 = level_pos[l], p1 = level_pos[l + 1];
-        if (p1 <= p0) continue;
-        const long long threads = static_cast<long long>(p1 - p0) * kp;
-        const int g = static_cast<int>((threads + kBlock - 1) / kBlock);
-        sweep_levels_k<TI, TO, F16><<<g, kBlock, 0, s>>>(d, p0, p1, kp, static_cast<const TI*>(rhs),
-                                                         static_cast<TO*>(out), st, bs);
+void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void* out, const pcg_state* st,
+               const batch_state* bs, hipStream_t s) {
+    for (int l = 0; l < b.nlevels; ++l) {
+        const int w0 = b.level_first[l], w1 = b.level_first[l + 1];
+        if (w1 <= w0) continue;
+        const int g = (w1 - w0 + kWavesPerBlock - 1) / kWavesPerBlock;
+        sweep_block<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b.pos, b.mask, b.col0, w0, w1, b.kp,
+                                                      static_cast<const TI*>(rhs), static_cast<TO*>(out), st, bs);
     }
-    check_launch("sweep_levels_k");
+    check_launch("sweep_block");
 }
 
 void check_kp(int kp) {
@@ -271,18 +270,18 @@ void check_kp(int kp) {
 
 }  // namespace
 
-void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, int kp, const void* rhs, void* out,
-                    const pcg_state* st, const batch_state* bs, const int* level_pos, int nlevels, hipStream_t s) {
-    check_kp(kp);
+void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sched& b, const void* rhs, void* out,
+                    const pcg_state* st, const batch_state* bs, hipStream_t s) {
+    check_kp(b.kp);
     if (p == vec_prec::fp64) {
-        if (fp16) sweep_k_t<double, double, true>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
-        else sweep_k_t<double, double, false>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+        if (fp16) sweep_k_t<double, double, true>(d, b, rhs, out, st, bs, s);
+        else sweep_k_t<double, double, false>(d, b, rhs, out, st, bs, s);
     } else if (!d.backward) {
-        if (fp16) sweep_k_t<double, float, true>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
-        else sweep_k_t<double, float, false>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+        if (fp16) sweep_k_t<double, float, true>(d, b, rhs, out, st, bs, s);
+        else sweep_k_t<double, float, false>(d, b, rhs, out, st, bs, s);
     } else {
-        if (fp16) sweep_k_t<float, float, true>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
-        else sweep_k_t<float, float, false>(d, kp, rhs, out, st, bs, level_pos, nlevels, s);
+        if (fp16) sweep_k_t<float, float, true>(d, b, rhs, out, st, bs, s);
+        else sweep_k_t<float, float, false>(d, b, rhs, out, st, bs, s);
     }
 }
 
@@ -290,7 +289,8 @@ void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx
                    const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
                    bool want_dot, hipStream_t s) {
     check_kp(kp);
-    const int g = block_grid(n, kp);
+    if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) throw std::invalid_argument("apxchol_amd: SpMM lanes");
+    const int g = spmm_grid(n, kp, lanes);
     if (val32)
         spmm_pAp_k<float><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const float*>(val), x, y, st, bs,
                                                part, counter, want_dot ? 1 : 0);
@@ -359,6 +359,31 @@ void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, 
     if (p == vec_prec::fp64) sum_cols_k<double><<<g, kBlock, 0, s>>>(n, kp, static_cast<const double*>(x), st, part, counter);
     else sum_cols_k<float><<<g, kBlock, 0, s>>>(n, kp, static_cast<const float*>(x), st, part, counter);
     check_launch("sum_cols_k");
+}
+
+void launch_permute(int m, const unsigned* Q, const double* src, double* dst, bool in, hipStream_t s) {
+    permute_vec<<<vec_blocks(m), kBlock, 0, s>>>(m, Q, src, dst, in ? 1 : 0);
+    check_launch("permute_vec");
+}
+
+void launch_end_iteration_k(int kp, pcg_state* st, batch_state* bs, hipStream_t s) {
+    end_iteration_k<<<1, 64, 0, s>>>(kp, st, bs);
+    check_launch("end_iteration_k");
+}
+
+void launch_admit_scatter_k(int n, int kp, const unsigned* perm, int ncol, const int* cols,
+                            const double* const* srcs, double* r, double* x, hipStream_t s) {
+    if (ncol <= 0) return;
+    admit_scatter_k<<<vec_blocks(static_cast<long long>(n) * ncol), kBlock, 0, s>>>(n, kp, perm, ncol, cols, srcs, r, x);
+    check_launch("admit_scatter_k");
+}
+
+void launch_gather_cols_k(int n, int kp, const unsigned* perm, int ncol, const int* cols, double* const* dsts,
+                          const double* x, const pcg_state* st, bool use_mean, hipStream_t s) {
+    if (ncol <= 0) return;
+    gather_cols_k<<<vec_blocks(static_cast<long long>(n) * ncol), kBlock, 0, s>>>(n, kp, perm, ncol, cols, dsts, x, st,
+                                                                                  use_mean ? 1 : 0);
+    check_launch("gather_cols_k");
 }
 
 void launch_shift_from_sum_k(int kp, pcg_state* st, bool center, hipStream_t s) {

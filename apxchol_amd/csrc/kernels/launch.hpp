@@ -84,13 +84,23 @@ void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* par
 void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* out, double* z, const pcg_state* st,
                          bool center, hipStream_t s);
 void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s);
+void launch_permute(int m, const unsigned* Q, const double* src, double* dst, bool in, hipStream_t s);
 
 // ---- blocked solves: kp columns (a power of two <= 64), row-major blocks ----
 
-// One blocked sweep as one launch per level; level_pos: host array of the
-// levels' first sweep positions (nlevels + 1 entries, in sweep order).
-void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, int kp, const void* rhs, void* out,
-                    const pcg_state* st, const batch_state* bs, const int* level_pos, int nlevels, hipStream_t s);
+// Wave table of a blocked sweep for one block width (device arrays) and its
+// level boundaries (host: first wave of each level, nlevels + 1 entries).
+struct block_sched {
+    int kp = 0;
+    const unsigned* pos = nullptr;
+    const unsigned long long* mask = nullptr;
+    const int* col0 = nullptr;
+    const int* level_first = nullptr;
+    int nlevels = 0;
+};
+// One blocked sweep as one launch per level.
+void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sched& bsc, const void* rhs, void* out,
+                    const pcg_state* st, const batch_state* bs, hipStream_t s);
 void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx, const void* val, bool val32,
                    const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
                    bool want_dot, hipStream_t s);
@@ -110,6 +120,12 @@ void launch_zero_tail_k(vec_prec p, int n, int m, int kp, void* a, hipStream_t s
 void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, unsigned* counter,
                        hipStream_t s);
 void launch_shift_from_sum_k(int kp, pcg_state* st, bool center, hipStream_t s);
+void launch_end_iteration_k(int kp, pcg_state* st, batch_state* bs, hipStream_t s);
+// cols, srcs / dsts: device arrays of ncol entries.
+void launch_admit_scatter_k(int n, int kp, const unsigned* perm, int ncol, const int* cols,
+                            const double* const* srcs, double* r, double* x, hipStream_t s);
+void launch_gather_cols_k(int n, int kp, const unsigned* perm, int ncol, const int* cols, double* const* dsts,
+                          const double* x, const pcg_state* st, bool use_mean, hipStream_t s);
 void launch_gather_k(vec_prec p, int n, int k, int kp, const unsigned* perm, const void* x, double* X,
                      const pcg_state* st, bool use_mean, hipStream_t s);
 

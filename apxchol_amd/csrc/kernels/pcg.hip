@@ -271,4 +271,15 @@ __global__ __launch_bounds__(kBlock) void apply_gather(int n, const unsigned* __
         zdst[v] = static_cast<double>(out[perm[v]]) - mu;
 }
 
+// Between factor order and storage order: dst[Q[q]] = src[q] (in) or
+// dst[q] = src[Q[q]] (out), q < m.
+__global__ __launch_bounds__(kBlock) void permute_vec(int m, const unsigned* __restrict__ Q,
+                                                      const double* __restrict__ src, double* __restrict__ dst,
+                                                      int in) {
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < m; q += gridDim.x * kBlock) {
+        if (in) dst[Q[q]] = src[q];
+        else dst[q] = src[Q[q]];
+    }
+}
+
 }  // namespace apxchol_amd

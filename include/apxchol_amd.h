@@ -47,6 +47,8 @@ typedef struct {
     int32_t tail_rows;       /* hybrid: levels with fewer rows go to the one-workgroup tail (0 = default) */
     int32_t graph;           /* 1 (default): each PCG iteration replays one captured HIP graph;
                                 0: eager launches (also used while kernel timing is on) */
+    int32_t relabel;         /* 1 (default): device vectors order every level's rows by original
+                                vertex (same factor, local gathers); 0: the factor order */
 } apx_options;
 
 /* Fill `o` with the defaults above. */
@@ -151,6 +153,33 @@ apx_status apx_solver_solve_batch_host(apx_solver_t s, int32_t k, const double* 
  * d_Z + c*n. Non-centring applications give each column the single apply's
  * bits. */
 apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, double* d_Z, void* stream);
+
+/* ---- serving (no counterpart in the reference) ---------------------------
+ * A request loop over resident factors: each factor keeps one blocked PCG of
+ * `slots` columns running on its own stream; requests are admitted into free
+ * columns every `iters_per_check` iterations and released when they finish
+ * (continuous batching), and factors are admitted against `memory_budget`
+ * bytes of HBM (0: 90% of the free memory). A request's result is the same
+ * bits alone or among others. A solver added to a server is used by the
+ * server only (the server keeps it alive after apx_solver_destroy). */
+typedef struct apx_server_s* apx_server_t;
+typedef struct {
+    int64_t submitted, completed, factors;
+    double p50_ms, p99_ms, mean_ms; /* submit -> completion latency */
+    double solves_per_s;            /* completed / (last completion - first submission) */
+    uint64_t memory_used, memory_budget;
+} apx_server_stats;
+apx_status apx_server_create(uint64_t memory_budget, int32_t slots, int32_t iters_per_check, apx_server_t* out);
+void apx_server_destroy(apx_server_t srv);
+apx_status apx_server_add(apx_server_t srv, apx_solver_t solver, int32_t* factor_id);
+/* b: n host doubles (copied); tol / max_iter as in apx_solver_solve. */
+apx_status apx_server_submit(apx_server_t srv, int32_t factor, const double* b, double tol, int32_t max_iter,
+                             int64_t* request_id);
+/* Waits up to timeout_s seconds (< 0: no limit); *ready = 1 and x (n host
+ * doubles) / info filled when the request is done (it is then forgotten). */
+apx_status apx_server_wait(apx_server_t srv, int64_t request_id, double* x, apx_solve_info* info, double timeout_s,
+                           int32_t* ready);
+apx_status apx_server_get_stats(apx_server_t srv, apx_server_stats* st);
 
 /* cpu_solver::apply(r) = apx_cholesky::_solve_impl (solve.h:103, preconditioner.h:344-353):
  * z = M^-1 r with the center-k schedule (every K-th application centres). */

@@ -1,0 +1,113 @@
+"""GPU tests of the serving loop (continuous batching over resident factors).
+
+Bars: every request's result is bit-identical whether it ran alone in its
+factor's block or admitted mid-flight among others (the per-column
+arithmetic does not depend on the other columns or the slot); it meets its
+own tolerance (true residual below 2 x tol); it agrees with a single-vector
+solve within one iteration; factors beyond the memory budget are refused."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import apxchol_amd as ax
+from apxchol_amd import matgen as mg
+
+pytestmark = pytest.mark.gpu
+
+FACTORS = {
+    "grid2d_64": lambda: mg.grid_laplacian((64, 64)),
+    "sddm3d_16": lambda: mg.add_excess(mg.grid_laplacian((16, 16, 16), "lognormal")),
+    "rmat_12": lambda: mg.rmat_laplacian(12),
+}
+_cache = {}
+
+
+def factor(name):
+    if name not in _cache:
+        A = FACTORS[name]()
+        _cache[name] = (A, ax.factorize(A, seed=0))
+    return _cache[name]
+
+
+def make_requests(count, seed):
+    rng = np.random.default_rng(seed)
+    names = list(FACTORS)
+    reqs = []
+    for i in range(count):
+        name = names[int(rng.integers(len(names)))]
+        A, F = factor(name)
+        b = rng.standard_normal(A.shape[0])
+        if not F.sddm:
+            b = mg.project_range(A, b)
+        tol = [1e-6, 1e-8, 1e-10][int(rng.integers(3))]
+        reqs.append((name, b, tol, float(rng.exponential(0.004))))
+    return reqs
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+@pytest.mark.parametrize("slots", [8, 32])
+def test_serving_mix_equals_alone(gpu, slots):
+    print(f"[case] serving slots={slots}", flush=True)
+    reqs = make_requests(36, 7)
+    # In the mix: Poisson-like arrivals from a producer thread while the
+    # server iterates.
+    mix = ax.Server(slots=slots, iters_per_check=2)
+    fid = {name: mix.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
+    ids = [None] * len(reqs)
+
+    def produce():
+        for i, (name, b, tol, gap) in enumerate(reqs):
+            time.sleep(gap)
+            ids[i] = mix.submit(fid[name], b, tol)
+
+    th = threading.Thread(target=produce)
+    th.start()
+    th.join()
+    mixed = [mix.wait(r, timeout=120) for r in ids]
+    st = mix.stats()
+    assert st["completed"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0 and st["solves_per_s"] > 0
+    mix.close()
+    # Alone: the same requests one at a time, each alone in its block.
+    alone = ax.Server(slots=slots, iters_per_check=2)
+    fid2 = {name: alone.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
+    for i, (name, b, tol, _) in enumerate(reqs):
+        x1, r1 = alone.wait(alone.submit(fid2[name], b, tol), timeout=120)
+        xm, rm = mixed[i]
+        assert r1.status == rm.status == 1, (i, r1.status, rm.status)
+        assert r1.iterations == rm.iterations, i
+        assert np.array_equal(bits(x1), bits(xm)), i
+        A, _ = factor(name)
+        assert np.linalg.norm(b - A @ xm) <= 2 * tol * np.linalg.norm(b), (i, name, tol)
+    alone.close()
+
+
+def test_serving_agrees_with_single_solves(gpu):
+    import torch
+    reqs = make_requests(9, 11)
+    srv = ax.Server(slots=8)
+    solvers = {name: ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32") for name in FACTORS}
+    fid = {name: srv.add(solvers[name]) for name in FACTORS}
+    ids = [srv.submit(fid[name], b, tol) for name, b, tol, _ in reqs]
+    out = [srv.wait(r, timeout=120) for r in ids]
+    srv.close()
+    for (name, b, tol, _), (x, r) in zip(reqs, out):
+        A, F = factor(name)
+        one = ax.Solver(A, factor=F, precision="fp32", tol=tol)
+        o = one.solve(torch.tensor(b, device=gpu))
+        one.close()
+        assert abs(o.iterations - r.iterations) <= 1
+        x1 = o.x.cpu().numpy()
+        assert np.linalg.norm(x - x1) <= 1e-4 * np.linalg.norm(x1) + 1e-300
+
+
+def test_serving_memory_budget(gpu):
+    A, F = factor("sddm3d_16")
+    srv = ax.Server(memory_budget=1 << 20, slots=8)
+    with pytest.raises(RuntimeError, match="memory budget"):
+        srv.add(ax.Solver(A, factor=F))
+    srv.close()
