@@ -65,6 +65,7 @@ _SIGS = {
     "apx_solver_solve_batch": ([P, I32, P, P, F64, I32, P, P], ctypes.c_int),
     "apx_solver_apply_batch": ([P, I32, P, P, P], ctypes.c_int),
     "apx_server_create": ([ctypes.c_uint64, I32, I32, P], ctypes.c_int),
+    "apx_solver_autotune": ([P, I32, P, P], ctypes.c_int),
     "apx_server_destroy": ([P], None),
     "apx_server_add": ([P, P, P], ctypes.c_int),
     "apx_server_submit": ([P, I32, P, F64, I32, P], ctypes.c_int),
@@ -262,8 +263,11 @@ class Solver:
             _check(lib.apx_solver_create_with_factor(n, cp.ctypes.data, ri.ctypes.data, v.ctypes.data, factor._h,
                                                      ctypes.byref(self._opts), ctypes.byref(h)))
         self._h = h
+        self._refresh_info()
+
+    def _refresh_info(self):
         inf = SolverInfo()
-        _check(lib.apx_solver_info_get(h, ctypes.byref(inf)))
+        _check(load_library().apx_solver_info_get(self._h, ctypes.byref(inf)))
         self.info = {k: (list(v) if isinstance(v, ctypes.Array) else v)
                      for k, v in ((k, getattr(inf, k)) for k, _ in SolverInfo._fields_)}
 
@@ -329,6 +333,18 @@ class Solver:
                            float(i.solve_vram_mb)) for i in list(infos)[:k]]
         return X, res
 
+    def autotune(self, max_block_width: int = 64, stream=None) -> dict:
+        """Measure and adopt the fastest SpMV lanes; record the best block
+        width for blocked solves (see apx_solver_autotune)."""
+        import torch
+        r = AutotuneResult()
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().apx_solver_autotune(self._h, int(max_block_width), ctypes.byref(r), s))
+        out = {"spmv_lanes": r.spmv_lanes, "block_width": r.block_width, "spmv_us": list(r.spmv_us),
+               "sweep_pair_us": list(r.sweep_pair_us)}
+        self._refresh_info()
+        return out
+
     def apply_batch(self, R, out=None, stream=None):
         """Preconditioner applied to the k rows of the device tensor R (k, n)
         at once (one application of the centring schedule)."""
@@ -377,6 +393,10 @@ class Solver:
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(load_library().apx_solver_sweep(self._h, int(backward), _ptr(rhs), _ptr(out), s))
         return out
+
+
+class AutotuneResult(ctypes.Structure):
+    _fields_ = [("spmv_lanes", I32), ("block_width", I32), ("spmv_us", F64 * 7), ("sweep_pair_us", F64 * 4)]
 
 
 class ServerStats(ctypes.Structure):

@@ -310,12 +310,25 @@ apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, 
     });
 }
 
+apx_status apx_solver_autotune(apx_solver_t s, int32_t max_block_width, apx_autotune_result* out, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        const auto r = s->s->autotune(max_block_width > 0 ? max_block_width : 64, static_cast<hipStream_t>(stream));
+        if (out) {
+            out->spmv_lanes = r.spmv_lanes;
+            out->block_width = r.block_width;
+            for (int k = 0; k < 7; ++k) out->spmv_us[k] = r.spmv_us[k];
+            for (int k = 0; k < 4; ++k) out->sweep_pair_us[k] = r.sweep_pair_us[k];
+        }
+    });
+}
+
 apx_status apx_server_create(uint64_t memory_budget, int32_t slots, int32_t iters_per_check, apx_server_t* out) {
     return guarded([&] {
         need(out, "out");
         server_options o;
         o.memory_budget = memory_budget;
-        o.slots = slots > 0 ? slots : 32;
+        o.slots = slots >= 0 ? slots : 32;
         o.iters_per_check = iters_per_check > 0 ? iters_per_check : 2;
         auto h = std::make_unique<apx_server_s>();
         h->s = std::make_unique<solve_server>(o);

@@ -436,8 +436,9 @@ struct gpu_solver::device_state {
     // Blocked solves: the block buffers (allocated on first use for the
     // widest block seen) and the sweep tables per block width.
     struct block_tables {
-        dbuf f_pos, f_mask, f_col0, b_pos, b_mask, b_col0;
-        std::vector<int> f_lf, b_lf;
+        dbuf f_pos, f_mask, f_col0, b_pos, b_mask, b_col0, f_lf_dev, b_lf_dev;
+        std::vector<int> f_lf, b_lf, f_lpos, b_lpos;
+        std::vector<int> f_seg[3], b_seg[3];   // l0, l1, tail flag
         block_sched f, b;
     };
     struct block_bufs {
@@ -1048,13 +1049,50 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
             dc.upload(bt.col0);
             std::vector<int>& lf = backward ? t->b_lf : t->f_lf;
             lf = bt.level_first;
+            dbuf& lfd = backward ? t->b_lf_dev : t->f_lf_dev;
+            lfd.upload(lf);
+            // First sweep position of each level.
+            std::vector<int>& lpos = backward ? t->b_lpos : t->f_lpos;
+            const std::size_t L = T_.level_ptr.size() - 1;
+            lpos.resize(L + 1);
+            for (std::size_t l = 0; l <= L; ++l)
+                lpos[l] = static_cast<int>(backward ? T_.m - T_.level_ptr[L - l] : T_.level_ptr[l]);
+            // Segments: runs of levels of at most kTailWaves waves whose rows
+            // fit the LDS go to one workgroup; the others are launched.
+            std::vector<int>(&seg)[3] = backward ? t->b_seg : t->f_seg;
+            const int cap = sweep_block_tail_max_rows(dev_->prec, kp);
+            const bool tails = opts_.schedule == sweep_schedule::hybrid && cap > 0;
+            const int nl = static_cast<int>(L);
+            for (int l = 0; l < nl;) {
+                auto small = [&](int q) { return tails && lf[q + 1] - lf[q] <= kTailWaves; };
+                if (!small(l)) {
+                    seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
+                    ++l;
+                    continue;
+                }
+                int e = l;
+                while (e < nl && small(e) && lpos[e + 1] - lpos[l] <= cap) ++e;
+                if (e == l) {   // one small level wider than the LDS: launch it
+                    seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
+                    ++l;
+                    continue;
+                }
+                seg[0].push_back(l); seg[1].push_back(e); seg[2].push_back(1);
+                l = e;
+            }
             block_sched& sc = backward ? t->b : t->f;
             sc.kp = kp;
             sc.pos = dp.as<unsigned>();
             sc.mask = dm.as<unsigned long long>();
             sc.col0 = dc.as<int>();
             sc.level_first = lf.data();
+            sc.level_first_dev = lfd.as<int>();
+            sc.level_pos = lpos.data();
             sc.nlevels = static_cast<int>(lf.size()) - 1;
+            sc.seg_l0 = seg[0].data();
+            sc.seg_l1 = seg[1].data();
+            sc.seg_tail = seg[2].data();
+            sc.nseg = static_cast<int>(seg[0].size());
         }
         it = b.tables.emplace(kp, std::move(t)).first;
     }
@@ -1128,6 +1166,71 @@ void gpu_solver::fresh_state(pcg_state& h, double tol, int max_iter) const {
     h.n = static_cast<int>(n_);
     h.prev_check = 1.0;
     h.phase = 1;
+}
+
+// ---- autotuner ----------------------------------------------------------
+
+gpu_solver::autotune_result gpu_solver::autotune(int max_block_width, hipStream_t s) const {
+    auto& d = *dev_;
+    autotune_result res;
+    const int n = static_cast<int>(n_);
+    if (n == 0) return res;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    auto timed = [&](auto&& launch, int reps) {
+        launch();   // warm-up
+        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) launch();
+        hip_check(hipEventRecord(e1, s), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return 1e3 * static_cast<double>(ms) / reps;
+    };
+    try {
+        // SpMV lanes (no dot product: no solver state is touched).
+        hip_check(hipMemsetAsync(d.p.get(), 0, static_cast<std::size_t>(n) * sizeof(double), s), "memset");
+        double best = 1e300;
+        for (int k = 0, lanes = 1; lanes <= 64; ++k, lanes <<= 1) {
+            res.spmv_us[k] = timed([&] {
+                launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, lanes,
+                            d.p.as<double>(), d.ap.as<double>(), nullptr, nullptr, nullptr, false, s);
+            }, 5);
+            if (res.spmv_us[k] < best) { best = res.spmv_us[k]; res.spmv_lanes = lanes; }
+        }
+        d.lanes = res.spmv_lanes;
+        stats_.spmv_lanes = res.spmv_lanes;
+        // Block width: one forward + back blocked sweep per candidate.
+        double best_rate = 0.0;
+        for (int k = 0, kp = 8; kp <= 64 && kp <= std::max(8, max_block_width); ++k, kp <<= 1) {
+            prepare_block(kp);
+            block_scheds(kp);
+            auto& b = d.blk;
+            const auto& t = *b.tables.at(kp);
+            const std::size_t vb = static_cast<std::size_t>(n) * kp * sizeof(double);
+            hip_check(hipMemsetAsync(b.r.get(), 0, vb, s), "memset");
+            hip_check(hipMemsetAsync(b.y.get(), 0, vb, s), "memset");
+            res.sweep_pair_us[k] = timed([&] {
+                launch_sweep_k(d.fwd, d.prec, d.fp16, t.f, b.r.get(), b.y.get(), nullptr, nullptr, s);
+                launch_sweep_k(d.bck, d.prec, d.fp16, t.b, b.y.get(), b.z.get(), nullptr, nullptr, s);
+            }, 2);
+            const double rate = kp / res.sweep_pair_us[k];
+            if (rate > best_rate) { best_rate = rate; res.block_width = kp; }
+        }
+        tuned_block_width_ = res.block_width;
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // Captured graphs hold the old lanes.
+    if (d.iter_exec) { (void)hipGraphExecDestroy(d.iter_exec); d.iter_exec = nullptr; }
+    if (d.iter_graph) { (void)hipGraphDestroy(d.iter_graph); d.iter_graph = nullptr; }
+    d.blk.drop_graph();
+    return res;
 }
 
 // ---- serving sessions: a running block that admits and releases columns ----

@@ -118,6 +118,20 @@ public:
     void session_read(std::vector<session_slot>& out, hipStream_t s) const;
     void session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s) const;
 
+    // Autotuner: measures on this device, for this factor and operator, the
+    // SpMV lanes per row (powers of two; the single SpMV and the blocked
+    // SpMM then use the winner) and the block width of blocked solves (the
+    // width with the most columns swept per second; used by servers that
+    // ask for automatic slots). Captured graphs are rebuilt afterwards.
+    struct autotune_result {
+        int spmv_lanes = 0;
+        double spmv_us[7] = {};           // lanes 1, 2, ..., 64
+        int block_width = 0;
+        double sweep_pair_us[4] = {};     // block widths 8, 16, 32, 64: forward + back sweep
+    };
+    autotune_result autotune(int max_block_width, hipStream_t s) const;
+    int preferred_block_width() const { return tuned_block_width_; }
+
     // k preconditioner applications at once (k calls of cpu_solver::apply,
     // solve.h:103, counting as ONE application of the center-k schedule).
     void apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const;
@@ -161,9 +175,10 @@ private:
 
     solve_options opts_;
     std::vector<std::uint32_t> relabel_;   // factor position -> storage position (empty: identity)
+    mutable int tuned_block_width_ = 0;
     factorization F_;
     trsv_factor T_;
-    solver_stats stats_;
+    mutable solver_stats stats_;
     std::int64_t n_ = 0;
     int center_k_ = 10;
     mutable std::uint64_t apply_count_ = 0;

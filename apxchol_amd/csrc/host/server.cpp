@@ -23,7 +23,7 @@ solve_server::solve_server(const server_options& opts) : opts_(opts) {
         check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
         opts_.memory_budget = static_cast<std::uint64_t>(static_cast<double>(fr) * 0.9);
     }
-    if (opts_.slots < 1 || opts_.slots > 64) throw std::invalid_argument("server: slots must be in [1, 64]");
+    if (opts_.slots < 0 || opts_.slots > 64) throw std::invalid_argument("server: slots must be in [0, 64]");
     opts_.iters_per_check = std::max(1, opts_.iters_per_check);
 }
 
@@ -44,8 +44,10 @@ solve_server::~solve_server() {
 
 int solve_server::add(std::shared_ptr<gpu_solver> solver) {
     if (!solver) throw std::invalid_argument("server: null solver");
+    // slots = 0: the factor's autotuned block width (32 if not tuned).
+    const int want = opts_.slots > 0 ? opts_.slots : (solver->preferred_block_width() > 0 ? solver->preferred_block_width() : 32);
     int kp = 1;
-    while (kp < opts_.slots) kp <<= 1;
+    while (kp < want) kp <<= 1;
     const std::int64_t n = solver->rows();
     // Solver arrays + the block (r, x, p, Ap, y, z) + the b / x staging.
     const std::uint64_t need = solver->stats().device_bytes +

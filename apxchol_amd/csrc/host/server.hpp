@@ -29,7 +29,8 @@ namespace apxchol_amd {
 
 struct server_options {
     std::uint64_t memory_budget = 0;   // bytes of HBM for factors + blocks (0: 90% of the free memory)
-    int slots = 32;                    // columns per factor block (power of two <= 64 after rounding)
+    int slots = 32;                    // columns per factor block (power of two <= 64 after rounding;
+                                       // 0: each factor's autotuned block width)
     int iters_per_check = 2;           // PCG iterations between admissions / completions
 };
 

@@ -106,6 +106,108 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
     out[o] = (r - part) / static_cast<TO>(d.diag[row]);
 }
 
+// A run of small levels of a blocked sweep, walked by ONE workgroup of
+// kTailBlock threads: levels [l0, l1) of the wave table (each at most
+// kTailWaves waves), sweep positions [p0, p1). The run's outputs live in LDS
+// ([p1 - p0][kp]) as well as in `out`; a dependency inside the run is read
+// from LDS after the level barrier, one before the run from `out` (final
+// when the kernel starts). The per-(row, column) arithmetic is sweep_block's.
+template <class TI, class TO, bool F16>
+__global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, const unsigned* __restrict__ wpos,
+                                                               const unsigned long long* __restrict__ wmask,
+                                                               const int* __restrict__ wcol0,
+                                                               const int* __restrict__ level_first, int l0, int l1,
+                                                               int p0, int kp, const TI* __restrict__ rhs, TO* out,
+                                                               const pcg_state* st, const batch_state* bs) {
+    if (bs && bs->active == 0) return;
+    extern __shared__ unsigned char btail_lds[];
+    TO* lds = reinterpret_cast<TO*>(btail_lds);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    for (int l = l0; l < l1; ++l) {
+        const int w = level_first[l] + wv;
+        if (w < level_first[l + 1]) {
+            const unsigned p = wpos[w];
+            const unsigned long long mask = wmask[w];
+            int pos, sub0;
+            if (mask) {
+                const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+                const unsigned long long mu = mask & upto;
+                pos = static_cast<int>(p) + __popcll(mu) - 1;
+                sub0 = lane - (63 - __clzll(mu));
+            } else {
+                pos = static_cast<int>(p);
+                sub0 = lane;
+            }
+            const int row = d.backward ? d.m - 1 - pos : pos;
+            const unsigned rb = d.ptr[row];
+            const unsigned len = d.ptr[row + 1] - rb;
+            const int G = lanes_for(len);
+            const int c = mask ? sub0 / G : wcol0[w] + lane / G;
+            const int s = sub0 & (G - 1);
+            const bool active = mask ? sub0 < G * kp : true;
+            auto value = [&](int j) -> TO {
+                const int pj = d.backward ? d.m - 1 - j : j;
+                return pj >= p0 ? lds[static_cast<size_t>(pj - p0) * kp + c] : out[static_cast<size_t>(j) * kp + c];
+            };
+            TO part = 0;
+            if (__builtin_expect(len > static_cast<unsigned>(kWave * kEpl), 0)) {
+                for (unsigned c0 = 0; c0 < len; c0 += kWave * kEpl) {
+                    int jj[kEpl];
+                    float vv[kEpl];
+                    TO xv[kEpl];
+#pragma unroll
+                    for (int k = 0; k < kEpl; ++k) {
+                        const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
+                        const bool valid = q < len;
+                        load_entry<F16>(d, valid ? rb + q : rb + c0, jj[k], vv[k]);
+                        if (!valid) vv[k] = 0.0f;
+                        xv[k] = valid ? value(jj[k]) : static_cast<TO>(0);
+                    }
+#pragma unroll
+                    for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
+                }
+            } else {
+                int jj[kEpl];
+                float vv[kEpl];
+                TO xv[kEpl];
+#pragma unroll
+                for (int k = 0; k < kEpl; ++k) {
+                    const unsigned q = static_cast<unsigned>(s + k * G);
+                    const bool valid = active && q < len;
+                    load_entry<F16>(d, valid ? rb + q : (len > 0 ? rb : 0u), jj[k], vv[k]);
+                    if (!valid) vv[k] = 0.0f;
+                    xv[k] = valid ? value(jj[k]) : static_cast<TO>(0);
+                }
+#pragma unroll
+                for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const TO o = __shfl_xor(part, off, kWave);
+                if (off < G) part += o;
+            }
+            if (active && s == 0) {
+                const size_t o = static_cast<size_t>(row) * kp + c;
+                TO r;
+                if (!d.backward) {
+                    const double shift = st ? st[c].shift : 0.0;
+                    r = static_cast<TO>(static_cast<double>(rhs[o]) - shift);
+                } else if (d.inv_scale) {
+                    const double sc = static_cast<double>(d.inv_scale[row]);
+                    r = static_cast<TO>(static_cast<double>(rhs[o]) * (sc * sc));
+                } else {
+                    r = static_cast<TO>(rhs[o]);
+                }
+                const TO v = (r - part) / static_cast<TO>(d.diag[row]);
+                out[o] = v;
+                lds[static_cast<size_t>(pos - p0) * kp + c] = v;
+            }
+        }
+        if (l + 1 < l1) lds_barrier();
+    }
+}
+
 // ---- per-column reductions -----------------------------------------------
 
 // Sum N values per thread over the threads of one column in a block (kp

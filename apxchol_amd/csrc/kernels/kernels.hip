@@ -254,12 +254,38 @@ namespace {
 template <class TI, class TO, bool F16>
 void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void* out, const pcg_state* st,
                const batch_state* bs, hipStream_t s) {
-    for (int l = 0; l < b.nlevels; ++l) {
+    auto level_launch = [&](int l) {
         const int w0 = b.level_first[l], w1 = b.level_first[l + 1];
-        if (w1 <= w0) continue;
+        if (w1 <= w0) return;
         const int g = (w1 - w0 + kWavesPerBlock - 1) / kWavesPerBlock;
         sweep_block<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b.pos, b.mask, b.col0, w0, w1, b.kp,
                                                       static_cast<const TI*>(rhs), static_cast<TO*>(out), st, bs);
+    };
+    if (b.nseg == 0) {
+        for (int l = 0; l < b.nlevels; ++l) level_launch(l);
+    } else {
+        static const bool attr = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_block_tail<TI, TO, F16>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
+            return true;
+        }();
+        (void)attr;
+        for (int k = 0; k < b.nseg; ++k) {
+            const int l0 = b.seg_l0[k], l1 = b.seg_l1[k];
+            if (!b.seg_tail[k]) {
+                for (int l = l0; l < l1; ++l) level_launch(l);
+                continue;
+            }
+            const int p0 = b.level_pos[l0], p1 = b.level_pos[l1];
+            const size_t lds = static_cast<size_t>(p1 - p0) * b.kp * sizeof(TO);
+            if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: blocked tail run exceeds LDS");
+            for (int l = l0; l < l1; ++l)
+                if (b.level_first[l + 1] - b.level_first[l] > kTailWaves)
+                    throw std::runtime_error("apxchol_amd: blocked tail level wider than the workgroup");
+            sweep_block_tail<TI, TO, F16><<<1, kTailBlock, lds, s>>>(
+                d, b.pos, b.mask, b.col0, b.level_first_dev, l0, l1, p0, b.kp, static_cast<const TI*>(rhs),
+                static_cast<TO*>(out), st, bs);
+        }
     }
     check_launch("sweep_block");
 }
@@ -269,6 +295,10 @@ void check_kp(int kp) {
 }
 
 }  // namespace
+
+int sweep_block_tail_max_rows(vec_prec p, int kp) {
+    return kTailLdsBytes / (kp * static_cast<int>(p == vec_prec::fp64 ? sizeof(double) : sizeof(float)));
+}
 
 void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sched& b, const void* rhs, void* out,
                     const pcg_state* st, const batch_state* bs, hipStream_t s) {

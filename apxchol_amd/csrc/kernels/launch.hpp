@@ -95,12 +95,22 @@ struct block_sched {
     const unsigned* pos = nullptr;
     const unsigned long long* mask = nullptr;
     const int* col0 = nullptr;
-    const int* level_first = nullptr;
+    const int* level_first = nullptr;       // host
+    const int* level_first_dev = nullptr;   // device copy (tail runs)
+    const int* level_pos = nullptr;         // host: first sweep position of each level (nlevels + 1)
     int nlevels = 0;
+    // Segments: levels [seg_l0, seg_l1), tail flag (one workgroup walks the
+    // run with its outputs in LDS) or one launch per level.
+    const int* seg_l0 = nullptr;
+    const int* seg_l1 = nullptr;
+    const int* seg_tail = nullptr;
+    int nseg = 0;
 };
 // One blocked sweep as one launch per level.
 void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sched& bsc, const void* rhs, void* out,
                     const pcg_state* st, const batch_state* bs, hipStream_t s);
+// Rows a blocked tail run of block width kp may hold (LDS).
+int sweep_block_tail_max_rows(vec_prec p, int kp);
 void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx, const void* val, bool val32,
                    const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
                    bool want_dot, hipStream_t s);

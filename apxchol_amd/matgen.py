@@ -144,4 +144,27 @@ def workload(name: str):
         s = int(name.split("_")[1])
         L = rmat_laplacian(s)
         return L, project_range(L, rhs(L.shape[0], 0))
+    if name.startswith("grid2d_"):
+        k = int(name.split("_")[1])
+        L = grid_laplacian((k, k))
+        b = rhs(L.shape[0], 0)
+        return L, b - b.mean()
+    if name.startswith("grid3d_"):
+        k = int(name.split("_")[1])
+        L = grid_laplacian((k, k, k))
+        b = rhs(L.shape[0], 0)
+        return L, b - b.mean()
     raise KeyError(name)
+
+
+# The serving mix: 8 resident factors -- 2-D grids of 1e4 to 1e6 vertices,
+# 3-D grids up to 96^3, R-MAT graphs of scale 16 to 18.
+SERVING_FACTORS = ("grid2d_100", "grid2d_316", "grid2d_1000", "grid3d_48", "grid3d_96", "rmat_16", "rmat_17",
+                   "rmat_18")
+
+
+def request_rhs(L: sp.csc_matrix, sddm: bool, seed: int) -> np.ndarray:
+    """One N(0,1) right-hand side (seed), projected onto the range of a
+    Laplacian per connected component."""
+    b = rhs(L.shape[0], seed)
+    return b if sddm else project_range(L, b)

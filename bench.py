@@ -49,6 +49,59 @@ def percentile(xs, q):
     return s[k]
 
 
+def run_serving(args, ax, mg, np) -> dict:
+    """The serving mix: 8 resident factors, Poisson arrivals at a fixed rate,
+    one N(0,1) right-hand side per request with tol drawn from {1e-6, 1e-8,
+    1e-10}; mixed precision, continuous batching. Every result is graded by
+    its true residual."""
+    import threading
+    t0 = time.perf_counter()
+    mats, solvers = [], []
+    srv = ax.Server(slots=args.serving_slots, iters_per_check=2)
+    fids = []
+    for name in mg.SERVING_FACTORS:
+        L, _ = mg.workload(name)
+        F = ax.factorize(L, seed=0)
+        S = ax.Solver(L, factor=F, precision="fp32")
+        mats.append((name, L, F.sddm))
+        fids.append(srv.add(S))
+        solvers.append(S)
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(6)
+    tols = [1e-6, 1e-8, 1e-10]
+    plan = []
+    for i in range(args.serving_requests):
+        f = int(rng.integers(len(mats)))
+        plan.append((f, float(tols[int(rng.integers(3))]), float(rng.exponential(1.0 / args.serving_qps))))
+    rhs = [mg.request_rhs(mats[f][1], mats[f][2], 1000 + i) for i, (f, _, _) in enumerate(plan)]
+    ids = [None] * len(plan)
+
+    def produce():
+        for i, (f, tol, gap) in enumerate(plan):
+            time.sleep(gap)
+            ids[i] = srv.submit(fids[f], rhs[i], tol)
+
+    th = threading.Thread(target=produce)
+    t1 = time.perf_counter()
+    th.start()
+    th.join()
+    worst = 0.0
+    for i, (f, tol, _) in enumerate(plan):
+        x, r = srv.wait(ids[i], timeout=300)
+        L = mats[f][1]
+        tr = float(np.linalg.norm(rhs[i] - L @ x) / np.linalg.norm(rhs[i]))
+        worst = max(worst, tr / tol)
+    wall = time.perf_counter() - t1
+    st = srv.stats()
+    srv.close()
+    print(f"[bench] serving: {st['solves_per_s']:.1f} solves/s p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms",
+          file=sys.stderr, flush=True)
+    return {"factors": [m[0] for m in mats], "requests": len(plan), "qps_offered": args.serving_qps,
+            "slots": args.serving_slots, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "wall_s": wall, "setup_s": t_setup,
+            "worst_true_residual_over_tol": worst, "memory_used_bytes": st["memory_used"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +112,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--timing-solves", type=int, default=2)
+    ap.add_argument("--batch-k", type=int, default=32, help="right-hand sides of the blocked measurement (0: skip)")
+    ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
+    ap.add_argument("--serving-qps", type=float, default=200.0)
+    ap.add_argument("--serving-slots", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibration of the HBM byte counters)")
     args = ap.parse_args()
@@ -166,6 +223,33 @@ def main() -> int:
         ms, cnt = tm[k]
         sweep_gbps[k] = (lb[di] + tb[di]) / (ms / 1e3 / max(cnt, 1)) / 1e9 if cnt else 0.0
 
+    # Blocked k-RHS solves of the same system (mixed), against the plain
+    # single-vector fp64 path.
+    batched = None
+    if args.batch_k > 0:
+        s = ax.Solver(A, factor=F, precision="fp32", iters_per_check=args.iters_per_check)
+        rng = np.random.default_rng(100)
+        B = torch.tensor(rng.standard_normal((args.batch_k, n)), device=dev)
+        s.solve_batch(B)
+        torch.cuda.synchronize()
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            Xb, rb = s.solve_batch(B)
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t0) / reps
+        Xh = Xb.cpu().numpy()
+        Bh = B.cpu().numpy()
+        worst = max(float(np.linalg.norm(Bh[c] - A @ Xh[c]) / np.linalg.norm(Bh[c])) for c in range(args.batch_k))
+        s.close()
+        plain_rate = (args.steps / plain["total"]) if plain else None
+        batched = {"k": args.batch_k, "solves_per_s": args.batch_k / tb, "ms_per_block": tb * 1e3,
+                   "iterations_max": max(r.iterations for r in rb), "worst_true_residual": worst,
+                   "over_fp64_single": (args.batch_k / tb) / plain_rate if plain_rate else None}
+        print(f"[bench] blocked k={args.batch_k}: {args.batch_k / tb:.1f} solves/s", file=sys.stderr, flush=True)
+
+    serving = run_serving(args, ax, mg, np) if args.serving_requests > 0 else None
+
     cpu = None
     x_cpu_err = None
     if not args.no_cpu_baseline:
@@ -214,6 +298,8 @@ def main() -> int:
                 "p99_ms": 1e3 * percentile(plain["lat"], 99), "iterations": plain["iters"][0],
                 "true_residual": plain["true_res"], "setup_s": plain["setup"]},
             "mixed_over_fp64": None if not plain else value / (args.steps / plain["total"]),
+            "batched_mixed": batched,
+            "serving": serving,
             "kernels_mixed": per_kernel,
             "sptrsv_sweep_gbps_mixed": sweep_gbps,
             "host_factorization_s": t_fact,

@@ -154,6 +154,19 @@ apx_status apx_solver_solve_batch_host(apx_solver_t s, int32_t k, const double* 
  * bits. */
 apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, double* d_Z, void* stream);
 
+/* Autotuner (no counterpart in the reference): measures, for this solver on
+ * this device, the SpMV lanes per row (1..64; the solver then uses the
+ * fastest, for the single SpMV and the blocked SpMM alike) and the block
+ * width of blocked solves up to max_block_width (the most columns swept per
+ * second; a server created with slots = 0 uses it). Times in microseconds:
+ * spmv_us[i] for 2^i lanes, sweep_pair_us[i] for block width 8 << i. */
+typedef struct {
+    int32_t spmv_lanes, block_width;
+    double spmv_us[7];
+    double sweep_pair_us[4];
+} apx_autotune_result;
+apx_status apx_solver_autotune(apx_solver_t s, int32_t max_block_width, apx_autotune_result* out, void* stream);
+
 /* ---- serving (no counterpart in the reference) ---------------------------
  * A request loop over resident factors: each factor keeps one blocked PCG of
  * `slots` columns running on its own stream; requests are admitted into free

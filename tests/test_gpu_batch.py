@@ -142,3 +142,26 @@ def test_solve_batch_graph_matches_eager(gpu, name):
         assert np.array_equal(bits(Xe.cpu().numpy()), bits(Xg.cpu().numpy()))
         assert [r.iterations for r in re] == [r.iterations for r in rg]
     g.close()
+
+
+def test_autotune_keeps_results_consistent(gpu):
+    """After autotuning (which may change the SpMV lanes), the single and the
+    blocked paths still agree: blocked apply bits = single apply bits, and a
+    blocked solve matches single solves within one iteration."""
+    import torch
+    A, F = case("sddm3d_16")
+    s = ax.Solver(A, factor=F, precision="fp32")
+    r = s.autotune(max_block_width=32)
+    assert r["spmv_lanes"] in (1, 2, 4, 8, 16, 32, 64) and r["block_width"] in (8, 16, 32)
+    assert s.info["spmv_lanes"] == r["spmv_lanes"]
+    B = torch.tensor(rhs_block(A, True, 4, 9), device=gpu)
+    s.reset_apply_count()
+    Z = s.apply_batch(B).cpu().numpy()
+    for c in range(4):
+        s.reset_apply_count()
+        assert np.array_equal(bits(s.apply(B[c].clone()).cpu().numpy()), bits(Z[c]))
+    X, res = s.solve_batch(B)
+    for c in range(4):
+        one = s.solve(B[c].clone())
+        assert abs(one.iterations - res[c].iterations) <= 1
+    s.close()
