@@ -218,6 +218,8 @@ __global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, con
 // Lane mapping: thread t of the grid serves column ((t / G) % kp) and only
 // its chain-0 lanes (t % G == 0) carry values (the others hold exact zeros,
 // which the block partial skips); G = 1 is one thread per (row, column).
+// The totals come back in threads 0 .. kp-1 of the last workgroup: thread j
+// holds column j, whatever column it served.
 template <int N>
 __device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* part, unsigned* counter, double* sh,
                                              int G = 1) {
@@ -326,9 +328,14 @@ __global__ __launch_bounds__(kBlock) void spmm_pAp_k(int n, int kp, int G, const
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    if (col_grid_sum<1>(v, kp, part, counter, sh, G) && !st[c].done && st[c].phase == 0) {
-        st[c].pAp = v[0];
-        if (v[0] <= 0.0) { st[c].done = 1; st[c].status = 2; }
+    // col_grid_sum returns column j's total in thread j (< kp) of the last
+    // workgroup -- not this thread's SpMM column.
+    if (col_grid_sum<1>(v, kp, part, counter, sh, G)) {
+        pcg_state& t = st[threadIdx.x];
+        if (!t.done && t.phase == 0) {
+            t.pAp = v[0];
+            if (v[0] <= 0.0) { t.done = 1; t.status = 2; }
+        }
     }
 }
 

@@ -76,13 +76,16 @@ __device__ __forceinline__ bool grid_sum(double (&v)[N], double* part, unsigned*
     return threadIdx.x == 0;
 }
 
-// Grid shape of the vector kernels: a pure function of n (so reductions are
-// reproducible), at most kMaxBlocks workgroups.
+// Grid shape of the vector kernels: a pure function of n and of the
+// process's cap (so reductions are reproducible), at most kMaxBlocks
+// workgroups (APXCHOL_AMD_VEC_BLOCKS lowers the cap).
 inline constexpr int kMaxBlocks = 2048;
+int vec_block_cap();
 inline int vec_blocks(long long n) {
     long long b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
-    return static_cast<int>(b > kMaxBlocks ? kMaxBlocks : b);
+    const int cap = vec_block_cap();
+    return static_cast<int>(b > cap ? cap : b);
 }
 
 template <class T> struct flag_bits;

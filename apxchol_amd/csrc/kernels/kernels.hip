@@ -4,6 +4,7 @@
 
 #include <hip/hip_ext.h>
 
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -12,6 +13,15 @@
 #include "block.hip"
 
 namespace apxchol_amd {
+
+int vec_block_cap() {
+    static const int cap = [] {
+        const char* e = std::getenv("APXCHOL_AMD_VEC_BLOCKS");
+        const int v = (e && *e) ? std::atoi(e) : 0;
+        return (v >= 1 && v <= kMaxBlocks) ? v : kMaxBlocks;
+    }();
+    return cap;
+}
 
 namespace {
 
