@@ -42,7 +42,8 @@ class Options(ctypes.Structure):
     _fields_ = [("tol", F64), ("max_iter", I32), ("stagnation_window", I32), ("seed", U32),
                 ("precision", I32), ("fp16_storage", I32), ("schedule", I32), ("drop_rel", F64),
                 ("keep_factor", I32), ("iters_per_check", I32), ("sweep_chunk", I32), ("sweep_blocks", I32),
-                ("tail_rows", I32), ("graph", I32), ("relabel", I32)]
+                ("tail_rows", I32), ("graph", I32), ("relabel", I32),
+                ("lane_batches", I32), ("depth_levels", I32)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -56,7 +57,8 @@ class SolverInfo(ctypes.Structure):
                 ("rounds", I64), ("levels", I64), ("fwd_batches", I64), ("bck_batches", I64), ("op_nnz", I64),
                 ("op_fp32", I32), ("spmv_lanes", I32), ("dataflow_grid", I32), ("fp16_storage", I32),
                 ("device_bytes", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64 * 2),
-                ("tail_bytes", ctypes.c_uint64 * 2), ("level_launches", I64 * 2), ("tail_runs", I64 * 2)]
+                ("tail_bytes", ctypes.c_uint64 * 2), ("level_launches", I64 * 2), ("tail_runs", I64 * 2),
+                ("lane_levels", I64 * 2), ("spmv_format", I32), ("spmv_sell_slots", ctypes.c_uint64)]
 
 
 _SIGS = {
@@ -90,6 +92,7 @@ _SIGS = {
     "apx_solver_apply_host": ([P, P, P], ctypes.c_int),
     "apx_solver_reset_apply_count": ([P], ctypes.c_int),
     "apx_solver_sweep": ([P, I32, P, P, P], ctypes.c_int),
+    "apx_solver_spmv": ([P, P, P, P], ctypes.c_int),
     "apx_solver_export_factor": ([P, P, P, P, P], ctypes.c_int),
     "apx_solver_set_timing": ([P, I32], ctypes.c_int),
     "apx_solver_take_timing": ([P, P, P], ctypes.c_int),
@@ -155,7 +158,8 @@ def _csc(A):
 
 def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
              schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=4, sweep_chunk=0,
-             sweep_blocks=0, tail_rows=0, graph=True, relabel=True) -> Options:
+             sweep_blocks=0, tail_rows=0, graph=True, relabel=True, lane_batches=0,
+             depth_levels=True) -> Options:
     o = Options()
     load_library().apx_options_default(ctypes.byref(o))
     o.tol = float(tol)
@@ -173,6 +177,8 @@ def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp
     o.tail_rows = int(tail_rows)
     o.graph = 1 if graph else 0
     o.relabel = 1 if relabel else 0
+    o.lane_batches = int(lane_batches)
+    o.depth_levels = 1 if depth_levels else 0
     return o
 
 
@@ -341,7 +347,7 @@ class Solver:
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(load_library().apx_solver_autotune(self._h, int(max_block_width), ctypes.byref(r), s))
         out = {"spmv_lanes": r.spmv_lanes, "block_width": r.block_width, "spmv_us": list(r.spmv_us),
-               "sweep_pair_us": list(r.sweep_pair_us)}
+               "sweep_pair_us": list(r.sweep_pair_us), "spmv_sell": r.spmv_sell, "spmv_sell_us": r.spmv_sell_us}
         self._refresh_info()
         return out
 
@@ -387,6 +393,15 @@ class Solver:
     def reset_apply_count(self):
         _check(load_library().apx_solver_reset_apply_count(self._h))
 
+    def spmv(self, x, out=None, stream=None):
+        """y = A x with the solver's single-vector SpMV (torch float64 CUDA tensors of length n)."""
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().apx_solver_spmv(self._h, _ptr(x), _ptr(out), s))
+        return out
+
     def sweep(self, rhs, out, backward: bool, stream=None):
         """One sweep in factor order (torch float64 CUDA tensors of length m)."""
         import torch
@@ -396,7 +411,8 @@ class Solver:
 
 
 class AutotuneResult(ctypes.Structure):
-    _fields_ = [("spmv_lanes", I32), ("block_width", I32), ("spmv_us", F64 * 7), ("sweep_pair_us", F64 * 4)]
+    _fields_ = [("spmv_lanes", I32), ("block_width", I32), ("spmv_us", F64 * 7), ("sweep_pair_us", F64 * 4),
+                ("spmv_sell", I32), ("spmv_sell_us", F64)]
 
 
 class ServerStats(ctypes.Structure):

@@ -83,6 +83,8 @@ solve_options to_opts(const apx_options* o) {
     so.sweep_blocks = o->sweep_blocks;
     so.graph = o->graph != 0;
     so.relabel = o->relabel != 0;
+    so.lane_batches = o->lane_batches;
+    so.depth_levels = o->depth_levels != 0;
     return so;
 }
 
@@ -120,6 +122,8 @@ void apx_options_default(apx_options* o) {
     o->tail_rows = 0;
     o->graph = 1;
     o->relabel = 1;
+    o->lane_batches = 0;
+    o->depth_levels = 1;
 }
 
 const char* apx_last_error(void) { return g_err.c_str(); }
@@ -242,11 +246,14 @@ apx_status apx_solver_info_get(apx_solver_t s, apx_solver_info* info) {
         info->op_fp32 = st.op_fp32 ? 1 : 0; info->spmv_lanes = st.spmv_lanes; info->dataflow_grid = st.dataflow_grid;
         info->fp16_storage = s->s->trsv().fp16 ? 1 : 0;
         info->device_bytes = st.device_bytes;
+        info->spmv_format = st.spmv_format;
+        info->spmv_sell_slots = st.spmv_sell_slots;
         for (int k = 0; k < 2; ++k) {
             info->level_bytes[k] = st.level_bytes[k];
             info->tail_bytes[k] = st.tail_bytes[k];
             info->level_launches[k] = st.level_launches[k];
             info->tail_runs[k] = st.tail_runs[k];
+            info->lane_levels[k] = st.lane_levels[k];
         }
     });
 }
@@ -319,6 +326,8 @@ apx_status apx_solver_autotune(apx_solver_t s, int32_t max_block_width, apx_auto
             out->block_width = r.block_width;
             for (int k = 0; k < 7; ++k) out->spmv_us[k] = r.spmv_us[k];
             for (int k = 0; k < 4; ++k) out->sweep_pair_us[k] = r.sweep_pair_us[k];
+            out->spmv_sell = r.spmv_sell;
+            out->spmv_sell_us = r.spmv_sell_us;
         }
     });
 }
@@ -416,6 +425,15 @@ apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rh
         need(d_rhs, "rhs");
         need(d_out, "out");
         s->s->sweep_device(backward != 0, d_rhs, d_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_solver_spmv(apx_solver_t s, const double* d_x, double* d_y, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        need(d_x, "x");
+        need(d_y, "y");
+        s->s->spmv_device(d_x, d_y, static_cast<hipStream_t>(stream));
     });
 }
 

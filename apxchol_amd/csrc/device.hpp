@@ -46,6 +46,27 @@ struct sweep_desc {
     int* ctrl = nullptr;                    // 2 ints: ticket, finished waves (zero between sweeps)
 };
 
+// Sliced-ELL form of the PCG operator (the single-vector SpMV). Slice s holds
+// 64 rows (lane = row rowmap[s*64 + lane], or s*64 + lane without a map; -1 or
+// >= n: none) and soff[s+1] - soff[s] entry columns:
+//   idx/val[(soff[s] + k)*64 + lane]   entry k of the lane's row, CSR order
+//                                      (padding: the row itself / 0, value 0)
+// Rows too long for a slice are listed in longrows and read from the CSR
+// arrays (ptr, cidx, cval) by a whole wave.
+struct sell_desc {
+    int n = 0;
+    int nslices = 0;
+    int nlong = 0;
+    const std::uint32_t* soff = nullptr;
+    const std::int32_t* idx = nullptr;
+    const void* val = nullptr;              // fp64, or fp32 when every value is fp32-exact
+    const std::int32_t* rowmap = nullptr;   // null: identity
+    const std::int32_t* longrows = nullptr;
+    const std::uint32_t* ptr = nullptr;
+    const std::int32_t* cidx = nullptr;
+    const void* cval = nullptr;
+};
+
 // Device-resident PCG state of one solve (all reductions land here; the host
 // reads it only to test completion).
 struct pcg_state {
@@ -105,6 +126,20 @@ struct tail_desc {
     const std::int32_t* ovf = nullptr;
     const entry32* ovf_ent = nullptr;
 };
+// Static lane layout of a launched level (sweep_lanes): everything a lane of
+// a batch needs sits at an address computed from (lane batch, lane), so one
+// round of coalesced loads brings a wave's whole batch and the gather follows
+// directly (two dependent loads per level instead of the batch table -> row
+// pointers -> entries -> values chain).
+//   rec[b*64 + lane]              the lane's row and chain shape
+//   ent[(b*kEpl + k)*64 + lane]   entry sub + k*G of the row (k < nvalid;
+//                                 padding (0, 0), never gathered)
+// fp16 storage keeps its widened (exact fp32) values here.
+struct alignas(8) lane_rec {
+    std::int32_t row;    // the lane's row, or -1 for an idle lane
+    std::uint32_t info;  // log2 G | sub << 3 | nvalid << 9
+};
+
 // Per (tail batch, lane), written by the coupling kernel before a run: the
 // lane's fma chain over the entries before the run, and the row's right-hand
 // side (writer lanes).

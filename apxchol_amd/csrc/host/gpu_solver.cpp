@@ -72,6 +72,12 @@ int env_int(const char* name, int dflt) {
     return v > 0 ? v : dflt;
 }
 
+// An integer knob that may be 0 (unset or empty: dflt).
+int env_flag(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return (e && *e) ? std::atoi(e) : dflt;
+}
+
 int center_k_from_env() {
     int k = 10;
     if (const char* e = std::getenv("APXCHOL_CENTER_K"); e && *e) {
@@ -408,6 +414,46 @@ int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, 
     return base;
 }
 
+long long append_lane_level(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl,
+                            lane_tables& lt) {
+    const auto& idx = backward ? T.b_idx : T.f_idx;
+    const auto& val = backward ? T.b_val : T.f_val;
+    const auto& hv = backward ? T.b_h : T.f_h;
+    auto fail = [](const std::string& w) { throw std::runtime_error("apxchol_amd: invalid lane layout: " + w); };
+    if (bf < 0 || bl <= bf || static_cast<std::size_t>(bl) > bt.mask.size()) fail("batch range");
+    const long long base = lt.batches;
+    for (int b = bf; b < bl; ++b) {
+        entry32 e[kEpl][kWave];
+        for (int lane = 0; lane < kWave; ++lane) {
+            const host_lane x = map_lane_host(T, bt, backward, b, lane);
+            if (x.len > static_cast<std::uint32_t>(kWave * kEpl)) fail("long row");
+            int nv = 0;
+            for (int k = 0; k < kEpl; ++k) {
+                e[k][lane] = {0, 0.0f};
+                const std::uint32_t q = static_cast<std::uint32_t>(x.sub + k * x.G);
+                if (!x.active || q >= x.len) continue;
+                const std::uint32_t p = x.rb + q;
+                const std::int32_t j = idx[p];
+                const std::uint32_t pj = backward ? T.m - 1 - static_cast<std::uint32_t>(j) : static_cast<std::uint32_t>(j);
+                if (j < 0 || static_cast<std::uint32_t>(j) >= T.m || pj >= bt.pos[bf]) fail("dependency not before the level");
+                if (nv != k) fail("valid slots not a prefix");
+                e[k][lane] = {j, T.fp16 ? fp16_widen(hv[p]) : val[p]};
+                ++nv;
+            }
+            int lg = 0;
+            while ((1 << lg) < x.G) ++lg;
+            if ((1 << lg) != x.G || x.sub < 0 || x.sub >= kWave) fail("lane group");
+            const std::uint32_t info = static_cast<std::uint32_t>(lg) | (static_cast<std::uint32_t>(x.active ? x.sub : 0) << 3) |
+                                       (static_cast<std::uint32_t>(nv) << 9);
+            lt.rec.push_back({x.active ? static_cast<std::int32_t>(x.row) : -1, x.active ? info : 0u});
+        }
+        for (int k = 0; k < kEpl; ++k)
+            for (int lane = 0; lane < kWave; ++lane) lt.ent.push_back(e[k][lane]);
+    }
+    lt.batches += bl - bf;
+    return base;
+}
+
 struct gpu_solver::device_state {
     vec_prec prec = vec_prec::fp64;
     bool fp16 = false;
@@ -417,10 +463,16 @@ struct gpu_solver::device_state {
     std::vector<int> f_levels, b_levels;
     dbuf f_levels_dev, b_levels_dev, tail_val;
     dbuf f_tlane, f_tent, f_tovf, f_tovf_ent, b_tlane, b_tent, b_tovf, b_tovf_ent;
+    dbuf f_lrec, f_lent, b_lrec, b_lent;
+    std::vector<int> f_level_lb, b_level_lb;   // first lane batch of each level, or -1
     std::vector<int> f_seg[6], b_seg[6];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch
     sweep_sched f_sched, b_sched;
     dbuf perm, op_ptr, op_idx, op_val;
     bool op32 = false;
+    dbuf sell_soff, sell_idx, sell_val, sell_map, sell_long;
+    sell_desc sell;
+    bool has_sell = false;   // the sliced-ELL form exists
+    bool use_sell = false;   // the single SpMV runs on it (else CSR lanes)
     int lanes = 8;
     dbuf r, x, p, ap, rp, y, z, xout;
     dbuf relabel;   // factor position -> storage position (empty: identity)
@@ -575,8 +627,8 @@ void gpu_solver::setup(const csc_view& L) {
     // the sweeps and the SpMV -- sit close together.
     std::vector<node_t> sperm = F_.perm;
     relabel_.clear();
-    if (opts_.relabel && env_int("APXCHOL_AMD_RELABEL", 1) != 0) {
-        relabel_ = relabel_within_levels(T_, F_.perm);
+    if (opts_.relabel && env_flag("APXCHOL_AMD_RELABEL", 1) != 0) {
+        relabel_ = relabel_within_levels(T_, F_.perm, opts_.depth_levels && env_flag("APXCHOL_AMD_DEPTH_LEVELS", 1) != 0);
         for (auto& q : sperm)
             if (q < T_.m) q = relabel_[q];
     }
@@ -648,6 +700,52 @@ void gpu_solver::setup(const csc_view& L) {
     int lanes = 1;
     while (lanes < 64 && lanes * 1.5 < avg) lanes <<= 1;
     d.lanes = lanes;
+    // Sliced-ELL form of the operator for the single SpMV: without a row map
+    // when 64-row slices in row order pad to <= 1.2 nnz, else sorted by length
+    // inside 4096-row windows (rows longer than max(64, 8 mean) walked by a
+    // wave) when that pads to <= 1.5 nnz; otherwise CSR lanes only.
+    // APXCHOL_AMD_SPMV=csr / sell forces the choice.
+    {
+        const std::int64_t nnz = op.ptr.back();
+        const std::int64_t long_len = std::max<std::int64_t>(64, 8 * static_cast<std::int64_t>(std::ceil(avg)));
+        const char* fe = std::getenv("APXCHOL_AMD_SPMV");
+        const std::string force = fe ? fe : "";
+        if (n > 0 && force != "csr") {
+            const std::uint64_t id_slots = sell_identity_slots(op, long_len);
+            sell_layout sl;
+            bool ok = false;
+            if (id_slots != UINT64_MAX && static_cast<double>(id_slots) <= 1.2 * static_cast<double>(nnz)) {
+                sl = build_sell(op, long_len, 1);
+                ok = true;
+            } else {
+                sl = build_sell(op, long_len, 4096);
+                std::int64_t long_nnz = 0;
+                for (const auto r : sl.longrows) long_nnz += op.ptr[r + 1] - op.ptr[r];
+                ok = force == "sell" ||
+                     static_cast<double>(sl.slots) <= 1.5 * static_cast<double>(std::max<std::int64_t>(nnz - long_nnz, 1));
+            }
+            if (ok) {
+                d.sell_soff.upload(sl.soff);
+                d.sell_idx.upload(sl.idx);
+                if (d.op32) d.sell_val.upload(sl.val32); else d.sell_val.upload(sl.val);
+                d.sell_map.upload(sl.rowmap);
+                d.sell_long.upload(sl.longrows);
+                d.sell.n = n;
+                d.sell.nslices = static_cast<int>(sl.soff.size()) - 1;
+                d.sell.nlong = static_cast<int>(sl.longrows.size());
+                d.sell.soff = d.sell_soff.as<std::uint32_t>();
+                d.sell.idx = d.sell_idx.as<std::int32_t>();
+                d.sell.val = d.sell_val.get();
+                d.sell.rowmap = sl.rowmap.empty() ? nullptr : d.sell_map.as<std::int32_t>();
+                d.sell.longrows = d.sell_long.as<std::int32_t>();
+                d.sell.ptr = d.op_ptr.as<std::uint32_t>();
+                d.sell.cidx = d.op_idx.as<std::int32_t>();
+                d.sell.cval = d.op_val.get();
+                d.has_sell = d.use_sell = true;
+                stats_.spmv_sell_slots = sl.slots;
+            }
+        }
+    }
 
     const std::size_t vb = static_cast<std::size_t>(std::max(n, 1)) * sizeof(double);
     d.r.alloc(vb); d.x.alloc(vb); d.p.alloc(vb); d.ap.alloc(vb); d.rp.alloc(vb); d.y.alloc(vb); d.z.alloc(vb);
@@ -698,8 +796,14 @@ void gpu_solver::setup(const csc_view& L) {
                                    sweep_tail_max_rows(d.prec));
     // A tail run's outputs live in LDS: a run longer than that holds is split.
     const int tail_cap = sweep_tail_max_rows(d.prec);
+    // Launched levels of at most lane_max batches (and no row longer than a
+    // wave's chunk) read the static lane layout.
+    const int lane_max = opts_.lane_batches < 0 ? 0
+                         : opts_.lane_batches > 0 ? opts_.lane_batches
+                                                  : env_int("APXCHOL_AMD_LANE_BATCHES", 4096);
     auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
-                          dbuf& lf_dev, std::vector<int> (&seg)[6], dbuf* const (&tbuf)[4]) {
+                          dbuf& lf_dev, std::vector<int> (&seg)[6], dbuf* const (&tbuf)[4], std::vector<int>& level_lb,
+                          dbuf& lrec, dbuf& lent) {
         lf_dev.upload(lf);
         tail_tables tt;
         const int L = static_cast<int>(lf.size()) - 1;
@@ -739,6 +843,40 @@ void gpu_solver::setup(const csc_view& L) {
         sc.seg_base = seg[5].data();
         sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
                                    : opts_.schedule == sweep_schedule::levels ? 1 : 2);
+        // Static lane layout of the launched levels.
+        level_lb.assign(static_cast<std::size_t>(L), -1);
+        lane_tables lt;
+        if (sc.kind != 0 && lane_max > 0) {
+            const auto& ptr = backward ? T_.b_ptr : T_.f_ptr;
+            std::vector<char> in_tail(static_cast<std::size_t>(L), 0);
+            if (sc.kind == 2)
+                for (std::size_t k = 0; k < seg[0].size(); ++k)
+                    if (seg[2][k])
+                        for (int q = seg[0][k]; q < seg[1][k]; ++q) in_tail[q] = 1;
+            for (int q = 0; q < L; ++q) {
+                const int nbq = lf[q + 1] - lf[q];
+                if (in_tail[q] || nbq <= 0 || nbq > lane_max) continue;
+                bool longrow = false;
+                for (int t = pos(q); t < pos(q + 1) && !longrow; ++t) {
+                    const int row = backward ? m - 1 - t : t;
+                    longrow = ptr[row + 1] - ptr[row] > static_cast<std::uint32_t>(kWave * kEpl);
+                }
+                if (longrow) continue;
+                const long long lb = append_lane_level(T_, bt, backward, lf[q], lf[q + 1], lt);
+                if (lb + nbq > INT32_MAX) throw std::runtime_error("apxchol_amd: lane layout too large");
+                level_lb[q] = static_cast<int>(lb);
+                ++stats_.lane_levels[backward ? 1 : 0];
+            }
+        }
+        if (lt.rec.size() != static_cast<std::size_t>(lt.batches) * kWave ||
+            lt.ent.size() != static_cast<std::size_t>(lt.batches) * kWave * kEpl)
+            throw std::runtime_error("apxchol_amd: inconsistent lane layout");
+        lrec.upload(lt.rec);
+        lent.upload(lt.ent);
+        sc.level_lb = level_lb.data();
+        sc.lane_rec_dev = lrec.as<lane_rec>();
+        sc.lane_ent_dev = lent.as<entry32>();
+        sc.lane_batches = lt.batches;
         // Algorithmic bytes per level (see solver_stats).
         if (sc.kind != 0) {
             const auto& ptr = backward ? T_.b_ptr : T_.f_ptr;
@@ -778,8 +916,8 @@ void gpu_solver::setup(const csc_view& L) {
     };
     dbuf* f_tb[4] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent};
     dbuf* b_tb[4] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent};
-    make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb);
-    make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb);
+    make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb, d.f_level_lb, d.f_lrec, d.f_lent);
+    make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb, d.b_level_lb, d.b_lrec, d.b_lent);
     // Coupling output, shared by the two directions (their sweeps never overlap).
     const std::size_t tail_batches = std::max<std::size_t>(
         1, static_cast<std::size_t>(std::max(d.f_sched.tail_batches, d.b_sched.tail_batches)));
@@ -803,11 +941,15 @@ void gpu_solver::setup(const csc_view& L) {
     stats_.op_nnz = op.ptr.back();
     stats_.op_fp32 = d.op32;
     stats_.spmv_lanes = lanes;
+    stats_.spmv_format = d.use_sell ? 1 : 0;
     stats_.dataflow_grid = d.grid;
     std::uint64_t bytes = 0;
     for (const dbuf* b : {&d.f_ptr, &d.f_ent, &d.f_idx, &d.f_h, &d.b_ptr, &d.b_ent, &d.b_idx, &d.b_h, &d.diag,
                           &d.inv_scale, &d.f_pos, &d.f_mask, &d.b_pos, &d.b_mask, &d.perm, &d.op_ptr, &d.op_idx,
-                          &d.op_val, &d.r, &d.x, &d.p, &d.ap, &d.rp, &d.y, &d.z, &d.xout})
+                          &d.op_val, &d.r, &d.x, &d.p, &d.ap, &d.rp, &d.y, &d.z, &d.xout, &d.f_lrec, &d.f_lent,
+                          &d.b_lrec, &d.b_lent, &d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.b_tlane,
+                          &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.tail_val, &d.relabel, &d.sell_soff,
+                          &d.sell_idx, &d.sell_val, &d.sell_map, &d.sell_long})
         bytes += b->bytes();
     stats_.device_bytes = bytes;
 
@@ -852,6 +994,16 @@ void gpu_solver::enqueue_precondition(bool first, int step, hipStream_t s) const
     });
 }
 
+void gpu_solver::spmv(const double* x, double* y, pcg_state* st, bool want_dot, hipStream_t s,
+                      const launch_hook& hook) const {
+    auto& d = *dev_;
+    if (d.use_sell)
+        launch_spmv_sell(d.sell, d.op32, x, y, st, d.part.as<double>(), d.counter.as<unsigned>(), want_dot, s, hook);
+    else
+        launch_spmv(static_cast<int>(n_), d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
+                    x, y, st, d.part.as<double>(), d.counter.as<unsigned>(), want_dot, s, hook);
+}
+
 void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
     auto& d = *dev_;
     timer& tm = *timer_;
@@ -860,8 +1012,7 @@ void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
     pcg_state* st = d.st.as<pcg_state>();
     double* part = d.part.as<double>();
     unsigned* cnt = d.counter.as<unsigned>();
-    launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes, d.p.as<double>(),
-                d.ap.as<double>(), st, part, cnt, true, s, tm.hook(it - 1));
+    spmv(d.p.as<double>(), d.ap.as<double>(), st, true, s, tm.hook(it - 1));
     tm.time(3, it - 1, s, [&] {
         launch_update_xr(d.prec, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
                          d.y.get(), d.z.get(), st, part, cnt, s);
@@ -898,8 +1049,7 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     launch_scatter_init(n, d.perm.as<unsigned>(), d_b, d_x0, d.r.as<double>(), d.x.as<double>(), s);
     if (d_x0) {
         launch_norm2_b(n, d_b, st, part, cnt, s);
-        launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes, d.x.as<double>(),
-                    d.ap.as<double>(), st, part, cnt, false, s);
+        spmv(d.x.as<double>(), d.ap.as<double>(), st, false, s, {});
         launch_init_residual(n, d.r.as<double>(), d.ap.as<double>(), st, part, cnt, s);
     } else {
         launch_init_residual(n, d.r.as<double>(), nullptr, st, part, cnt, s);
@@ -1201,6 +1351,15 @@ gpu_solver::autotune_result gpu_solver::autotune(int max_block_width, hipStream_
         }
         d.lanes = res.spmv_lanes;
         stats_.spmv_lanes = res.spmv_lanes;
+        // The sliced-ELL form, when it exists, against the best lane count.
+        if (d.has_sell) {
+            res.spmv_sell_us = timed([&] {
+                launch_spmv_sell(d.sell, d.op32, d.p.as<double>(), d.ap.as<double>(), nullptr, nullptr, nullptr, false, s);
+            }, 5);
+            d.use_sell = res.spmv_sell_us < best;
+        }
+        res.spmv_sell = d.use_sell ? 1 : 0;
+        stats_.spmv_format = d.use_sell ? 1 : 0;
         // Block width: one forward + back blocked sweep per candidate.
         double best_rate = 0.0;
         for (int k = 0, kp = 8; kp <= 64 && kp <= std::max(8, max_block_width); ++k, kp <<= 1) {
@@ -1476,6 +1635,15 @@ void gpu_solver::sweep_device(bool backward, const double* d_rhs, double* d_out,
     launch_sweep(backward ? d.bck : d.fwd, vec_prec::fp64, d.fp16, d.rp.get(), d.xout.get(), nullptr, d.err.as<int>(),
                  backward ? d.b_sched : d.f_sched, s);
     launch_permute(m, Q, d.xout.as<double>(), d_out, false, s);
+}
+
+void gpu_solver::spmv_device(const double* d_x, double* d_y, hipStream_t s) const {
+    auto& d = *dev_;
+    const int n = static_cast<int>(n_);
+    if (n == 0) return;
+    launch_scatter_init(n, d.perm.as<unsigned>(), d_x, nullptr, d.rp.as<double>(), d.xout.as<double>(), s);
+    spmv(d.rp.as<double>(), d.ap.as<double>(), nullptr, false, s, {});
+    launch_gather_shift(n, d.perm.as<unsigned>(), d.ap.as<double>(), d_y, nullptr, false, s);
 }
 
 solve_result solve(const csc_view& L, const std::vector<double>& b, const solve_options& opts) {

@@ -34,6 +34,9 @@ struct solve_options {
     int sweep_blocks = 0;         // dataflow grid (0: APXCHOL_AMD_SWEEP_BLOCKS or the resident grid)
     bool graph = true;            // replay each PCG iteration as one captured HIP graph
     bool relabel = true;          // sort every level's rows by original vertex (APXCHOL_AMD_RELABEL=0: off)
+    bool depth_levels = true;     // with relabel: levels = topological depth (APXCHOL_AMD_DEPTH_LEVELS=0: rounds)
+    int lane_batches = 0;         // launched levels of <= this many batches use the lane layout
+                                  // (0: APXCHOL_AMD_LANE_BATCHES or 4096; < 0: never)
 };
 
 // solve.h:27-42. status: 0 running/none, 1 converged, 2 pAp <= 0 breakdown,
@@ -63,9 +66,13 @@ struct solver_stats {
     // inverse scale on the fp16 back sweep]) + 12 per batch (batch table).
     std::uint64_t level_bytes[2] = {0, 0}, tail_bytes[2] = {0, 0};
     std::int64_t level_launches[2] = {0, 0}, tail_runs[2] = {0, 0};
+    std::int64_t lane_levels[2] = {0, 0};   // launched levels on the static lane layout
+    int spmv_format = 0;                    // single SpMV: 0 CSR lanes, 1 sliced ELL
+    std::uint64_t spmv_sell_slots = 0;      // padded entries of the sliced-ELL form (0: none)
 };
 
 struct block_sched;
+struct launch_hook;
 
 class gpu_solver {
 public:
@@ -128,6 +135,8 @@ public:
         double spmv_us[7] = {};           // lanes 1, 2, ..., 64
         int block_width = 0;
         double sweep_pair_us[4] = {};     // block widths 8, 16, 32, 64: forward + back sweep
+        int spmv_sell = 0;                // 1: the single SpMV now runs on the sliced-ELL form
+        double spmv_sell_us = 0.0;        // its time (0: no sliced-ELL form)
     };
     autotune_result autotune(int max_block_width, hipStream_t s) const;
     int preferred_block_width() const { return tuned_block_width_; }
@@ -137,6 +146,8 @@ public:
     void apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const;
     // One sweep in factor order with fp64 vectors (rhs, out: m doubles).
     void sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const;
+    // y = A x in the caller's order (n doubles each) with the single SpMV.
+    void spmv_device(const double* d_x, double* d_y, hipStream_t s) const;
 
     void reset_apply_count() const { apply_count_ = 0; }
 
@@ -162,6 +173,8 @@ private:
     // Enqueue PCG iteration `it` (SpMV, x/r update, both sweeps, the
     // preconditioned dot and the p update); step labels the timing records.
     void enqueue_iteration(int step, hipStream_t s) const;
+    // y = A x with the solver's single-vector SpMV (sliced ELL or CSR lanes).
+    void spmv(const double* x, double* y, pcg_state* st, bool want_dot, hipStream_t s, const launch_hook& hook) const;
     void enqueue_precondition(bool first, int step, hipStream_t s) const;
     std::vector<solve_result> solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
                                           hipStream_t s) const;
@@ -225,5 +238,17 @@ struct tail_tables {
 // tail batch. Throws if a lane's chain does not have the shape the tail
 // kernel relies on (entries before the run form a prefix of the chain).
 int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl, tail_tables& tt);
+
+// Static lane layout of launched levels (device.hpp's lane_rec / entry32
+// planes), level by level.
+struct lane_tables {
+    std::vector<lane_rec> rec;
+    std::vector<entry32> ent;
+    long long batches = 0;
+};
+// Appends the batches [bf, bl) of one level and returns its first lane batch.
+// Throws on a row longer than 64 * kEpl or a dependency inside the level.
+long long append_lane_level(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl,
+                            lane_tables& lt);
 
 }  // namespace apxchol_amd

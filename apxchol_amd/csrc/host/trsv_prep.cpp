@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 
 namespace apxchol_amd {
 
@@ -211,17 +212,45 @@ trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel) {
     return T;
 }
 
-std::vector<std::uint32_t> relabel_within_levels(trsv_factor& T, const std::vector<node_t>& perm) {
+std::vector<std::uint32_t> relabel_within_levels(trsv_factor& T, const std::vector<node_t>& perm, bool by_depth) {
     const std::uint32_t m = T.m;
     std::vector<std::uint32_t> vert(perm.size());
     for (std::size_t v = 0; v < perm.size(); ++v) vert[perm[v]] = static_cast<std::uint32_t>(v);
     std::vector<std::uint32_t> Q(m), order;
-    for (std::size_t l = 0; l + 1 < T.level_ptr.size(); ++l) {
-        const std::uint32_t a = T.level_ptr[l], b = T.level_ptr[l + 1];
-        order.resize(b - a);
-        for (std::uint32_t q = a; q < b; ++q) order[q - a] = q;
-        std::sort(order.begin(), order.end(), [&](std::uint32_t x, std::uint32_t y) { return vert[x] < vert[y]; });
-        for (std::uint32_t k = 0; k < b - a; ++k) Q[order[k]] = a + k;
+    if (by_depth) {
+        // Depth of every row in factor order (its dependencies precede it).
+        std::vector<std::uint32_t> depth(m, 0);
+        std::uint32_t levels = m > 0 ? 1 : 0;
+        for (std::uint32_t r = 0; r < m; ++r) {
+            std::uint32_t dr = 0;
+            for (std::uint32_t p = T.f_ptr[r]; p < T.f_ptr[r + 1]; ++p) {
+                const std::uint32_t j = static_cast<std::uint32_t>(T.f_idx[p]);
+                if (j >= r) throw std::runtime_error("apxchol_amd: forward dependency not before its row");
+                dr = std::max(dr, depth[j] + 1);
+            }
+            depth[r] = dr;
+            levels = std::max(levels, dr + 1);
+        }
+        // Rows grouped by depth (stable: factor order inside a group).
+        std::vector<std::uint32_t> lp(static_cast<std::size_t>(levels) + 1, 0);
+        for (std::uint32_t r = 0; r < m; ++r) ++lp[depth[r] + 1];
+        for (std::uint32_t l = 0; l < levels; ++l) lp[l + 1] += lp[l];
+        std::vector<std::uint32_t> at(lp.begin(), lp.end() - 1), byd(m);
+        for (std::uint32_t r = 0; r < m; ++r) byd[at[depth[r]]++] = r;
+        for (std::uint32_t l = 0; l < levels; ++l) {
+            const std::uint32_t a = lp[l], b = lp[l + 1];
+            std::sort(byd.begin() + a, byd.begin() + b, [&](std::uint32_t x, std::uint32_t y) { return vert[x] < vert[y]; });
+            for (std::uint32_t k = a; k < b; ++k) Q[byd[k]] = k;
+        }
+        T.level_ptr.swap(lp);
+    } else {
+        for (std::size_t l = 0; l + 1 < T.level_ptr.size(); ++l) {
+            const std::uint32_t a = T.level_ptr[l], b = T.level_ptr[l + 1];
+            order.resize(b - a);
+            for (std::uint32_t q = a; q < b; ++q) order[q - a] = q;
+            std::sort(order.begin(), order.end(), [&](std::uint32_t x, std::uint32_t y) { return vert[x] < vert[y]; });
+            for (std::uint32_t k = 0; k < b - a; ++k) Q[order[k]] = a + k;
+        }
     }
     auto relist = [&](std::vector<std::uint32_t>& ptr, std::vector<std::int32_t>& idx, std::vector<float>& val,
                       std::vector<std::uint16_t>& h, bool descending) {
@@ -293,6 +322,81 @@ permuted_operator build_permuted_operator(const csc_view& A, const std::vector<n
         std::vector<double>().swap(op.val);
     }
     return op;
+}
+
+std::uint64_t sell_identity_slots(const permuted_operator& op, std::int64_t long_len) {
+    const std::int64_t n = op.n;
+    std::uint64_t slots = 0;
+    for (std::int64_t s0 = 0; s0 < n; s0 += 64) {
+        std::int64_t w = 0;
+        for (std::int64_t r = s0; r < std::min(n, s0 + 64); ++r) {
+            const std::int64_t len = op.ptr[r + 1] - op.ptr[r];
+            if (len > long_len) return UINT64_MAX;
+            w = std::max(w, len);
+        }
+        slots += static_cast<std::uint64_t>(w) * 64;
+    }
+    return slots;
+}
+
+sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::int64_t sigma) {
+    sell_layout s;
+    const std::int64_t n = op.n;
+    const bool map = sigma > 1;
+    std::vector<std::int32_t> order;   // short rows in slice order
+    order.reserve(static_cast<std::size_t>(n));
+    auto len = [&](std::int64_t r) { return op.ptr[r + 1] - op.ptr[r]; };
+    for (std::int64_t w0 = 0; w0 < n; w0 += std::max<std::int64_t>(sigma, 1)) {
+        const std::int64_t w1 = std::min(n, w0 + std::max<std::int64_t>(sigma, 1));
+        const std::size_t first = order.size();
+        for (std::int64_t r = w0; r < w1; ++r) {
+            if (len(r) > long_len) {
+                if (!map) throw std::invalid_argument("apxchol_amd: sliced ELL without a row map has a long row");
+                s.longrows.push_back(static_cast<std::int32_t>(r));
+            } else {
+                order.push_back(static_cast<std::int32_t>(r));
+            }
+        }
+        if (map)
+            std::stable_sort(order.begin() + static_cast<std::ptrdiff_t>(first), order.end(),
+                             [&](std::int32_t a, std::int32_t b) { return len(a) > len(b); });
+    }
+    const std::int64_t nrows = map ? static_cast<std::int64_t>(order.size()) : n;
+    const std::int64_t nslices = (nrows + 63) / 64;
+    s.soff.assign(static_cast<std::size_t>(nslices) + 1, 0);
+    for (std::int64_t q = 0; q < nslices; ++q) {
+        std::int64_t w = 0;
+        for (std::int64_t l = q * 64; l < std::min(nrows, q * 64 + 64); ++l) w = std::max(w, len(map ? order[l] : l));
+        const std::uint64_t next = static_cast<std::uint64_t>(s.soff[q]) + static_cast<std::uint64_t>(w);
+        if (next > UINT32_MAX / 64) throw std::runtime_error("apxchol_amd: sliced ELL too large");
+        s.soff[q + 1] = static_cast<std::uint32_t>(next);
+    }
+    s.slots = static_cast<std::uint64_t>(s.soff[nslices]) * 64;
+    const bool f32 = op.fp32_exact;
+    s.idx.assign(s.slots, 0);
+    if (f32) s.val32.assign(s.slots, 0.0f); else s.val.assign(s.slots, 0.0);
+    if (map) s.rowmap.assign(static_cast<std::size_t>(nslices) * 64, -1);
+    for (std::int64_t q = 0; q < nslices; ++q) {
+        const std::int64_t w = s.soff[q + 1] - s.soff[q];
+        for (std::int64_t l = 0; l < 64; ++l) {
+            const std::int64_t pos = q * 64 + l;
+            if (pos >= nrows) continue;
+            const std::int64_t r = map ? order[pos] : pos;
+            if (map) s.rowmap[pos] = static_cast<std::int32_t>(r);
+            const std::int64_t b = op.ptr[r], e = op.ptr[r + 1];
+            for (std::int64_t k = 0; k < w; ++k) {
+                const std::size_t at = (static_cast<std::size_t>(s.soff[q]) + static_cast<std::size_t>(k)) * 64 +
+                                       static_cast<std::size_t>(l);
+                if (b + k < e) {
+                    s.idx[at] = op.idx[b + k];
+                    if (f32) s.val32[at] = op.val32[b + k]; else s.val[at] = op.val[b + k];
+                } else {
+                    s.idx[at] = static_cast<std::int32_t>(r);   // padding: value 0
+                }
+            }
+        }
+    }
+    return s;
 }
 
 }  // namespace apxchol_amd

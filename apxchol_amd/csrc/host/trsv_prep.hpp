@@ -57,7 +57,13 @@ trsv_factor prepare_trsv(const factorization& F, bool fp16, double rel = -1.0);
 // row / column lists (entries re-sorted: forward rows ascending, back
 // columns descending), diag and inv_scale, and returns Q (m entries). The
 // vertex at factor position q is the v with perm[v] == q.
-std::vector<std::uint32_t> relabel_within_levels(trsv_factor& T, const std::vector<node_t>& perm);
+// With by_depth the levels are first replaced by the topological depth of
+// the forward sweep's dependency graph (level of a row = 1 + the deepest
+// level it reads; omp.h:865-1038 offers the same choice), which merges rounds
+// and peel columns that do not depend on each other; the back sweep walks the
+// same levels in reverse.
+std::vector<std::uint32_t> relabel_within_levels(trsv_factor& T, const std::vector<node_t>& perm,
+                                                 bool by_depth = false);
 
 // P A P^T as a full-symmetric CSR with rows' columns ascending
 // (solver/pcg_cuda.h:304-404, solve.cpp:320-476); values stored fp32 when every
@@ -71,5 +77,24 @@ struct permuted_operator {
     bool fp32_exact = false;
 };
 permuted_operator build_permuted_operator(const csc_view& A, const std::vector<node_t>& perm);
+
+// Sliced-ELL form of the operator (device.hpp's sell_desc): rows with more
+// than long_len entries are listed in longrows; the others are packed 64 to a
+// slice, in row order (sigma <= 1: slice s = rows [64s, 64s+64), no row map --
+// only valid without long rows) or sorted by decreasing length (stable) inside
+// windows of sigma rows, with a row map.
+struct sell_layout {
+    std::vector<std::uint32_t> soff;
+    std::vector<std::int32_t> idx;
+    std::vector<double> val;
+    std::vector<float> val32;
+    std::vector<std::int32_t> rowmap;     // empty: identity
+    std::vector<std::int32_t> longrows;
+    std::uint64_t slots = 0;              // padded entries of the slices
+};
+// Padded entries of the identity slicing (no map), or UINT64_MAX when a row is
+// longer than long_len.
+std::uint64_t sell_identity_slots(const permuted_operator& op, std::int64_t long_len);
+sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::int64_t sigma);
 
 }  // namespace apxchol_amd

@@ -64,6 +64,14 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         const int b0 = sc.level_first[l], b1 = sc.level_first[l + 1];
         if (b1 <= b0) return;
         int g = (b1 - b0 + kWavesPerBlock - 1) / kWavesPerBlock;
+        const int lb = sc.level_lb ? sc.level_lb[l] : -1;
+        if (lb >= 0) {
+            if (static_cast<long long>(lb) + (b1 - b0) > sc.lane_batches)
+                throw std::runtime_error("apxchol_amd: level outside the lane layout");
+            launch_timed(hook, 0, sweep_lanes<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, lb, b1 - b0,
+                         sc.lane_rec_dev, sc.lane_ent_dev, in, o, st);
+            return;
+        }
         if (g > kLevelMaxBlocks) g = kLevelMaxBlocks;
         launch_timed(hook, 0, sweep_levels<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, b0, b1, in, o, st, err);
     };
@@ -166,6 +174,21 @@ void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bo
                  const launch_hook& hook) {
     if (val32) spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, st, part, counter, want_dot, s, hook);
     else spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, st, part, counter, want_dot, s, hook);
+}
+
+void launch_spmv_sell(const sell_desc& a, bool val32, const double* x, double* y, pcg_state* st, double* part,
+                      unsigned* counter, bool want_dot, hipStream_t s, const launch_hook& hook) {
+    const long long items = static_cast<long long>(a.nslices) + a.nlong;
+    if (items <= 0) return;
+    long long g = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (g > vec_block_cap()) g = vec_block_cap();
+    const int w = want_dot ? 1 : 0;
+    auto go = [&](auto kernel) {
+        launch_timed(hook, 4, kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, a, x, y, st, part, counter, w);
+    };
+    if (val32) { if (a.rowmap) go(spmv_sell<float, true>); else go(spmv_sell<float, false>); }
+    else { if (a.rowmap) go(spmv_sell<double, true>); else go(spmv_sell<double, false>); }
+    check_launch("spmv_sell");
 }
 
 void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,

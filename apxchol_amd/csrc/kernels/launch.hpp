@@ -37,6 +37,10 @@ struct sweep_sched {
     int tail_batches = 0;                  // tail batches of this direction
     void* tail_val = nullptr;              // coupling output, tail_val<T> per (tail batch, lane) (device)
     size_t tail_val_bytes = 0;
+    const int* level_lb = nullptr;         // host: first lane batch of a launched level, or -1 (batch table)
+    const lane_rec* lane_rec_dev = nullptr;  // static lane layout of the launched levels (device)
+    const entry32* lane_ent_dev = nullptr;
+    long long lane_batches = 0;
 };
 
 // Kernel timing: for kernel `kind` (0 sweep_levels, 1 sweep_tail,
@@ -60,6 +64,10 @@ void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_s
 void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
                  double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
                  const launch_hook& hook = {});
+
+// y = A x (and x.y into st->pAp when want_dot) over the sliced-ELL form.
+void launch_spmv_sell(const sell_desc& a, bool val32, const double* x, double* y, pcg_state* st, double* part,
+                      unsigned* counter, bool want_dot, hipStream_t s, const launch_hook& hook = {});
 
 void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,
                          hipStream_t s);

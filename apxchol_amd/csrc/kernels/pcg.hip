@@ -46,6 +46,75 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
     }
 }
 
+// y = A x over the sliced-ELL form (sell_desc): one lane per row, a slice's
+// entries column-major so every entry column is one coalesced load, eight
+// columns' loads and gathers in flight per lane; rows too long for a slice are
+// walked by a whole wave from the CSR arrays (lane-strided + butterfly). A
+// row's sum is its entries in CSR order (padding adds fma(0, x[row], s) = s).
+// With want_dot, x.y lands in st->pAp as in spmv_pAp.
+template <class VT, bool MAP>
+__global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* __restrict__ x, double* __restrict__ y,
+                                                    pcg_state* st, double* part, unsigned* counter, int want_dot) {
+    if (st && st->done) return;
+    __shared__ double sh[kWavesPerBlock * 2];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * kWavesPerBlock +
+                                                    static_cast<int>(threadIdx.x) / kWave);
+    const int waves = static_cast<int>(gridDim.x) * kWavesPerBlock;
+    const int items = a.nslices + a.nlong;
+    const int* __restrict__ sidx = a.idx;
+    const VT* __restrict__ sval = static_cast<const VT*>(a.val);
+    double acc = 0.0;
+    for (int it = wave; it < items; it += waves) {
+        if (it < a.nslices) {
+            const unsigned c0 = a.soff[it], c1 = a.soff[it + 1];
+            const int w = static_cast<int>(c1 - c0);
+            const size_t base = static_cast<size_t>(c0) * kWave + lane;
+            const int row = MAP ? a.rowmap[static_cast<size_t>(it) * kWave + lane] : it * kWave + lane;
+            double s = 0.0;
+            int k = 0;
+            for (; k + 8 <= w; k += 8) {
+                int j[8];
+                double v[8], g[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    j[u] = sidx[base + static_cast<size_t>(k + u) * kWave];
+                    v[u] = static_cast<double>(sval[base + static_cast<size_t>(k + u) * kWave]);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) g[u] = x[j[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s = fma(v[u], g[u], s);
+            }
+            for (; k < w; ++k) {
+                const size_t q = base + static_cast<size_t>(k) * kWave;
+                s = fma(static_cast<double>(sval[q]), x[sidx[q]], s);
+            }
+            if (row >= 0 && row < a.n) {
+                y[row] = s;
+                acc = fma(x[row], s, acc);
+            }
+        } else {
+            const int row = a.longrows[it - a.nslices];
+            const unsigned b = a.ptr[row], e = a.ptr[row + 1];
+            const VT* __restrict__ cval = static_cast<const VT*>(a.cval);
+            double s = 0.0;
+            for (unsigned p = b + lane; p < e; p += kWave) s = fma(static_cast<double>(cval[p]), x[a.cidx[p]], s);
+            s = wave_sum(s);
+            if (lane == 0) {
+                y[row] = s;
+                acc = fma(x[row], s, acc);
+            }
+        }
+    }
+    if (!want_dot) return;
+    double v[1] = {acc};
+    if (grid_sum<1>(v, part, counter, sh)) {
+        st->pAp = v[0];
+        if (v[0] <= 0.0) { st->done = 1; st->status = 2; }
+    }
+}
+
 // r[perm[v]] = b[v]; x[perm[v]] = x0[v] (or 0).
 __global__ __launch_bounds__(kBlock) void scatter_init(int n, const unsigned* __restrict__ perm,
                                                        const double* __restrict__ b, const double* __restrict__ x0,

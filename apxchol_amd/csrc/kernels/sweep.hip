@@ -286,6 +286,52 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
     for (int b = b0 + w; b < b1; b += waves) sweep_batch<TI, TO, F16, false>(d, b, rhs, out, shift, err);
 }
 
+// One launched level from the static lane layout: wave w runs lane batch
+// lb0 + w. The layout loads are issued before the kernel reads the solve
+// state, so the completion test does not add a dependent load; the per-row
+// arithmetic is sweep_batch's (same slots, fma chain and butterfly), so the
+// bits equal the other schedules'.
+template <class TI, class TO, bool F16>
+__global__ __launch_bounds__(kBlock) void sweep_lanes(sweep_desc d, int lb0, int nlb, const lane_rec* __restrict__ rec,
+                                                      const entry32* __restrict__ ent, const TI* __restrict__ rhs,
+                                                      TO* out, const pcg_state* st) {
+    using AT = TO;
+    const int w = static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    if (w >= nlb) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const size_t b = static_cast<size_t>(lb0 + w);
+    const lane_rec rc = rec[b * kWave + lane];
+    entry32 e[kEpl];
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) e[k] = ent[(b * kEpl + k) * kWave + lane];
+    asm volatile("" ::: "memory");
+    if (st && st->done) return;
+    const double shift = (st && !d.backward) ? st->shift : 0.0;
+    const int row = rc.row;
+    const int G = 1 << (rc.info & 7u);
+    const int sub = static_cast<int>((rc.info >> 3) & 63u);
+    const int nv = static_cast<int>((rc.info >> 9) & 7u);
+    AT xv[kEpl];
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) xv[k] = k < nv ? out[e[k].idx] : static_cast<AT>(0);
+    const bool head = row >= 0 && sub == 0;
+    AT r = 0;
+    AT dg = 1;
+    if (head) {
+        r = row_rhs<TI, AT>(d, rhs, row, shift);
+        dg = static_cast<AT>(d.diag[row]);
+    }
+    AT part = 0;
+#pragma unroll
+    for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(e[k].val), xv[k], part);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const AT o = __shfl_xor(part, off, kWave);
+        if (off < G) part += o;
+    }
+    if (head) out[row] = (r - part) / dg;
+}
+
 // A run of small levels [l0, l1) -- sweep positions [p0, p1) -- walked by ONE
 // workgroup of kTailBlock threads. The run's outputs are kept in LDS as well as
 // written out, so a dependency inside the run is an LDS read after the level

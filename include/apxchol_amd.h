@@ -49,6 +49,10 @@ typedef struct {
                                 0: eager launches (also used while kernel timing is on) */
     int32_t relabel;         /* 1 (default): device vectors order every level's rows by original
                                 vertex (same factor, local gathers); 0: the factor order */
+    int32_t lane_batches;    /* launched levels of at most this many batches read the static lane
+                                layout (0 = default 4096, < 0 = never: batch tables only) */
+    int32_t depth_levels;    /* 1 (default, with relabel): sweep levels = topological depth of the
+                                dependency graph; 0: the elimination rounds (round-as-level) */
 } apx_options;
 
 /* Fill `o` with the defaults above. */
@@ -77,6 +81,9 @@ typedef struct {
      * width) + 12 per batch). */
     uint64_t level_bytes[2], tail_bytes[2];
     int64_t level_launches[2], tail_runs[2];
+    int64_t lane_levels[2];  /* launched levels that read the static lane layout */
+    int32_t spmv_format;     /* single SpMV: 0 CSR lanes, 1 sliced ELL (APXCHOL_AMD_SPMV=csr|sell forces) */
+    uint64_t spmv_sell_slots; /* padded entries of the sliced-ELL form (0: none built) */
 } apx_solver_info;
 
 const char* apx_last_error(void);
@@ -156,7 +163,8 @@ apx_status apx_solver_apply_batch(apx_solver_t s, int32_t k, const double* d_R, 
 
 /* Autotuner (no counterpart in the reference): measures, for this solver on
  * this device, the SpMV lanes per row (1..64; the solver then uses the
- * fastest, for the single SpMV and the blocked SpMM alike) and the block
+ * fastest, for the single SpMV and the blocked SpMM alike; the single SpMV
+ * keeps the sliced-ELL form instead when that is faster still) and the block
  * width of blocked solves up to max_block_width (the most columns swept per
  * second; a server created with slots = 0 uses it). Times in microseconds:
  * spmv_us[i] for 2^i lanes, sweep_pair_us[i] for block width 8 << i. */
@@ -164,6 +172,8 @@ typedef struct {
     int32_t spmv_lanes, block_width;
     double spmv_us[7];
     double sweep_pair_us[4];
+    int32_t spmv_sell;       /* 1: the single SpMV now runs on the sliced-ELL form (faster than any lanes) */
+    double spmv_sell_us;     /* its time (0: the solver has no sliced-ELL form) */
 } apx_autotune_result;
 apx_status apx_solver_autotune(apx_solver_t s, int32_t max_block_width, apx_autotune_result* out, void* stream);
 
@@ -206,6 +216,12 @@ apx_status apx_solver_reset_apply_count(apx_solver_t s);
  * device boundary of dataflow_solve (include/apxchol/solver/sptrsv/cuda_dataflow.h:185-208)
  * and omp_sptrsv::forward_solve / transpose_solve (omp.h:1048-1058). d_rhs, d_out: m doubles. */
 apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rhs, double* d_out, void* stream);
+
+/* y = L x in the caller's order with the solver's single-vector SpMV (sliced
+ * ELL or CSR lanes, fp64 accumulation) -- the device boundary of
+ * pcg_cuda::spmv_pAp (include/apxchol/solver/pcg_cuda_kernels.h:75-93) without
+ * the dot product. d_x, d_y: n doubles. */
+apx_status apx_solver_spmv(apx_solver_t s, const double* d_x, double* d_y, void* stream);
 
 /* Kernel timing with HIP events on the launching stream (no reference
  * counterpart; the reference's APXCHOL_PCG_TRACE per-component ms/iter,

@@ -67,25 +67,77 @@ def test_sweeps_fp64_match_oracle(gpu, name, schedule):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["grid2d_64", "sddm3d_16", "rmat_16"])
-def test_schedules_bit_identical(gpu, name):
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("fmt", ["auto", "csr", "sell"])
+def test_spmv_matches_operator(gpu, name, fmt, monkeypatch):
+    """The single-vector SpMV in each layout -- CSR lanes, sliced ELL in row
+    order, sliced ELL sorted by length with a row map and wave-walked long
+    rows (R-MAT) -- against scipy's fp64 A x, row by row within 1e-13 of
+    (|A| |x|)_i (the rounding of a fixed-order sum of <= a few hundred terms)."""
+    import torch
+    A, F, _ = case(name)
+    print(f"[case] spmv {name} {fmt}", flush=True)
+    if fmt != "auto":
+        monkeypatch.setenv("APXCHOL_AMD_SPMV", fmt)
+    s = ax.Solver(A, factor=F)
+    if fmt == "csr":
+        assert s.info["spmv_format"] == 0 and s.info["spmv_sell_slots"] == 0
+    if fmt == "sell":
+        assert s.info["spmv_format"] == 1 and s.info["spmv_sell_slots"] > 0   # long rows are not in slices
+    x = np.random.default_rng(12).standard_normal(A.shape[0])
+    y = s.spmv(torch.tensor(x, device=gpu)).cpu().numpy()
+    y_ref = A @ x
+    bound = abs(A) @ abs(x)
+    assert np.all(np.abs(y - y_ref) <= 1e-13 * bound + 1e-300), float(np.max(np.abs(y - y_ref) / (bound + 1e-300)))
+    y2 = s.spmv(torch.tensor(x, device=gpu)).cpu().numpy()
+    assert np.array_equal(y.view(np.int64), y2.view(np.int64))
+    s.close()
+
+
+SCHEDULE_CONFIGS = [
+    dict(schedule="hybrid"),
+    dict(schedule="dataflow"),
+    dict(schedule="levels"),
+    dict(schedule="levels", lane_batches=-1),    # every level from the batch tables
+    dict(schedule="hybrid", lane_batches=-1),
+    dict(schedule="levels", lane_batches=1),     # lane layout for one-batch levels only
+]
+
+
+@pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "rmat_16"])
+@pytest.mark.parametrize("precision,fp16", [("fp64", False), ("fp32", False), ("fp32", True)])
+def test_schedules_bit_identical(gpu, name, precision, fp16):
+    """Every schedule -- the dataflow launch, per-level launches from the batch
+    tables or from the static lane layout, the hybrid with its one-workgroup
+    tail -- gives the same bits: two sweeps with fp64 vectors, and one
+    preconditioner application in the solver's own vector precision."""
     import torch
     A, F, fac = case(name)
-    print(f"[case] schedules {name}", flush=True)
-    outs = []
-    for schedule in ("hybrid", "dataflow", "levels"):
-        s = ax.Solver(A, factor=F, schedule=schedule)
+    print(f"[case] schedules {name} {precision} fp16={fp16}", flush=True)
+    sweeps, applies = [], []
+    for cfg in SCHEDULE_CONFIGS:
+        s = ax.Solver(A, factor=F, precision=precision, fp16_storage=fp16, **cfg)
+        lanes = sum(s.info["lane_levels"])
+        if cfg["schedule"] == "dataflow" or cfg.get("lane_batches", 0) < 0:
+            assert lanes == 0, cfg
+        elif cfg.get("lane_batches", 0) == 0:
+            assert lanes > 0, cfg
         m = s.info["m"]
         x = torch.tensor(np.random.default_rng(3).standard_normal(m), device=gpu)
         y = torch.empty_like(x)
         z = torch.empty_like(x)
         s.sweep(x, y, backward=False)
         s.sweep(y, z, backward=True)
+        r = torch.tensor(np.random.default_rng(4).standard_normal(A.shape[0]), device=gpu)
+        s.reset_apply_count()
+        a = s.apply(r)
         torch.cuda.synchronize()
-        outs.append(z.cpu().numpy().view(np.int64).copy())
+        sweeps.append(z.cpu().numpy().view(np.int64).copy())
+        applies.append(a.cpu().numpy().view(np.int64).copy())
         s.close()
-    assert np.array_equal(outs[0], outs[1])
-    assert np.array_equal(outs[0], outs[2])
+    for k in range(1, len(SCHEDULE_CONFIGS)):
+        assert np.array_equal(sweeps[0], sweeps[k]), SCHEDULE_CONFIGS[k]
+        assert np.array_equal(applies[0], applies[k]), SCHEDULE_CONFIGS[k]
 
 
 @pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "aniso3d_16", "rmat_12"])
@@ -214,16 +266,23 @@ def test_graph_replay_matches_eager(gpu, name, precision):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["grid2d_256", "sddm3d_16", "rmat_12"])
-def test_relabel_matches_factor_order(gpu, name):
-    """Sorting each level's rows by original vertex changes only the storage
-    order (and so the order of each row's entries): sweeps agree with the
-    factor-order solver to 1e-12, solves within one iteration and 1e-6."""
+@pytest.mark.parametrize("name", ["grid2d_256", "sddm3d_16", "rmat_12", "rmat_16"])
+@pytest.mark.parametrize("depth", [False, True])
+def test_relabel_matches_factor_order(gpu, name, depth):
+    """Sorting each level's rows by original vertex -- with the elimination
+    rounds as levels, or with levels regrouped by topological depth -- changes
+    only the storage order (and so the order of each row's entries): sweeps
+    agree with the factor-order solver to 1e-12, solves within one iteration
+    and 1e-6. Depth levels are never more than the rounds."""
     import torch
     A, F, fac = case(name)
-    print(f"[case] relabel {name}", flush=True)
+    print(f"[case] relabel {name} depth={depth}", flush=True)
     s0 = ax.Solver(A, factor=F, relabel=False)
-    s1 = ax.Solver(A, factor=F, relabel=True)
+    s1 = ax.Solver(A, factor=F, relabel=True, depth_levels=depth)
+    if depth:
+        assert s1.info["levels"] <= s0.info["levels"]
+    else:
+        assert s1.info["levels"] == s0.info["levels"]
     m = s0.info["m"]
     x = torch.tensor(np.random.default_rng(8).standard_normal(m), device=gpu)
     for backward in (False, True):
