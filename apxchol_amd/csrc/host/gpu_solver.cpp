@@ -476,7 +476,7 @@ struct gpu_solver::device_state {
     int lanes = 8;
     dbuf r, x, p, ap, rp, y, z, xout;
     dbuf relabel;   // factor position -> storage position (empty: identity)
-    dbuf part, counter, st, st_apply, ctrl, err;
+    dbuf part, st, st_apply, ctrl, err;
     pcg_state* host_st = nullptr;
     int* host_err = nullptr;
     sweep_desc fwd, bck;
@@ -496,7 +496,7 @@ struct gpu_solver::device_state {
     struct block_bufs {
         int kp = 0;
         std::map<int, std::unique_ptr<block_tables>> tables;   // per block width
-        dbuf r, x, p, ap, y, z, st, bs, part, counter;
+        dbuf r, x, p, ap, y, z, st, bs, part;
         pcg_state* host_st = nullptr;
         batch_state* host_bs = nullptr;
         hipGraph_t graph = nullptr;
@@ -751,8 +751,6 @@ void gpu_solver::setup(const csc_view& L) {
     d.r.alloc(vb); d.x.alloc(vb); d.p.alloc(vb); d.ap.alloc(vb); d.rp.alloc(vb); d.y.alloc(vb); d.z.alloc(vb);
     d.xout.alloc(vb);
     d.part.alloc(2 * 2048 * sizeof(double));
-    d.counter.alloc(64);
-    d.counter.zero();
     d.st.alloc(sizeof(pcg_state));
     d.st_apply.alloc(sizeof(pcg_state));
     d.st_apply.zero();
@@ -978,7 +976,6 @@ void gpu_solver::enqueue_precondition(bool first, int step, hipStream_t s) const
     const int m = static_cast<int>(T_.m);
     pcg_state* st = d.st.as<pcg_state>();
     double* part = d.part.as<double>();
-    unsigned* cnt = d.counter.as<unsigned>();
     int* err = d.err.as<int>();
     const vec_prec pr = d.prec;
     if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
@@ -989,7 +986,7 @@ void gpu_solver::enqueue_precondition(bool first, int step, hipStream_t s) const
         launch_sweep(d.bck, pr, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s, tm.hook(step));
     });
     tm.time(3, step, s, [&] {
-        launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, cnt, first, s);
+        launch_dot_center(pr, n, d.r.as<double>(), d.z.get(), st, part, first, s);
         launch_update_p(pr, n, d.p.as<double>(), d.z.get(), st, first, s);
     });
 }
@@ -998,10 +995,10 @@ void gpu_solver::spmv(const double* x, double* y, pcg_state* st, bool want_dot, 
                       const launch_hook& hook) const {
     auto& d = *dev_;
     if (d.use_sell)
-        launch_spmv_sell(d.sell, d.op32, x, y, st, d.part.as<double>(), d.counter.as<unsigned>(), want_dot, s, hook);
+        launch_spmv_sell(d.sell, d.op32, x, y, st, d.part.as<double>(), want_dot, s, hook);
     else
         launch_spmv(static_cast<int>(n_), d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, d.lanes,
-                    x, y, st, d.part.as<double>(), d.counter.as<unsigned>(), want_dot, s, hook);
+                    x, y, st, d.part.as<double>(), want_dot, s, hook);
 }
 
 void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
@@ -1011,11 +1008,10 @@ void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
     const int m = static_cast<int>(T_.m);
     pcg_state* st = d.st.as<pcg_state>();
     double* part = d.part.as<double>();
-    unsigned* cnt = d.counter.as<unsigned>();
     spmv(d.p.as<double>(), d.ap.as<double>(), st, true, s, tm.hook(it - 1));
     tm.time(3, it - 1, s, [&] {
         launch_update_xr(d.prec, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
-                         d.y.get(), d.z.get(), st, part, cnt, s);
+                         d.y.get(), d.z.get(), st, part, s);
     });
     enqueue_precondition(false, it, s);
 }
@@ -1026,7 +1022,6 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     if (max_iter < 0) max_iter = opts_.max_iter;
     auto& d = *dev_;
     const int n = static_cast<int>(n_);
-    const int m = static_cast<int>(T_.m);
     solve_result res;
     res.lumped_offdiag = F_.lumped_offdiag;
     if (n == 0) return res;
@@ -1041,18 +1036,15 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     h.prev_check = 1.0;
     pcg_state* st = d.st.as<pcg_state>();
     double* part = d.part.as<double>();
-    unsigned* cnt = d.counter.as<unsigned>();
-    int* err = d.err.as<int>();
-    const vec_prec pr = d.prec;
     hip_check(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s), "state upload");
 
     launch_scatter_init(n, d.perm.as<unsigned>(), d_b, d_x0, d.r.as<double>(), d.x.as<double>(), s);
     if (d_x0) {
-        launch_norm2_b(n, d_b, st, part, cnt, s);
+        launch_norm2_b(n, d_b, st, part, s);
         spmv(d.x.as<double>(), d.ap.as<double>(), st, false, s, {});
-        launch_init_residual(n, d.r.as<double>(), d.ap.as<double>(), st, part, cnt, s);
+        launch_init_residual(n, d.r.as<double>(), d.ap.as<double>(), st, part, s);
     } else {
-        launch_init_residual(n, d.r.as<double>(), nullptr, st, part, cnt, s);
+        launch_init_residual(n, d.r.as<double>(), nullptr, st, part, s);
     }
     timer& tm = *timer_;
     enqueue_precondition(true, 0, s);
@@ -1086,7 +1078,7 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
         hip_check(hipStreamSynchronize(s), "solve");
         if (h.done || enqueued > max_iter + J) break;
     }
-    if (!F_.sddm) launch_sum_vec(n, d.x.as<double>(), st, part, cnt, s);
+    if (!F_.sddm) launch_sum_vec(n, d.x.as<double>(), st, part, s);
     launch_gather_shift(n, d.perm.as<unsigned>(), d.x.as<double>(), d_x, st, !F_.sddm, s);
     hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
     hip_check(hipStreamSynchronize(s), "solve");
@@ -1119,8 +1111,6 @@ int gpu_solver::prepare_block(int k) const {
     // Partials: 2 values x kp columns per workgroup; the SpMM grid may round
     // kMaxBlocksHost up by at most 15 (a multiple of G kp / 256 <= 16).
     b.part.alloc(static_cast<std::size_t>(kMaxBlocksHost + 16) * 2 * kp * sizeof(double));
-    b.counter.alloc(64);
-    b.counter.zero();
     if (b.host_st) (void)hipHostFree(b.host_st);
     b.host_st = nullptr;
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&b.host_st), static_cast<std::size_t>(kp) * sizeof(pcg_state),
@@ -1152,21 +1142,20 @@ void gpu_solver::apply_block(int k, const double* d_R, double* d_Z, bool center,
     pcg_state* st = b.st.as<pcg_state>();
     batch_state* bs = b.bs.as<batch_state>();
     double* part = b.part.as<double>();
-    unsigned* cnt = b.counter.as<unsigned>();
     hip_check(hipMemcpyAsync(st, b.host_st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyHostToDevice, s),
               "state upload");
     hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
     launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_R, b.r.as<double>(), b.x.as<double>(), s);
     launch_zero_tail_k(d.prec, n, m, kp, b.z.get(), s);
     if (center) {
-        launch_sum_cols_k(vec_prec::fp64, n, kp, b.r.get(), st, part, cnt, s);
+        launch_sum_cols_k(vec_prec::fp64, n, kp, b.r.get(), st, part, s);
         launch_shift_from_sum_k(kp, st, true, s);
     }
     block_scheds(kp);
     const auto& t = *b.tables.at(kp);
     launch_sweep_k(d.fwd, d.prec, d.fp16, t.f, b.r.get(), b.y.get(), st, bs, s);
     launch_sweep_k(d.bck, d.prec, d.fp16, t.b, b.y.get(), b.z.get(), st, bs, s);
-    if (center) launch_sum_cols_k(d.prec, n, kp, b.z.get(), st, part, cnt, s);
+    if (center) launch_sum_cols_k(d.prec, n, kp, b.z.get(), st, part, s);
     launch_gather_k(d.prec, n, k, kp, d.perm.as<unsigned>(), b.z.get(), d_Z, st, center, s);
 }
 
@@ -1259,8 +1248,7 @@ void gpu_solver::enqueue_precondition_k(bool first, int kp, hipStream_t s) const
     const auto& t = *b.tables.at(kp);
     launch_sweep_k(d.fwd, d.prec, d.fp16, t.f, b.r.get(), b.y.get(), st, bs, s);
     launch_sweep_k(d.bck, d.prec, d.fp16, t.b, b.y.get(), b.z.get(), st, bs, s);
-    launch_dot_center_k(d.prec, n, kp, b.r.as<double>(), b.z.get(), st, bs, b.part.as<double>(),
-                        b.counter.as<unsigned>(), first, s);
+    launch_dot_center_k(d.prec, n, kp, b.r.as<double>(), b.z.get(), st, bs, b.part.as<double>(), first, s);
     launch_update_p_k(d.prec, n, kp, b.p.as<double>(), b.z.get(), st, bs, first, s);
 }
 
@@ -1271,9 +1259,9 @@ void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s) const {
     pcg_state* st = b.st.as<pcg_state>();
     batch_state* bs = b.bs.as<batch_state>();
     launch_spmm_k(n, kp, d.lanes, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32,
-                  b.p.as<double>(), b.ap.as<double>(), st, bs, b.part.as<double>(), b.counter.as<unsigned>(), true, s);
+                  b.p.as<double>(), b.ap.as<double>(), st, bs, b.part.as<double>(), true, s);
     launch_update_xr_k(n, kp, b.x.as<double>(), b.p.as<double>(), b.r.as<double>(), b.ap.as<double>(), st, bs,
-                       b.part.as<double>(), b.counter.as<unsigned>(), s);
+                       b.part.as<double>(), s);
     launch_refresh_active_k(kp, st, bs, s);
     enqueue_precondition_k(false, kp, s);
     launch_end_iteration_k(kp, st, bs, s);
@@ -1345,7 +1333,7 @@ gpu_solver::autotune_result gpu_solver::autotune(int max_block_width, hipStream_
         for (int k = 0, lanes = 1; lanes <= 64; ++k, lanes <<= 1) {
             res.spmv_us[k] = timed([&] {
                 launch_spmv(n, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32, lanes,
-                            d.p.as<double>(), d.ap.as<double>(), nullptr, nullptr, nullptr, false, s);
+                            d.p.as<double>(), d.ap.as<double>(), nullptr, nullptr, false, s);
             }, 5);
             if (res.spmv_us[k] < best) { best = res.spmv_us[k]; res.spmv_lanes = lanes; }
         }
@@ -1354,7 +1342,7 @@ gpu_solver::autotune_result gpu_solver::autotune(int max_block_width, hipStream_
         // The sliced-ELL form, when it exists, against the best lane count.
         if (d.has_sell) {
             res.spmv_sell_us = timed([&] {
-                launch_spmv_sell(d.sell, d.op32, d.p.as<double>(), d.ap.as<double>(), nullptr, nullptr, nullptr, false, s);
+                launch_spmv_sell(d.sell, d.op32, d.p.as<double>(), d.ap.as<double>(), nullptr, nullptr, false, s);
             }, 5);
             d.use_sell = res.spmv_sell_us < best;
         }
@@ -1442,7 +1430,7 @@ void gpu_solver::session_admit(const std::vector<session_column>& cols, hipStrea
     launch_admit_scatter_k(static_cast<int>(n_), kp, d.perm.as<unsigned>(), nc, dcols, dsrc, b.r.as<double>(),
                            b.x.as<double>(), s);
     launch_init_residual_k(static_cast<int>(n_), kp, b.r.as<double>(), b.st.as<pcg_state>(), b.bs.as<batch_state>(),
-                           b.part.as<double>(), b.counter.as<unsigned>(), s);
+                           b.part.as<double>(), s);
     launch_refresh_active_k(kp, b.st.as<pcg_state>(), b.bs.as<batch_state>(), s);
     // The host copies must stay valid until the stream has consumed them.
     hip_check(hipStreamSynchronize(s), "admit");
@@ -1485,8 +1473,7 @@ void gpu_solver::session_collect(const std::vector<int>& slots, const std::vecto
               "collect upload");
     pcg_state* st = b.st.as<pcg_state>();
     if (!F_.sddm)
-        launch_sum_cols_k(vec_prec::fp64, static_cast<int>(n_), kp, b.x.get(), st, b.part.as<double>(),
-                          b.counter.as<unsigned>(), s);
+        launch_sum_cols_k(vec_prec::fp64, static_cast<int>(n_), kp, b.x.get(), st, b.part.as<double>(), s);
     launch_gather_cols_k(static_cast<int>(n_), kp, d.perm.as<unsigned>(), nc, b.adm_dev.as<int>(),
                          reinterpret_cast<double* const*>(b.adm_dev.as<unsigned char>() + 64 * sizeof(int)),
                          b.x.as<double>(), st, !F_.sddm, s);
@@ -1514,7 +1501,7 @@ std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, doub
     hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
     launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_B, b.r.as<double>(), b.x.as<double>(), s);
     launch_zero_tail_k(d.prec, n, m, kp, b.z.get(), s);
-    launch_init_residual_k(n, kp, b.r.as<double>(), st, bs, b.part.as<double>(), b.counter.as<unsigned>(), s);
+    launch_init_residual_k(n, kp, b.r.as<double>(), st, bs, b.part.as<double>(), s);
     launch_refresh_active_k(kp, st, bs, s);
     const int J = std::max(1, opts_.iters_per_check);
     int enqueued = 0;
@@ -1526,7 +1513,7 @@ std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, doub
         if (b.host_bs->active == 0 || enqueued > max_iter + J + 1) break;
     }
     if (!F_.sddm)
-        launch_sum_cols_k(vec_prec::fp64, n, kp, b.x.get(), st, b.part.as<double>(), b.counter.as<unsigned>(), s);
+        launch_sum_cols_k(vec_prec::fp64, n, kp, b.x.get(), st, b.part.as<double>(), s);
     launch_gather_k(vec_prec::fp64, n, k, kp, d.perm.as<unsigned>(), b.x.get(), d_X, st, !F_.sddm, s);
     hip_check(hipMemcpyAsync(b.host_st, st, static_cast<std::size_t>(kp) * sizeof(pcg_state), hipMemcpyDeviceToHost, s),
               "state read");
@@ -1596,13 +1583,12 @@ void gpu_solver::apply_device(const double* d_r, double* d_z, hipStream_t s) con
     const bool center = !F_.sddm && (++apply_count_ % static_cast<std::uint64_t>(center_k_) == 0);
     pcg_state* st = d.st_apply.as<pcg_state>();
     double* part = d.part.as<double>();
-    unsigned* cnt = d.counter.as<unsigned>();
     int* err = d.err.as<int>();
-    launch_sum_shift(n, d_r, st, part, cnt, center, s);
+    launch_sum_shift(n, d_r, st, part, center, s);
     launch_apply_scatter(d.prec, n, m, d.perm.as<unsigned>(), d_r, d.rp.as<double>(), d.y.get(), d.z.get(), s);
     launch_sweep(d.fwd, d.prec, d.fp16, d.rp.get(), d.y.get(), st, err, d.f_sched, s);
     launch_sweep(d.bck, d.prec, d.fp16, d.y.get(), d.z.get(), st, err, d.b_sched, s);
-    if (center) launch_sum_out(d.prec, n, d.z.get(), st, part, cnt, s);
+    if (center) launch_sum_out(d.prec, n, d.z.get(), st, part, s);
     launch_apply_gather(d.prec, n, d.perm.as<unsigned>(), d.z.get(), d_z, st, center, s);
 }
 

@@ -210,26 +210,19 @@ __global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, con
 
 // ---- per-column reductions -----------------------------------------------
 
-// Sum N values per thread over the threads of one column in a block (kp
-// columns, kBlock / kp rows per block), in row order, then over blocks in
-// block order (last block). Thread c < kp of the last block returns true
-// with the totals of column c in v.
+// Two-stage per-column reductions (kp columns, kBlock / kp rows per block),
+// as block_partials / finish_sum for one vector. Stage 1: the block's column
+// sums, in row order, to part[(block * N + k) * kp + j].
 //
 // Lane mapping: thread t of the grid serves column ((t / G) % kp) and only
 // its chain-0 lanes (t % G == 0) carry values (the others hold exact zeros,
 // which the block partial skips); G = 1 is one thread per (row, column).
-// The totals come back in threads 0 .. kp-1 of the last workgroup: thread j
-// holds column j, whatever column it served.
 template <int N>
-__device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* part, unsigned* counter, double* sh,
-                                             int G = 1) {
+__device__ __forceinline__ void col_block_partials(double (&v)[N], int kp, double* part, double* sh, int G = 1) {
     const int c = threadIdx.x & (kp - 1);
-    const int rows = kBlock / kp;
-    __shared__ double col[N * 64];
 #pragma unroll
     for (int k = 0; k < N; ++k) sh[k * kBlock + threadIdx.x] = v[k];
     __syncthreads();
-    __shared__ int is_last;
     if (threadIdx.x < kp) {
         const int U = G * kp;
 #pragma unroll
@@ -243,36 +236,26 @@ __device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* par
                 const int off = (c - first + kp) % kp;
                 if (off < kBlock / G) s = sh[k * kBlock + off * G];
             }
-            col[k * 64 + c] = s;
+            part[(static_cast<size_t>(blockIdx.x) * N + k) * kp + c] = s;
         }
     }
-    __syncthreads();
-    // One thread publishes the block's partials, then counts the block in
-    // (the same store -> fence -> counter order as grid_sum).
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < N; ++k)
-            for (int j = 0; j < kp; ++j)
-                __hip_atomic_store(&part[(static_cast<size_t>(blockIdx.x) * N + k) * kp + j], col[k * 64 + j],
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    // Last block: thread (j, c) sums the partials of blocks j, j + rows, ...
-    // of column c (all loads in flight at once), then column c's `rows`
-    // sums are added in j order.
+}
+
+// Stage 2 (one workgroup) over the partials of nb blocks: thread (j, c) sums
+// blocks j, j + rows, ... of column c (all loads in flight at once), then
+// column c's `rows` sums are added in j order. Thread c < kp returns true
+// with the totals of column c in v.
+template <int N>
+__device__ __forceinline__ bool col_finish_sum(double (&v)[N], int kp, const double* part, int nb, double* sh) {
+    const int c = threadIdx.x & (kp - 1);
+    const int rows = kBlock / kp;
     const int j = threadIdx.x / kp;
     double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] = 0.0;
-    for (unsigned b = static_cast<unsigned>(j); b < gridDim.x; b += static_cast<unsigned>(rows))
+    for (int b = j; b < nb; b += rows)
 #pragma unroll
-        for (int k = 0; k < N; ++k)
-            acc[k] += __hip_atomic_load(&part[(static_cast<size_t>(b) * N + k) * kp + c], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < N; ++k) acc[k] += part[(static_cast<size_t>(b) * N + k) * kp + c];
 #pragma unroll
     for (int k = 0; k < N; ++k) sh[k * kBlock + threadIdx.x] = acc[k];
     __syncthreads();
@@ -283,7 +266,6 @@ __device__ __forceinline__ bool col_grid_sum(double (&v)[N], int kp, double* par
         for (int i = 1; i < rows; ++i) s += sh[k * kBlock + i * kp + c];
         v[k] = s;
     }
-    if (c == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
@@ -296,15 +278,14 @@ inline int block_grid(long long rows, int kp) { return vec_blocks(rows * kp); }
 // Y = A P for kp columns with spmv_pAp's lanes: thread t serves row
 // t / (G kp), column (t / G) % kp, chain t % G (lane-strided entries, fma
 // from 0, butterfly over the G lanes), so every column gets the single
-// SpMV's bits. With want_dot, p.Ap per column into st[c].pAp (breakdown
-// ends that column). The grid stride is a multiple of G kp (host), so a
+// SpMV's bits. With want_dot, the per-column partials of p.Ap go to part
+// (block_finish kBFinPAp: st[c].pAp; breakdown ends that column). The grid stride is a multiple of G kp (host), so a
 // thread's column is fixed.
 template <class VT>
 __global__ __launch_bounds__(kBlock) void spmm_pAp_k(int n, int kp, int G, const unsigned* __restrict__ ptr,
                                                      const int* __restrict__ idx, const VT* __restrict__ val,
                                                      const double* __restrict__ x, double* __restrict__ y,
-                                                     pcg_state* st, batch_state* bs, double* part, unsigned* counter,
-                                                     int want_dot) {
+                                                     const batch_state* bs, double* part, int want_dot) {
     if (bs->active == 0) return;
     __shared__ double sh[kBlock];
     const long long U = static_cast<long long>(G) * kp;
@@ -328,15 +309,7 @@ __global__ __launch_bounds__(kBlock) void spmm_pAp_k(int n, int kp, int G, const
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    // col_grid_sum returns column j's total in thread j (< kp) of the last
-    // workgroup -- not this thread's SpMM column.
-    if (col_grid_sum<1>(v, kp, part, counter, sh, G)) {
-        pcg_state& t = st[threadIdx.x];
-        if (!t.done && t.phase == 0) {
-            t.pAp = v[0];
-            if (v[0] <= 0.0) { t.done = 1; t.status = 2; }
-        }
-    }
+    col_block_partials<1>(v, kp, part, sh, G);
 }
 
 // Workgroups of the SpMM: a multiple of G kp / kBlock when one row's lanes
@@ -382,11 +355,10 @@ __device__ __forceinline__ void count_active(int kp, const pcg_state* st, batch_
     }
 }
 
-// rr, rs per column of r; the start of each joining column's solve (zero
-// right-hand side, max_iter, first centring decision). Other columns are
-// left as they are.
-__global__ __launch_bounds__(kBlock) void init_residual_k(int n, int kp, const double* __restrict__ r, pcg_state* st,
-                                                          batch_state* bs, double* part, unsigned* counter) {
+// Partials of rr, rs per column of r (block_finish kBFinInit starts each
+// joining column's solve: zero right-hand side, max_iter, first centring
+// decision; other columns are left as they are).
+__global__ __launch_bounds__(kBlock) void init_residual_k(int n, int kp, const double* __restrict__ r, double* part) {
     __shared__ double sh[2 * kBlock];
     const int c = threadIdx.x & (kp - 1);
     const int rows_per_grid = gridDim.x * (kBlock / kp);
@@ -397,28 +369,15 @@ __global__ __launch_bounds__(kBlock) void init_residual_k(int n, int kp, const d
         s += ri;
     }
     double v[2] = {q, s};
-    const bool last = col_grid_sum<2>(v, kp, part, counter, sh);
-    if (last && !st[c].done && st[c].phase == 1) {
-        pcg_state& t = st[c];
-        t.rr = v[0];
-        t.rs = v[1];
-        t.iter = 0;
-        t.prev_check = 1.0;
-        t.apply_count = 0;
-        t.bnorm = sqrt(v[0]);
-        t.residual = 1.0;
-        if (t.bnorm == 0.0) { t.done = 1; t.status = 5; t.residual = 0.0; }
-        else if (t.max_iter <= 0) { t.done = 1; t.status = 4; }
-        else next_application_k(t, v[1]);
-    }
+    col_block_partials<2>(v, kp, part, sh);
 }
 
-// x += alpha p; r -= alpha Ap per live column; rr, rs; each column's loop
-// bookkeeping as update_xr's.
+// x += alpha p; r -= alpha Ap per live column; partials of rr, rs (each
+// column's loop bookkeeping, as update_xr's, in block_finish kBFinXR).
 __global__ __launch_bounds__(kBlock) void update_xr_k(int n, int kp, double* __restrict__ x,
                                                       const double* __restrict__ p, double* __restrict__ r,
-                                                      const double* __restrict__ ap, pcg_state* st, batch_state* bs,
-                                                      double* part, unsigned* counter) {
+                                                      const double* __restrict__ ap, const pcg_state* st,
+                                                      const batch_state* bs, double* part) {
     if (bs->active == 0) return;
     __shared__ double sh[2 * kBlock];
     const int c = threadIdx.x & (kp - 1);
@@ -437,29 +396,15 @@ __global__ __launch_bounds__(kBlock) void update_xr_k(int n, int kp, double* __r
         }
     }
     double v[2] = {q, s};
-    if (col_grid_sum<2>(v, kp, part, counter, sh) && live) {
-        pcg_state& t = st[c];
-        t.rr = v[0];
-        t.rs = v[1];
-        t.iter += 1;
-        const double rn = sqrt(v[0]) / t.bnorm;
-        t.residual = rn;
-        bool stop = false;
-        if (rn < t.tol) { t.done = 1; t.status = 1; stop = true; }
-        if (!stop && t.stag_window > 0 && t.iter % t.stag_window == 0) {
-            if (rn > t.prev_check * 0.5) { t.done = 1; t.status = 3; stop = true; }
-            else t.prev_check = rn;
-        }
-        if (!stop && t.iter >= t.max_iter) { t.done = 1; t.status = 4; stop = true; }
-        if (!stop) next_application_k(t, v[1]);
-    }
+    col_block_partials<2>(v, kp, part, sh);
 }
 
-// r.out, sum(out) per column; centring shift mu, rz, beta (dot_center's).
+// Partials of r.out, sum(out) per column (centring shift mu, rz, beta --
+// dot_center's -- in block_finish kBFinDot).
 template <class TO>
 __global__ __launch_bounds__(kBlock) void dot_center_k(int n, int kp, const double* __restrict__ r,
-                                                       const TO* __restrict__ out, pcg_state* st, batch_state* bs,
-                                                       double* part, unsigned* counter, int first) {
+                                                       const TO* __restrict__ out, const pcg_state* st,
+                                                       const batch_state* bs, double* part) {
     if (bs->active == 0) return;
     __shared__ double sh[2 * kBlock];
     const int c = threadIdx.x & (kp - 1);
@@ -475,20 +420,7 @@ __global__ __launch_bounds__(kBlock) void dot_center_k(int n, int kp, const doub
         }
     }
     double v[2] = {d, s};
-    if (col_grid_sum<2>(v, kp, part, counter, sh) && live) {
-        pcg_state& t = st[c];
-        const double mu = t.center_now ? v[1] / static_cast<double>(n) : 0.0;
-        const double rz = v[0] - mu * t.rs;
-        t.out_sum = v[1];
-        t.mu = mu;
-        if (first || t.phase == 1) {
-            t.rz = rz;
-            t.beta = 0.0;
-        } else {
-            t.beta = rz / t.rz;
-            t.rz = rz;
-        }
-    }
+    col_block_partials<2>(v, kp, part, sh);
 }
 
 // p = (out - mu) + beta p per live column; the count of active columns is
@@ -564,11 +496,11 @@ __global__ __launch_bounds__(kBlock) void gather_cols_k(int n, int kp, const uns
     }
 }
 
-// Column sums into st[c].out_sum (the Laplacian min-norm centring of the
-// result; the centring shifts of a blocked application).
+// Partials of the column sums (block_finish kBFinOutSum: st[c].out_sum, for
+// the Laplacian min-norm centring of the result and the centring shifts of a
+// blocked application).
 template <class T>
-__global__ __launch_bounds__(kBlock) void sum_cols_k(int n, int kp, const T* __restrict__ x, pcg_state* st,
-                                                     double* part, unsigned* counter) {
+__global__ __launch_bounds__(kBlock) void sum_cols_k(int n, int kp, const T* __restrict__ x, double* part) {
     __shared__ double sh[kBlock];
     const int c = threadIdx.x & (kp - 1);
     const int rows_per_grid = gridDim.x * (kBlock / kp);
@@ -576,7 +508,94 @@ __global__ __launch_bounds__(kBlock) void sum_cols_k(int n, int kp, const T* __r
     for (int row = (blockIdx.x * kBlock + threadIdx.x) / kp; row < n; row += rows_per_grid)
         s += static_cast<double>(x[static_cast<size_t>(row) * kp + c]);
     double v[1] = {s};
-    if (col_grid_sum<1>(v, kp, part, counter, sh)) st[c].out_sum = v[0];
+    col_block_partials<1>(v, kp, part, sh);
+}
+
+// Stage 2 of the blocked reductions (one workgroup, launched right after the
+// producer of `part`; nb = the producer's grid), per column c < kp:
+//   kBFinPAp        st[c].pAp of an iterating column; breakdown ends it
+//   kBFinInit       start of each joining column (init_residual's decisions)
+//   kBFinXR         each live column's loop bookkeeping (update_xr's)
+//   kBFinDot(First) r.z, mu, beta (dot_center's; joining columns start fresh)
+//   kBFinOutSum     st[c].out_sum
+// The in-loop ops do nothing when no column is active, as their producers.
+enum : int { kBFinPAp, kBFinInit, kBFinXR, kBFinDot, kBFinDotFirst, kBFinOutSum };
+
+__global__ __launch_bounds__(kBlock) void block_finish(int op, int n, int kp, const double* __restrict__ part, int nb,
+                                                       pcg_state* st, const batch_state* bs) {
+    if ((op == kBFinPAp || op == kBFinXR || op == kBFinDot || op == kBFinDotFirst) && bs->active == 0) return;
+    __shared__ double sh[2 * kBlock];
+    double v[2] = {0.0, 0.0};
+    bool mine;
+    if (op == kBFinPAp || op == kBFinOutSum) {
+        double w[1];
+        mine = col_finish_sum<1>(w, kp, part, nb, sh);
+        v[0] = w[0];
+    } else {
+        mine = col_finish_sum<2>(v, kp, part, nb, sh);
+    }
+    if (!mine) return;
+    pcg_state& t = st[threadIdx.x];
+    switch (op) {
+        case kBFinPAp:
+            if (!t.done && t.phase == 0) {
+                t.pAp = v[0];
+                if (v[0] <= 0.0) { t.done = 1; t.status = 2; }
+            }
+            break;
+        case kBFinInit:
+            if (!t.done && t.phase == 1) {
+                t.rr = v[0];
+                t.rs = v[1];
+                t.iter = 0;
+                t.prev_check = 1.0;
+                t.apply_count = 0;
+                t.bnorm = sqrt(v[0]);
+                t.residual = 1.0;
+                if (t.bnorm == 0.0) { t.done = 1; t.status = 5; t.residual = 0.0; }
+                else if (t.max_iter <= 0) { t.done = 1; t.status = 4; }
+                else next_application_k(t, v[1]);
+            }
+            break;
+        case kBFinXR:
+            if (!t.done && t.phase == 0) {
+                t.rr = v[0];
+                t.rs = v[1];
+                t.iter += 1;
+                const double rn = sqrt(v[0]) / t.bnorm;
+                t.residual = rn;
+                bool stop = false;
+                if (rn < t.tol) { t.done = 1; t.status = 1; stop = true; }
+                if (!stop && t.stag_window > 0 && t.iter % t.stag_window == 0) {
+                    if (rn > t.prev_check * 0.5) { t.done = 1; t.status = 3; stop = true; }
+                    else t.prev_check = rn;
+                }
+                if (!stop && t.iter >= t.max_iter) { t.done = 1; t.status = 4; stop = true; }
+                if (!stop) next_application_k(t, v[1]);
+            }
+            break;
+        case kBFinDot:
+        case kBFinDotFirst:
+            if (!t.done) {
+                const double mu = t.center_now ? v[1] / static_cast<double>(n) : 0.0;
+                const double rz = v[0] - mu * t.rs;
+                t.out_sum = v[1];
+                t.mu = mu;
+                if (op == kBFinDotFirst || t.phase == 1) {
+                    t.rz = rz;
+                    t.beta = 0.0;
+                } else {
+                    t.beta = rz / t.rz;
+                    t.rz = rz;
+                }
+            }
+            break;
+        case kBFinOutSum:
+            t.out_sum = v[0];
+            break;
+        default:
+            break;
+    }
 }
 
 // Input shift of a blocked application: shift = center ? out_sum / n : 0.

@@ -41,38 +41,36 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* sh) {
         }
 }
 
-// Two-stage deterministic grid reduction: each block stores its N partials in
-// part[block * N + k]; the last block to arrive (atomic counter) sums all
-// partials in block order and returns true in thread 0 with the totals in v.
-// The counter is reset by that block, so it is zero between launches.
+// Deterministic grid reductions in two launches. Stage 1 runs in every
+// workgroup of the producing kernel: the workgroup's totals go to
+// part[block * N + k] with plain stores. Stage 2 (finish_sum) runs in a
+// separate one-workgroup launch that the stream orders after the producer;
+// the kernel boundary makes the partials visible. The producer therefore
+// needs no fence: an agent-scope fence in every workgroup writes back and
+// invalidates its XCD's L2 each time, which halved the bandwidth of the
+// streaming kernels that reduce.
 template <int N>
-__device__ __forceinline__ bool grid_sum(double (&v)[N], double* part, unsigned* counter, double* sh) {
+__device__ __forceinline__ void block_partials(double (&v)[N], double* part, double* sh) {
     block_sum<N>(v, sh);
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < N; ++k)
-            __hip_atomic_store(&part[blockIdx.x * N + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+        for (int k = 0; k < N; ++k) part[blockIdx.x * N + k] = v[k];
+}
+
+// Stage 2 over the partials of nb workgroups, in block order: strided
+// per-thread sums, then block_sum. True in thread 0, with the totals in v.
+template <int N>
+__device__ __forceinline__ bool finish_sum(double (&v)[N], const double* part, int nb, double* sh) {
     double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+    for (int b = static_cast<int>(threadIdx.x); b < nb; b += kBlock)
 #pragma unroll
-        for (int k = 0; k < N; ++k)
-            acc[k] += __hip_atomic_load(&part[b * N + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < N; ++k) acc[k] += part[b * N + k];
     block_sum<N>(acc, sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < N; ++k) v[k] = acc[k];
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     return threadIdx.x == 0;
 }
 

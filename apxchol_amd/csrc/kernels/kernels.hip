@@ -148,14 +148,19 @@ void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_s
     check_launch("fill_outputs");
 }
 
+// Stage 2 of a PCG reduction over the partials of an nb-workgroup producer.
+void finish(int op, int n, const double* part, int nb, pcg_state* st, hipStream_t s) {
+    pcg_finish<<<1, kBlock, 0, s>>>(op, n, part, nb, st);
+    check_launch("pcg_finish");
+}
+
 template <class VT>
 static void spmv_t(int n, const unsigned* ptr, const int* idx, const VT* val, int lanes, const double* x, double* y,
-                   pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
-                   const launch_hook& hook) {
+                   pcg_state* st, double* part, bool want_dot, hipStream_t s, const launch_hook& hook) {
     const int g = vec_blocks(static_cast<long long>(n) * lanes);
     const int w = want_dot ? 1 : 0;
     auto go = [&](auto kernel) {
-        launch_timed(hook, 4, kernel, dim3(g), dim3(kBlock), 0, s, n, ptr, idx, val, x, y, st, part, counter, w);
+        launch_timed(hook, 4, kernel, dim3(g), dim3(kBlock), 0, s, n, ptr, idx, val, x, y, st, part, w);
     };
     switch (lanes) {
         case 1: go(spmv_pAp<VT, 1>); break;
@@ -167,28 +172,31 @@ static void spmv_t(int n, const unsigned* ptr, const int* idx, const VT* val, in
         default: go(spmv_pAp<VT, 64>); break;
     }
     check_launch("spmv_pAp");
+    if (want_dot) finish(kFinPAp, n, part, g, st, s);
 }
 
 void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
-                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
-                 const launch_hook& hook) {
-    if (val32) spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, st, part, counter, want_dot, s, hook);
-    else spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, st, part, counter, want_dot, s, hook);
+                 double* y, pcg_state* st, double* part, bool want_dot, hipStream_t s, const launch_hook& hook) {
+    if (want_dot && !st) throw std::invalid_argument("apxchol_amd: SpMV dot product without a solve state");
+    if (val32) spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, st, part, want_dot, s, hook);
+    else spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, st, part, want_dot, s, hook);
 }
 
 void launch_spmv_sell(const sell_desc& a, bool val32, const double* x, double* y, pcg_state* st, double* part,
-                      unsigned* counter, bool want_dot, hipStream_t s, const launch_hook& hook) {
+                      bool want_dot, hipStream_t s, const launch_hook& hook) {
+    if (want_dot && !st) throw std::invalid_argument("apxchol_amd: SpMV dot product without a solve state");
     const long long items = static_cast<long long>(a.nslices) + a.nlong;
     if (items <= 0) return;
     long long g = (items + kWavesPerBlock - 1) / kWavesPerBlock;
     if (g > vec_block_cap()) g = vec_block_cap();
     const int w = want_dot ? 1 : 0;
     auto go = [&](auto kernel) {
-        launch_timed(hook, 4, kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, a, x, y, st, part, counter, w);
+        launch_timed(hook, 4, kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, a, x, y, st, part, w);
     };
     if (val32) { if (a.rowmap) go(spmv_sell<float, true>); else go(spmv_sell<float, false>); }
     else { if (a.rowmap) go(spmv_sell<double, true>); else go(spmv_sell<double, false>); }
     check_launch("spmv_sell");
+    if (want_dot) finish(kFinPAp, a.n, part, static_cast<int>(g), st, s);
 }
 
 void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,
@@ -197,33 +205,38 @@ void launch_scatter_init(int n, const unsigned* perm, const double* b, const dou
     check_launch("scatter_init");
 }
 
-void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
-    norm2_b<<<vec_blocks(n), kBlock, 0, s>>>(n, b, st, part, counter);
+void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, hipStream_t s) {
+    const int g = vec_blocks(n);
+    norm2_b<<<g, kBlock, 0, s>>>(n, b, part);
     check_launch("norm2_b");
+    finish(kFinBnorm, n, part, g, st, s);
 }
 
-void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, unsigned* counter,
-                          hipStream_t s) {
-    init_residual<<<vec_blocks(n), kBlock, 0, s>>>(n, r, ax0, st, part, counter);
+void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, hipStream_t s) {
+    const int g = vec_blocks(n);
+    init_residual<<<g, kBlock, 0, s>>>(n, r, ax0, part);
     check_launch("init_residual");
+    finish(ax0 ? kFinInitX0 : kFinInit, n, part, g, st, s);
 }
 
 void launch_update_xr(vec_prec p, int n, int m, double* x, const double* pv, double* r, const double* ap, void* y,
-                      void* z, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
+                      void* z, pcg_state* st, double* part, hipStream_t s) {
     const int g = vec_blocks(n);
     if (p == vec_prec::fp64)
-        update_xr<double><<<g, kBlock, 0, s>>>(n, m, x, pv, r, ap, static_cast<double*>(y), static_cast<double*>(z), st, part, counter);
+        update_xr<double><<<g, kBlock, 0, s>>>(n, m, x, pv, r, ap, static_cast<double*>(y), static_cast<double*>(z), st, part);
     else
-        update_xr<float><<<g, kBlock, 0, s>>>(n, m, x, pv, r, ap, static_cast<float*>(y), static_cast<float*>(z), st, part, counter);
+        update_xr<float><<<g, kBlock, 0, s>>>(n, m, x, pv, r, ap, static_cast<float*>(y), static_cast<float*>(z), st, part);
     check_launch("update_xr");
+    finish(kFinXR, n, part, g, st, s);
 }
 
-void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part,
-                       unsigned* counter, bool first, hipStream_t s) {
+void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part, bool first,
+                       hipStream_t s) {
     const int g = vec_blocks(n);
-    if (p == vec_prec::fp64) dot_center<double><<<g, kBlock, 0, s>>>(n, r, static_cast<const double*>(out), st, part, counter, first ? 1 : 0);
-    else dot_center<float><<<g, kBlock, 0, s>>>(n, r, static_cast<const float*>(out), st, part, counter, first ? 1 : 0);
+    if (p == vec_prec::fp64) dot_center<double><<<g, kBlock, 0, s>>>(n, r, static_cast<const double*>(out), st, part);
+    else dot_center<float><<<g, kBlock, 0, s>>>(n, r, static_cast<const float*>(out), st, part);
     check_launch("dot_center");
+    finish(first ? kFinDotFirst : kFinDot, n, part, g, st, s);
 }
 
 void launch_update_p(vec_prec p, int n, double* pv, const void* out, const pcg_state* st, bool first, hipStream_t s) {
@@ -233,9 +246,11 @@ void launch_update_p(vec_prec p, int n, double* pv, const void* out, const pcg_s
     check_launch("update_p");
 }
 
-void launch_sum_vec(int n, const double* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
-    sum_vec<<<vec_blocks(n), kBlock, 0, s>>>(n, a, st, part, counter);
+void launch_sum_vec(int n, const double* a, pcg_state* st, double* part, hipStream_t s) {
+    const int g = vec_blocks(n);
+    sum_vec<<<g, kBlock, 0, s>>>(n, a, part);
     check_launch("sum_vec");
+    finish(kFinOutSum, n, part, g, st, s);
 }
 
 void launch_gather_shift(int n, const unsigned* perm, const double* src, double* dst, const pcg_state* st,
@@ -244,10 +259,14 @@ void launch_gather_shift(int n, const unsigned* perm, const double* src, double*
     check_launch("gather_shift");
 }
 
-void launch_sum_shift(int n, const double* r, pcg_state* st, double* part, unsigned* counter, bool center,
-                      hipStream_t s) {
-    sum_shift<<<vec_blocks(n), kBlock, 0, s>>>(n, r, st, part, counter, center ? 1 : 0);
-    check_launch("sum_shift");
+void launch_sum_shift(int n, const double* r, pcg_state* st, double* part, bool center, hipStream_t s) {
+    // No centring: shift = 0, the finish of an empty reduction.
+    const int g = center ? vec_blocks(n) : 0;
+    if (center) {
+        sum_shift<<<g, kBlock, 0, s>>>(n, r, part);
+        check_launch("sum_shift");
+    }
+    finish(kFinShift, n, part, g, st, s);
 }
 
 void launch_apply_scatter(vec_prec p, int n, int m, const unsigned* perm, const double* r, double* rp, void* y,
@@ -258,11 +277,12 @@ void launch_apply_scatter(vec_prec p, int n, int m, const unsigned* perm, const 
     check_launch("apply_scatter");
 }
 
-void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s) {
+void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* part, hipStream_t s) {
     const int g = vec_blocks(n);
-    if (p == vec_prec::fp64) sum_out<double><<<g, kBlock, 0, s>>>(n, static_cast<const double*>(a), st, part, counter);
-    else sum_out<float><<<g, kBlock, 0, s>>>(n, static_cast<const float*>(a), st, part, counter);
+    if (p == vec_prec::fp64) sum_out<double><<<g, kBlock, 0, s>>>(n, static_cast<const double*>(a), part);
+    else sum_out<float><<<g, kBlock, 0, s>>>(n, static_cast<const float*>(a), part);
     check_launch("sum_out");
+    finish(kFinOutSum, n, part, g, st, s);
 }
 
 void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* out, double* z, const pcg_state* st,
@@ -327,6 +347,12 @@ void check_kp(int kp) {
     if (kp < 1 || kp > 64 || (kp & (kp - 1)) != 0) throw std::invalid_argument("apxchol_amd: block width must be a power of two <= 64");
 }
 
+// Stage 2 of a blocked reduction over the partials of an nb-workgroup producer.
+void finish_k(int op, int n, int kp, const double* part, int nb, pcg_state* st, const batch_state* bs, hipStream_t s) {
+    block_finish<<<1, kBlock, 0, s>>>(op, n, kp, part, nb, st, bs);
+    check_launch("block_finish");
+}
+
 }  // namespace
 
 int sweep_block_tail_max_rows(vec_prec p, int kp) {
@@ -349,18 +375,19 @@ void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sche
 }
 
 void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx, const void* val, bool val32,
-                   const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
-                   bool want_dot, hipStream_t s) {
+                   const double* x, double* y, pcg_state* st, batch_state* bs, double* part, bool want_dot,
+                   hipStream_t s) {
     check_kp(kp);
     if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) throw std::invalid_argument("apxchol_amd: SpMM lanes");
     const int g = spmm_grid(n, kp, lanes);
     if (val32)
-        spmm_pAp_k<float><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const float*>(val), x, y, st, bs,
-                                               part, counter, want_dot ? 1 : 0);
+        spmm_pAp_k<float><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const float*>(val), x, y, bs, part,
+                                               want_dot ? 1 : 0);
     else
-        spmm_pAp_k<double><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const double*>(val), x, y, st,
-                                                bs, part, counter, want_dot ? 1 : 0);
+        spmm_pAp_k<double><<<g, kBlock, 0, s>>>(n, kp, lanes, ptr, idx, static_cast<const double*>(val), x, y, bs,
+                                                part, want_dot ? 1 : 0);
     check_launch("spmm_pAp_k");
+    if (want_dot) finish_k(kBFinPAp, n, kp, part, g, st, bs, s);
 }
 
 void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const double* B, double* r, double* x,
@@ -370,15 +397,19 @@ void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const dou
 }
 
 void launch_init_residual_k(int n, int kp, const double* r, pcg_state* st, batch_state* bs, double* part,
-                            unsigned* counter, hipStream_t s) {
-    init_residual_k<<<block_grid(n, kp), kBlock, 0, s>>>(n, kp, r, st, bs, part, counter);
+                            hipStream_t s) {
+    const int g = block_grid(n, kp);
+    init_residual_k<<<g, kBlock, 0, s>>>(n, kp, r, part);
     check_launch("init_residual_k");
+    finish_k(kBFinInit, n, kp, part, g, st, bs, s);
 }
 
 void launch_update_xr_k(int n, int kp, double* x, const double* p, double* r, const double* ap, pcg_state* st,
-                        batch_state* bs, double* part, unsigned* counter, hipStream_t s) {
-    update_xr_k<<<block_grid(n, kp), kBlock, 0, s>>>(n, kp, x, p, r, ap, st, bs, part, counter);
+                        batch_state* bs, double* part, hipStream_t s) {
+    const int g = block_grid(n, kp);
+    update_xr_k<<<g, kBlock, 0, s>>>(n, kp, x, p, r, ap, st, bs, part);
     check_launch("update_xr_k");
+    finish_k(kBFinXR, n, kp, part, g, st, bs, s);
 }
 
 void launch_refresh_active_k(int kp, const pcg_state* st, batch_state* bs, hipStream_t s) {
@@ -387,15 +418,14 @@ void launch_refresh_active_k(int kp, const pcg_state* st, batch_state* bs, hipSt
 }
 
 void launch_dot_center_k(vec_prec p, int n, int kp, const double* r, const void* out, pcg_state* st, batch_state* bs,
-                         double* part, unsigned* counter, bool first, hipStream_t s) {
+                         double* part, bool first, hipStream_t s) {
     const int g = block_grid(n, kp);
     if (p == vec_prec::fp64)
-        dot_center_k<double><<<g, kBlock, 0, s>>>(n, kp, r, static_cast<const double*>(out), st, bs, part, counter,
-                                                  first ? 1 : 0);
+        dot_center_k<double><<<g, kBlock, 0, s>>>(n, kp, r, static_cast<const double*>(out), st, bs, part);
     else
-        dot_center_k<float><<<g, kBlock, 0, s>>>(n, kp, r, static_cast<const float*>(out), st, bs, part, counter,
-                                                 first ? 1 : 0);
+        dot_center_k<float><<<g, kBlock, 0, s>>>(n, kp, r, static_cast<const float*>(out), st, bs, part);
     check_launch("dot_center_k");
+    finish_k(first ? kBFinDotFirst : kBFinDot, n, kp, part, g, st, bs, s);
 }
 
 void launch_update_p_k(vec_prec p, int n, int kp, double* pv, const void* out, const pcg_state* st, batch_state* bs,
@@ -416,12 +446,12 @@ void launch_zero_tail_k(vec_prec p, int n, int m, int kp, void* a, hipStream_t s
     check_launch("zero_tail_k");
 }
 
-void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, unsigned* counter,
-                       hipStream_t s) {
+void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, hipStream_t s) {
     const int g = block_grid(n, kp);
-    if (p == vec_prec::fp64) sum_cols_k<double><<<g, kBlock, 0, s>>>(n, kp, static_cast<const double*>(x), st, part, counter);
-    else sum_cols_k<float><<<g, kBlock, 0, s>>>(n, kp, static_cast<const float*>(x), st, part, counter);
+    if (p == vec_prec::fp64) sum_cols_k<double><<<g, kBlock, 0, s>>>(n, kp, static_cast<const double*>(x), part);
+    else sum_cols_k<float><<<g, kBlock, 0, s>>>(n, kp, static_cast<const float*>(x), part);
     check_launch("sum_cols_k");
+    finish_k(kBFinOutSum, n, kp, part, g, st, nullptr, s);
 }
 
 void launch_permute(int m, const unsigned* Q, const double* src, double* dst, bool in, hipStream_t s) {

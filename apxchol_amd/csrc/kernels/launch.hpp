@@ -62,33 +62,31 @@ void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, v
 void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s);
 
 void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
-                 double* y, pcg_state* st, double* part, unsigned* counter, bool want_dot, hipStream_t s,
+                 double* y, pcg_state* st, double* part, bool want_dot, hipStream_t s,
                  const launch_hook& hook = {});
 
 // y = A x (and x.y into st->pAp when want_dot) over the sliced-ELL form.
 void launch_spmv_sell(const sell_desc& a, bool val32, const double* x, double* y, pcg_state* st, double* part,
-                      unsigned* counter, bool want_dot, hipStream_t s, const launch_hook& hook = {});
+                      bool want_dot, hipStream_t s, const launch_hook& hook = {});
 
 void launch_scatter_init(int n, const unsigned* perm, const double* b, const double* x0, double* r, double* x,
                          hipStream_t s);
-void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
-void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, unsigned* counter,
-                          hipStream_t s);
+void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, hipStream_t s);
+void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, hipStream_t s);
 void launch_update_xr(vec_prec p, int n, int m, double* x, const double* pv, double* r, const double* ap, void* y,
-                      void* z, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
-void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part,
-                       unsigned* counter, bool first, hipStream_t s);
+                      void* z, pcg_state* st, double* part, hipStream_t s);
+void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part, bool first, hipStream_t s);
 void launch_update_p(vec_prec p, int n, double* pv, const void* out, const pcg_state* st, bool first, hipStream_t s);
-void launch_sum_vec(int n, const double* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
+void launch_sum_vec(int n, const double* a, pcg_state* st, double* part, hipStream_t s);
 void launch_gather_shift(int n, const unsigned* perm, const double* src, double* dst, const pcg_state* st,
                          bool use_mean, hipStream_t s);
 
 // Stand-alone application pieces.
-void launch_sum_shift(int n, const double* r, pcg_state* st, double* part, unsigned* counter, bool center,
+void launch_sum_shift(int n, const double* r, pcg_state* st, double* part, bool center,
                       hipStream_t s);
 void launch_apply_scatter(vec_prec p, int n, int m, const unsigned* perm, const double* r, double* rp, void* y,
                           void* z, hipStream_t s);
-void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* part, unsigned* counter, hipStream_t s);
+void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* part, hipStream_t s);
 void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* out, double* z, const pcg_state* st,
                          bool center, hipStream_t s);
 void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s);
@@ -120,23 +118,20 @@ void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sche
 // Rows a blocked tail run of block width kp may hold (LDS).
 int sweep_block_tail_max_rows(vec_prec p, int kp);
 void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx, const void* val, bool val32,
-                   const double* x, double* y, pcg_state* st, batch_state* bs, double* part, unsigned* counter,
-                   bool want_dot, hipStream_t s);
+                   const double* x, double* y, pcg_state* st, batch_state* bs, double* part, bool want_dot, hipStream_t s);
 void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const double* B, double* r, double* x,
                            hipStream_t s);
-void launch_init_residual_k(int n, int kp, const double* r, pcg_state* st, batch_state* bs, double* part,
-                            unsigned* counter, hipStream_t s);
+void launch_init_residual_k(int n, int kp, const double* r, pcg_state* st, batch_state* bs, double* part, hipStream_t s);
 void launch_update_xr_k(int n, int kp, double* x, const double* p, double* r, const double* ap, pcg_state* st,
-                        batch_state* bs, double* part, unsigned* counter, hipStream_t s);
+                        batch_state* bs, double* part, hipStream_t s);
 void launch_refresh_active_k(int kp, const pcg_state* st, batch_state* bs, hipStream_t s);
 void launch_dot_center_k(vec_prec p, int n, int kp, const double* r, const void* out, pcg_state* st, batch_state* bs,
-                         double* part, unsigned* counter, bool first, hipStream_t s);
+                         double* part, bool first, hipStream_t s);
 void launch_update_p_k(vec_prec p, int n, int kp, double* pv, const void* out, const pcg_state* st, batch_state* bs,
                        bool first, hipStream_t s);
 void launch_zero_tail_k(vec_prec p, int n, int m, int kp, void* a, hipStream_t s);
 // x: fp64 (p == fp64) or fp32 block.
-void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, unsigned* counter,
-                       hipStream_t s);
+void launch_sum_cols_k(vec_prec p, int n, int kp, const void* x, pcg_state* st, double* part, hipStream_t s);
 void launch_shift_from_sum_k(int kp, pcg_state* st, bool center, hipStream_t s);
 void launch_end_iteration_k(int kp, pcg_state* st, batch_state* bs, hipStream_t s);
 // cols, srcs / dsts: device arrays of ncol entries.

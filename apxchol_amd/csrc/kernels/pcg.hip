@@ -1,8 +1,9 @@
 // GPU-resident PCG kernels (gfx950). Vectors live in the factor's permuted
-// space; every reduction is two-stage and fixed-order (grid_sum), and every
-// scalar the recurrence needs stays on the device in pcg_state. Each kernel
-// returns at once when the solve has finished (st->done), so the host can
-// enqueue (or graph-replay) several iterations and test completion rarely.
+// space; every reduction is two-stage and fixed-order (block_partials in the
+// producer, pcg_finish after it), and every scalar the recurrence needs stays
+// on the device in pcg_state. Each kernel returns at once when the solve has
+// finished (st->done), so the host can enqueue (or graph-replay) several
+// iterations and test completion rarely.
 //
 // Reference counterparts: src/cuda_pcg_kernels.cu:18-152 (spmv_pAp,
 // update_xr, dot, update_p, reduce_partials) driven by cuda_pcg::solve
@@ -14,13 +15,14 @@
 namespace apxchol_amd {
 
 // y = A x over the permuted full-symmetric CSR, G lanes per row (lane-strided
-// entries + xor butterfly). With want_dot, x.y is reduced into st->pAp and a
-// non-positive pAp ends the solve (solve.cpp:596, breakdown).
+// entries + xor butterfly). With want_dot, the partials of x.y go to part;
+// pcg_finish (kFinPAp) sets st->pAp and ends the solve on a non-positive pAp
+// (solve.cpp:596, breakdown).
 template <class VT, int G>
 __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __restrict__ ptr,
                                                    const int* __restrict__ idx, const VT* __restrict__ val,
                                                    const double* __restrict__ x, double* __restrict__ y,
-                                                   pcg_state* st, double* part, unsigned* counter, int want_dot) {
+                                                   const pcg_state* st, double* part, int want_dot) {
     if (st && st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     const int tid = blockIdx.x * kBlock + threadIdx.x;
@@ -40,10 +42,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    if (grid_sum<1>(v, part, counter, sh)) {
-        st->pAp = v[0];
-        if (v[0] <= 0.0) { st->done = 1; st->status = 2; }
-    }
+    block_partials<1>(v, part, sh);
 }
 
 // y = A x over the sliced-ELL form (sell_desc): one lane per row, a slice's
@@ -51,10 +50,10 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
 // columns' loads and gathers in flight per lane; rows too long for a slice are
 // walked by a whole wave from the CSR arrays (lane-strided + butterfly). A
 // row's sum is its entries in CSR order (padding adds fma(0, x[row], s) = s).
-// With want_dot, x.y lands in st->pAp as in spmv_pAp.
+// With want_dot, the partials of x.y go to part as in spmv_pAp.
 template <class VT, bool MAP>
 __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* __restrict__ x, double* __restrict__ y,
-                                                    pcg_state* st, double* part, unsigned* counter, int want_dot) {
+                                                    const pcg_state* st, double* part, int want_dot) {
     if (st && st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     const int lane = threadIdx.x & (kWave - 1);
@@ -71,24 +70,23 @@ __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* _
             const int w = static_cast<int>(c1 - c0);
             const size_t base = static_cast<size_t>(c0) * kWave + lane;
             const int row = MAP ? a.rowmap[static_cast<size_t>(it) * kWave + lane] : it * kWave + lane;
+            // Columns in groups of eight (the last group predicated): all of a
+            // group's loads, then its gathers, are in flight together.
             double s = 0.0;
-            int k = 0;
-            for (; k + 8 <= w; k += 8) {
+            for (int k = 0; k < w; k += 8) {
                 int j[8];
                 double v[8], g[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    j[u] = sidx[base + static_cast<size_t>(k + u) * kWave];
-                    v[u] = static_cast<double>(sval[base + static_cast<size_t>(k + u) * kWave]);
+                    const bool ok = k + u < w;
+                    j[u] = ok ? sidx[base + static_cast<size_t>(k + u) * kWave] : 0;
+                    v[u] = ok ? static_cast<double>(sval[base + static_cast<size_t>(k + u) * kWave]) : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) g[u] = x[j[u]];
+                for (int u = 0; u < 8; ++u) g[u] = k + u < w ? x[j[u]] : 0.0;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) s = fma(v[u], g[u], s);
-            }
-            for (; k < w; ++k) {
-                const size_t q = base + static_cast<size_t>(k) * kWave;
-                s = fma(static_cast<double>(sval[q]), x[sidx[q]], s);
+                for (int u = 0; u < 8; ++u)
+                    if (k + u < w) s = fma(v[u], g[u], s);
             }
             if (row >= 0 && row < a.n) {
                 y[row] = s;
@@ -109,10 +107,7 @@ __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* _
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    if (grid_sum<1>(v, part, counter, sh)) {
-        st->pAp = v[0];
-        if (v[0] <= 0.0) { st->done = 1; st->status = 2; }
-    }
+    block_partials<1>(v, part, sh);
 }
 
 // r[perm[v]] = b[v]; x[perm[v]] = x0[v] (or 0).
@@ -127,13 +122,12 @@ __global__ __launch_bounds__(kBlock) void scatter_init(int n, const unsigned* __
 }
 
 // Sum of squares of b (original order), for the warm-start ||b||.
-__global__ __launch_bounds__(kBlock) void norm2_b(int n, const double* __restrict__ b, pcg_state* st, double* part,
-                                                  unsigned* counter) {
+__global__ __launch_bounds__(kBlock) void norm2_b(int n, const double* __restrict__ b, double* part) {
     __shared__ double sh[kWavesPerBlock * 2];
     double q = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) q = fma(b[i], b[i], q);
     double v[1] = {q};
-    if (grid_sum<1>(v, part, counter, sh)) st->bnorm = sqrt(v[0]);
+    block_partials<1>(v, part, sh);
 }
 
 // Decide whether the NEXT preconditioner application centres (the center-k
@@ -144,10 +138,10 @@ __device__ __forceinline__ void next_application(pcg_state* st, double rs) {
     st->shift = st->center_now ? rs / static_cast<double>(st->n) : 0.0;
 }
 
-// r = r - Ax0 (Ax0 may be null); rr = r.r, rs = sum r; decides the start of
-// the solve (solve.cpp:529-566).
+// r = r - Ax0 (Ax0 may be null); partials of rr = r.r, rs = sum r (the start
+// of the solve, solve.cpp:529-566, is decided by kFinInit / kFinInitX0).
 __global__ __launch_bounds__(kBlock) void init_residual(int n, double* __restrict__ r, const double* __restrict__ ax0,
-                                                        pcg_state* st, double* part, unsigned* counter) {
+                                                        double* part) {
     __shared__ double sh[kWavesPerBlock * 2];
     double q = 0.0, s = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -157,35 +151,17 @@ __global__ __launch_bounds__(kBlock) void init_residual(int n, double* __restric
         s += ri;
     }
     double v[2] = {q, s};
-    if (grid_sum<2>(v, part, counter, sh)) {
-        st->rr = v[0];
-        st->rs = v[1];
-        st->iter = 0;
-        st->prev_check = 1.0;
-        st->apply_count = 0;
-        if (!ax0) {
-            st->bnorm = sqrt(v[0]);
-            st->residual = 1.0;
-            if (st->bnorm == 0.0) { st->done = 1; st->status = 5; st->residual = 0.0; return; }
-        } else {
-            if (st->bnorm == 0.0) { st->done = 1; st->status = 5; st->residual = 0.0; return; }
-            st->residual = sqrt(v[0]) / st->bnorm;
-            if (st->residual < st->tol) { st->done = 1; st->status = 1; return; }
-        }
-        if (st->max_iter <= 0) { st->done = 1; st->status = 4; return; }
-        next_application(st, v[1]);
-    }
+    block_partials<2>(v, part, sh);
 }
 
-// x += alpha p; r -= alpha Ap; rr, rs of the new r (solve.cpp:169-208), plus
-// the sentinel fill of the next application's sweep outputs. The last block
-// runs the loop's bookkeeping: iteration count, convergence, stagnation
-// (solve.cpp:604-615), max_iter, and the next application's centring.
+// x += alpha p; r -= alpha Ap; partials of rr, rs of the new r
+// (solve.cpp:169-208), plus the sentinel fill of the next application's sweep
+// outputs. The loop's bookkeeping follows in pcg_finish (kFinXR).
 template <class TO>
 __global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __restrict__ x, const double* __restrict__ p,
                                                     double* __restrict__ r, const double* __restrict__ ap,
-                                                    TO* __restrict__ y, TO* __restrict__ z, pcg_state* st,
-                                                    double* part, unsigned* counter) {
+                                                    TO* __restrict__ y, TO* __restrict__ z, const pcg_state* st,
+                                                    double* part) {
     if (st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     const double alpha = st->rz / st->pAp;
@@ -203,20 +179,7 @@ __global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __rest
         else z[i] = static_cast<TO>(0);
     }
     double v[2] = {q, s};
-    if (grid_sum<2>(v, part, counter, sh)) {
-        st->rr = v[0];
-        st->rs = v[1];
-        st->iter += 1;
-        const double rn = sqrt(v[0]) / st->bnorm;
-        st->residual = rn;
-        if (rn < st->tol) { st->done = 1; st->status = 1; return; }
-        if (st->stag_window > 0 && st->iter % st->stag_window == 0) {
-            if (rn > st->prev_check * 0.5) { st->done = 1; st->status = 3; return; }
-            st->prev_check = rn;
-        }
-        if (st->iter >= st->max_iter) { st->done = 1; st->status = 4; return; }
-        next_application(st, v[1]);
-    }
+    block_partials<2>(v, part, sh);
 }
 
 // Sentinel fill for the first application of a solve (update_xr does it for
@@ -234,12 +197,12 @@ __global__ __launch_bounds__(kBlock) void fill_outputs(int n, int m, TO* __restr
     }
 }
 
-// r.out and sum(out) of the application's output; the output centring
-// shift mu = mean(out) on centring applications (preconditioner.h:516-525);
-// r.z = r.out - mu sum(r); beta = r.z / previous r.z.
+// Partials of r.out and sum(out) of the application's output; kFinDot then
+// sets the output centring shift mu = mean(out) on centring applications
+// (preconditioner.h:516-525), r.z = r.out - mu sum(r), beta = r.z / previous r.z.
 template <class TO>
 __global__ __launch_bounds__(kBlock) void dot_center(int n, const double* __restrict__ r, const TO* __restrict__ out,
-                                                     pcg_state* st, double* part, unsigned* counter, int first) {
+                                                     const pcg_state* st, double* part) {
     if (st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     double d = 0.0, s = 0.0;
@@ -249,19 +212,7 @@ __global__ __launch_bounds__(kBlock) void dot_center(int n, const double* __rest
         s += o;
     }
     double v[2] = {d, s};
-    if (grid_sum<2>(v, part, counter, sh)) {
-        const double mu = st->center_now ? v[1] / static_cast<double>(n) : 0.0;
-        const double rz = v[0] - mu * st->rs;
-        st->out_sum = v[1];
-        st->mu = mu;
-        if (first) {
-            st->rz = rz;
-            st->beta = 0.0;
-        } else {
-            st->beta = rz / st->rz;
-            st->rz = rz;
-        }
-    }
+    block_partials<2>(v, part, sh);
 }
 
 // p = (out - mu) + beta p  (first application: p = out - mu).
@@ -278,13 +229,12 @@ __global__ __launch_bounds__(kBlock) void update_p(int n, double* __restrict__ p
 
 // Final min-norm centring (Laplacian, solve.cpp:262-269) folded into the
 // gather back to the caller's order: x_out[v] = x[perm[v]] - mean(x).
-__global__ __launch_bounds__(kBlock) void sum_vec(int n, const double* __restrict__ a, pcg_state* st, double* part,
-                                                  unsigned* counter) {
+__global__ __launch_bounds__(kBlock) void sum_vec(int n, const double* __restrict__ a, double* part) {
     __shared__ double sh[kWavesPerBlock * 2];
     double s = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += a[i];
     double v[1] = {s};
-    if (grid_sum<1>(v, part, counter, sh)) st->out_sum = v[0];
+    block_partials<1>(v, part, sh);
 }
 
 __global__ __launch_bounds__(kBlock) void gather_shift(int n, const unsigned* __restrict__ perm,
@@ -311,24 +261,23 @@ __global__ __launch_bounds__(kBlock) void apply_scatter(int n, int m, const unsi
     }
 }
 
-// Input shift of a stand-alone application: shift = center ? sum(r)/n : 0.
-__global__ __launch_bounds__(kBlock) void sum_shift(int n, const double* __restrict__ r, pcg_state* st, double* part,
-                                                    unsigned* counter, int center) {
+// Partials of sum(r), for the input shift of a stand-alone centring
+// application (kFinShift: shift = sum(r)/n).
+__global__ __launch_bounds__(kBlock) void sum_shift(int n, const double* __restrict__ r, double* part) {
     __shared__ double sh[kWavesPerBlock * 2];
     double s = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += r[i];
     double v[1] = {s};
-    if (grid_sum<1>(v, part, counter, sh)) st->shift = center ? v[0] / static_cast<double>(n) : 0.0;
+    block_partials<1>(v, part, sh);
 }
 
 template <class TO>
-__global__ __launch_bounds__(kBlock) void sum_out(int n, const TO* __restrict__ a, pcg_state* st, double* part,
-                                                  unsigned* counter) {
+__global__ __launch_bounds__(kBlock) void sum_out(int n, const TO* __restrict__ a, double* part) {
     __shared__ double sh[kWavesPerBlock * 2];
     double s = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += static_cast<double>(a[i]);
     double v[1] = {s};
-    if (grid_sum<1>(v, part, counter, sh)) st->out_sum = v[0];
+    block_partials<1>(v, part, sh);
 }
 
 template <class TO>
@@ -338,6 +287,104 @@ __global__ __launch_bounds__(kBlock) void apply_gather(int n, const unsigned* __
     const double mu = center ? st->out_sum / static_cast<double>(n) : 0.0;
     for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock)
         zdst[v] = static_cast<double>(out[perm[v]]) - mu;
+}
+
+// Stage 2 of the PCG reductions (one workgroup, launched right after the
+// producer of `part`; nb = the producer's grid) and the bookkeeping that
+// depends on the totals:
+//   kFinPAp        st->pAp; a non-positive pAp ends the solve (breakdown)
+//   kFinBnorm      st->bnorm = ||b|| (warm start)
+//   kFinInit(X0)   rr, rs of the initial residual and the start decisions
+//                  (solve.cpp:529-566): zero b, warm-start convergence, max_iter
+//   kFinXR         rr, rs; iteration count, convergence, stagnation
+//                  (solve.cpp:604-615), max_iter, the next application's centring
+//   kFinDot(First) r.z, mu, beta of the application's output
+//   kFinOutSum     st->out_sum (min-norm centring, application output shift)
+//   kFinShift      st->shift = sum / n (input shift of a centring application)
+// The in-loop ops (pAp, XR, Dot) do nothing once the solve has finished, as
+// their producers do not run then.
+enum : int { kFinPAp, kFinBnorm, kFinInit, kFinInitX0, kFinXR, kFinDot, kFinDotFirst, kFinOutSum, kFinShift };
+
+__global__ __launch_bounds__(kBlock) void pcg_finish(int op, int n, const double* __restrict__ part, int nb,
+                                                     pcg_state* st) {
+    if ((op == kFinPAp || op == kFinXR || op == kFinDot || op == kFinDotFirst) && st->done) return;
+    __shared__ double sh[kWavesPerBlock * 2];
+    double v[2] = {0.0, 0.0};
+    bool t0;
+    if (op == kFinInit || op == kFinInitX0 || op == kFinXR || op == kFinDot || op == kFinDotFirst) {
+        t0 = finish_sum<2>(v, part, nb, sh);
+    } else {
+        double w[1];
+        t0 = finish_sum<1>(w, part, nb, sh);
+        v[0] = w[0];
+    }
+    if (!t0) return;
+    switch (op) {
+        case kFinPAp:
+            st->pAp = v[0];
+            if (v[0] <= 0.0) { st->done = 1; st->status = 2; }
+            break;
+        case kFinBnorm:
+            st->bnorm = sqrt(v[0]);
+            break;
+        case kFinInit:
+        case kFinInitX0:
+            st->rr = v[0];
+            st->rs = v[1];
+            st->iter = 0;
+            st->prev_check = 1.0;
+            st->apply_count = 0;
+            if (op == kFinInit) {
+                st->bnorm = sqrt(v[0]);
+                st->residual = 1.0;
+                if (st->bnorm == 0.0) { st->done = 1; st->status = 5; st->residual = 0.0; return; }
+            } else {
+                if (st->bnorm == 0.0) { st->done = 1; st->status = 5; st->residual = 0.0; return; }
+                st->residual = sqrt(v[0]) / st->bnorm;
+                if (st->residual < st->tol) { st->done = 1; st->status = 1; return; }
+            }
+            if (st->max_iter <= 0) { st->done = 1; st->status = 4; return; }
+            next_application(st, v[1]);
+            break;
+        case kFinXR: {
+            st->rr = v[0];
+            st->rs = v[1];
+            st->iter += 1;
+            const double rn = sqrt(v[0]) / st->bnorm;
+            st->residual = rn;
+            if (rn < st->tol) { st->done = 1; st->status = 1; return; }
+            if (st->stag_window > 0 && st->iter % st->stag_window == 0) {
+                if (rn > st->prev_check * 0.5) { st->done = 1; st->status = 3; return; }
+                st->prev_check = rn;
+            }
+            if (st->iter >= st->max_iter) { st->done = 1; st->status = 4; return; }
+            next_application(st, v[1]);
+            break;
+        }
+        case kFinDot:
+        case kFinDotFirst: {
+            const double mu = st->center_now ? v[1] / static_cast<double>(n) : 0.0;
+            const double rz = v[0] - mu * st->rs;
+            st->out_sum = v[1];
+            st->mu = mu;
+            if (op == kFinDotFirst) {
+                st->rz = rz;
+                st->beta = 0.0;
+            } else {
+                st->beta = rz / st->rz;
+                st->rz = rz;
+            }
+            break;
+        }
+        case kFinOutSum:
+            st->out_sum = v[0];
+            break;
+        case kFinShift:
+            st->shift = v[0] / static_cast<double>(n);
+            break;
+        default:
+            break;
+    }
 }
 
 // Between factor order and storage order: dst[Q[q]] = src[q] (in) or

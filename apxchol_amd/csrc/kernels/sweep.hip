@@ -482,8 +482,22 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
     part = tail_pair_fma(it.e3, lds, part);
     if (__builtin_expect(c > 2 * kTailPairs, 0)) {
         // A row longer than 64 * 2 * kTailPairs: the lane's further entries.
+        // Eight entries' loads are issued together, then consumed in chain
+        // order (the same fmas as one at a time).
         const size_t o = static_cast<size_t>(td.ovf[t]);
-        for (int s = 0; s < c - 2 * kTailPairs; ++s) {
+        const int nov = c - 2 * kTailPairs;
+        int s = 0;
+        for (; s + 8 <= nov; s += 8) {
+            entry32 e[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) e[u] = td.ovf_ent[(o + static_cast<size_t>(s + u)) * kWave + lane];
+            TO g[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) g[u] = lds[e[u].idx];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) part = fma(static_cast<TO>(e[u].val), g[u], part);
+        }
+        for (; s < nov; ++s) {
             const entry32 e = td.ovf_ent[(o + static_cast<size_t>(s)) * kWave + lane];
             part = fma(static_cast<TO>(e.val), lds[e.idx], part);
         }
