@@ -93,6 +93,7 @@ _SIGS = {
     "apx_solver_reset_apply_count": ([P], ctypes.c_int),
     "apx_solver_sweep": ([P, I32, P, P, P], ctypes.c_int),
     "apx_solver_spmv": ([P, P, P, P], ctypes.c_int),
+    "apx_solver_spmv_batch": ([P, I32, P, P, P], ctypes.c_int),
     "apx_solver_export_factor": ([P, P, P, P, P], ctypes.c_int),
     "apx_solver_set_timing": ([P, I32], ctypes.c_int),
     "apx_solver_take_timing": ([P, P, P], ctypes.c_int),
@@ -400,6 +401,16 @@ class Solver:
             out = torch.empty_like(x)
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _check(load_library().apx_solver_spmv(self._h, _ptr(x), _ptr(out), s))
+        return out
+
+    def spmv_batch(self, X, out=None, stream=None):
+        """Y = A X for the k rows of X (torch float64 CUDA tensor k x n) with the blocked SpMM."""
+        import torch
+        X = X.contiguous()
+        if out is None:
+            out = torch.empty_like(X)
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(load_library().apx_solver_spmv_batch(self._h, int(X.shape[0]), _ptr(X), _ptr(out), s))
         return out
 
     def sweep(self, rhs, out, backward: bool, stream=None):

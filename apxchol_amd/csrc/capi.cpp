@@ -437,6 +437,15 @@ apx_status apx_solver_spmv(apx_solver_t s, const double* d_x, double* d_y, void*
     });
 }
 
+apx_status apx_solver_spmv_batch(apx_solver_t s, int32_t k, const double* d_X, double* d_Y, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        need(d_X, "X");
+        need(d_Y, "Y");
+        s->s->spmv_batch_device(k, d_X, d_Y, static_cast<hipStream_t>(stream));
+    });
+}
+
 apx_status apx_solver_set_timing(apx_solver_t s, int32_t on) {
     return guarded([&] {
         need(s, "solver");

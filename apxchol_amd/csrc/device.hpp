@@ -57,6 +57,7 @@ struct sell_desc {
     int n = 0;
     int nslices = 0;
     int nlong = 0;
+    int max_blocks = 2048;                  // workgroup cap of the launch (partials buffer size)
     const std::uint32_t* soff = nullptr;
     const std::int32_t* idx = nullptr;
     const void* val = nullptr;              // fp64, or fp32 when every value is fp32-exact

@@ -64,6 +64,10 @@ int lanes_for_host(std::uint32_t len) {
 
 // Workgroup cap of the vector kernels (their partial buffers are sized by it).
 constexpr int kMaxBlocksHost = 2048;
+// Workgroup cap of the sliced-ELL SpMV (the partials buffer holds this many),
+// and the XCD count its grid is a multiple of.
+constexpr int kSellMaxBlocks = 8192;
+constexpr int kXcdsHost = 8;
 
 int env_int(const char* name, int dflt) {
     const char* e = std::getenv(name);
@@ -700,29 +704,36 @@ void gpu_solver::setup(const csc_view& L) {
     int lanes = 1;
     while (lanes < 64 && lanes * 1.5 < avg) lanes <<= 1;
     d.lanes = lanes;
-    // Sliced-ELL form of the operator for the single SpMV: without a row map
-    // when 64-row slices in row order pad to <= 1.2 nnz, else sorted by length
-    // inside 4096-row windows (rows longer than max(64, 8 mean) walked by a
-    // wave) when that pads to <= 1.5 nnz; otherwise CSR lanes only.
-    // APXCHOL_AMD_SPMV=csr / sell forces the choice.
+    // Sliced-ELL form of the operator for the single SpMV. Slices take the
+    // rows in original vertex order (a row map), so a band of slices -- one
+    // XCD's share, see spmv_sell -- gathers from a band of vertices and its
+    // neighbours; unsorted when 64-row slices pad to <= 1.2 x the entries,
+    // else sorted by length inside 4096-row windows (rows longer than
+    // max(64, 8 mean) walked by a wave) when that pads to <= 1.5 x; otherwise
+    // CSR lanes only. APXCHOL_AMD_SPMV=csr / sell forces the choice;
+    // APXCHOL_AMD_SELL_ORDER=storage slices in storage order instead.
     {
         const std::int64_t nnz = op.ptr.back();
         const std::int64_t long_len = std::max<std::int64_t>(64, 8 * static_cast<std::int64_t>(std::ceil(avg)));
         const char* fe = std::getenv("APXCHOL_AMD_SPMV");
         const std::string force = fe ? fe : "";
+        const char* oe = std::getenv("APXCHOL_AMD_SELL_ORDER");
+        const bool storage_order = oe && std::strcmp(oe, "storage") == 0;
         if (n > 0 && force != "csr") {
-            const std::uint64_t id_slots = sell_identity_slots(op, long_len);
+            std::vector<std::int32_t> rows(static_cast<std::size_t>(n));
+            for (int v = 0; v < n; ++v) rows[v] = storage_order ? v : static_cast<std::int32_t>(sperm[v]);
+            std::int64_t long_nnz = 0;
+            for (int r = 0; r < n; ++r)
+                if (op.ptr[r + 1] - op.ptr[r] > long_len) long_nnz += op.ptr[r + 1] - op.ptr[r];
+            const double short_nnz = static_cast<double>(std::max<std::int64_t>(nnz - long_nnz, 1));
             sell_layout sl;
             bool ok = false;
-            if (id_slots != UINT64_MAX && static_cast<double>(id_slots) <= 1.2 * static_cast<double>(nnz)) {
-                sl = build_sell(op, long_len, 1);
+            if (static_cast<double>(sell_slots(op, long_len, rows)) <= 1.2 * short_nnz) {
+                sl = build_sell(op, long_len, 1, rows);
                 ok = true;
             } else {
-                sl = build_sell(op, long_len, 4096);
-                std::int64_t long_nnz = 0;
-                for (const auto r : sl.longrows) long_nnz += op.ptr[r + 1] - op.ptr[r];
-                ok = force == "sell" ||
-                     static_cast<double>(sl.slots) <= 1.5 * static_cast<double>(std::max<std::int64_t>(nnz - long_nnz, 1));
+                sl = build_sell(op, long_len, 4096, rows);
+                ok = force == "sell" || static_cast<double>(sl.slots) <= 1.5 * short_nnz;
             }
             if (ok) {
                 d.sell_soff.upload(sl.soff);
@@ -731,6 +742,9 @@ void gpu_solver::setup(const csc_view& L) {
                 d.sell_map.upload(sl.rowmap);
                 d.sell_long.upload(sl.longrows);
                 d.sell.n = n;
+                d.sell.max_blocks = kSellMaxBlocks;
+                if (const char* gb = std::getenv("APXCHOL_AMD_SELL_BLOCKS"); gb && *gb)
+                    d.sell.max_blocks = std::clamp(std::atoi(gb), kXcdsHost, kSellMaxBlocks);
                 d.sell.nslices = static_cast<int>(sl.soff.size()) - 1;
                 d.sell.nlong = static_cast<int>(sl.longrows.size());
                 d.sell.soff = d.sell_soff.as<std::uint32_t>();
@@ -750,7 +764,7 @@ void gpu_solver::setup(const csc_view& L) {
     const std::size_t vb = static_cast<std::size_t>(std::max(n, 1)) * sizeof(double);
     d.r.alloc(vb); d.x.alloc(vb); d.p.alloc(vb); d.ap.alloc(vb); d.rp.alloc(vb); d.y.alloc(vb); d.z.alloc(vb);
     d.xout.alloc(vb);
-    d.part.alloc(2 * 2048 * sizeof(double));
+    d.part.alloc(2 * static_cast<std::size_t>(std::max(kMaxBlocksHost, kSellMaxBlocks)) * sizeof(double));
     d.st.alloc(sizeof(pcg_state));
     d.st_apply.alloc(sizeof(pcg_state));
     d.st_apply.zero();
@@ -1159,6 +1173,33 @@ void gpu_solver::apply_block(int k, const double* d_R, double* d_Z, bool center,
     launch_gather_k(d.prec, n, k, kp, d.perm.as<unsigned>(), b.z.get(), d_Z, st, center, s);
 }
 
+void gpu_solver::spmv_block(int k, const double* d_X, double* d_Y, hipStream_t s) const {
+    auto& d = *dev_;
+    auto& b = d.blk;
+    const int n = static_cast<int>(n_);
+    const int kp = prepare_block(k);
+    std::memset(b.host_bs, 0, sizeof(batch_state));
+    b.host_bs->active = kp;
+    batch_state* bs = b.bs.as<batch_state>();
+    hip_check(hipMemcpyAsync(bs, b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
+    launch_scatter_init_k(n, k, kp, d.perm.as<unsigned>(), d_X, b.p.as<double>(), b.x.as<double>(), s);
+    if (d.use_sell)
+        launch_spmm_sell_k(d.sell, d.op32, kp, b.p.as<double>(), b.ap.as<double>(), nullptr, bs, b.part.as<double>(),
+                           false, s);
+    else
+        launch_spmm_k(n, kp, d.lanes, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32,
+                      b.p.as<double>(), b.ap.as<double>(), nullptr, bs, b.part.as<double>(), false, s);
+    launch_gather_k(vec_prec::fp64, n, k, kp, d.perm.as<unsigned>(), b.ap.get(), d_Y, b.st.as<pcg_state>(), false, s);
+}
+
+void gpu_solver::spmv_batch_device(int k, const double* d_X, double* d_Y, hipStream_t s) const {
+    if (k < 0) throw std::invalid_argument("gpu_solver::spmv_batch: negative column count");
+    if (n_ == 0 || k == 0) return;
+    for (int c0 = 0; c0 < k; c0 += 64)
+        spmv_block(std::min(64, k - c0), d_X + static_cast<std::size_t>(c0) * n_, d_Y + static_cast<std::size_t>(c0) * n_,
+                   s);
+}
+
 void gpu_solver::apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const {
     if (k < 0) throw std::invalid_argument("gpu_solver::apply_batch: negative column count");
     if (n_ == 0 || k == 0) return;
@@ -1258,8 +1299,11 @@ void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s) const {
     const int n = static_cast<int>(n_);
     pcg_state* st = b.st.as<pcg_state>();
     batch_state* bs = b.bs.as<batch_state>();
-    launch_spmm_k(n, kp, d.lanes, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32,
-                  b.p.as<double>(), b.ap.as<double>(), st, bs, b.part.as<double>(), true, s);
+    if (d.use_sell)
+        launch_spmm_sell_k(d.sell, d.op32, kp, b.p.as<double>(), b.ap.as<double>(), st, bs, b.part.as<double>(), true, s);
+    else
+        launch_spmm_k(n, kp, d.lanes, d.op_ptr.as<unsigned>(), d.op_idx.as<int>(), d.op_val.get(), d.op32,
+                      b.p.as<double>(), b.ap.as<double>(), st, bs, b.part.as<double>(), true, s);
     launch_update_xr_k(n, kp, b.x.as<double>(), b.p.as<double>(), b.r.as<double>(), b.ap.as<double>(), st, bs,
                        b.part.as<double>(), s);
     launch_refresh_active_k(kp, st, bs, s);

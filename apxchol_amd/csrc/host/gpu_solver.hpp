@@ -148,6 +148,9 @@ public:
     void sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const;
     // y = A x in the caller's order (n doubles each) with the single SpMV.
     void spmv_device(const double* d_x, double* d_y, hipStream_t s) const;
+    // Y = A X for k columns (column c at d_X + c n) with the blocked SpMM,
+    // blocks of at most 64 columns; every column gets spmv_device's bits.
+    void spmv_batch_device(int k, const double* d_X, double* d_Y, hipStream_t s) const;
 
     void reset_apply_count() const { apply_count_ = 0; }
 
@@ -182,6 +185,7 @@ private:
     void replay_iterations_k(int kp, int count, hipStream_t s) const;
     void fresh_state(pcg_state& h, double tol, int max_iter) const;
     const block_sched* block_scheds(int kp) const;
+    void spmv_block(int k, const double* d_X, double* d_Y, hipStream_t s) const;
     void apply_block(int k, const double* d_R, double* d_Z, bool center, hipStream_t s) const;
     void enqueue_iteration_k(int kp, hipStream_t s) const;
     void enqueue_precondition_k(bool first, int kp, hipStream_t s) const;

@@ -324,32 +324,36 @@ permuted_operator build_permuted_operator(const csc_view& A, const std::vector<n
     return op;
 }
 
-std::uint64_t sell_identity_slots(const permuted_operator& op, std::int64_t long_len) {
+std::uint64_t sell_slots(const permuted_operator& op, std::int64_t long_len, const std::vector<std::int32_t>& rows) {
     const std::int64_t n = op.n;
     std::uint64_t slots = 0;
-    for (std::int64_t s0 = 0; s0 < n; s0 += 64) {
-        std::int64_t w = 0;
-        for (std::int64_t r = s0; r < std::min(n, s0 + 64); ++r) {
-            const std::int64_t len = op.ptr[r + 1] - op.ptr[r];
-            if (len > long_len) return UINT64_MAX;
-            w = std::max(w, len);
-        }
-        slots += static_cast<std::uint64_t>(w) * 64;
+    std::int64_t in_slice = 0, w = 0;
+    for (std::int64_t i = 0; i < n; ++i) {
+        const std::int64_t r = rows.empty() ? i : rows[i];
+        const std::int64_t len = op.ptr[r + 1] - op.ptr[r];
+        if (len > long_len) continue;
+        w = std::max(w, len);
+        if (++in_slice == 64) { slots += static_cast<std::uint64_t>(w) * 64; in_slice = 0; w = 0; }
     }
-    return slots;
+    return slots + static_cast<std::uint64_t>(w) * 64;
 }
 
-sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::int64_t sigma) {
+sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::int64_t sigma,
+                       const std::vector<std::int32_t>& rows) {
     sell_layout s;
     const std::int64_t n = op.n;
-    const bool map = sigma > 1;
+    const bool map = !rows.empty();
+    if (map && static_cast<std::int64_t>(rows.size()) != n) throw std::invalid_argument("apxchol_amd: sliced ELL row order");
+    if (!map && sigma > 1) throw std::invalid_argument("apxchol_amd: sliced ELL sorting needs a row map");
+    auto len = [&](std::int64_t r) { return op.ptr[r + 1] - op.ptr[r]; };
     std::vector<std::int32_t> order;   // short rows in slice order
     order.reserve(static_cast<std::size_t>(n));
-    auto len = [&](std::int64_t r) { return op.ptr[r + 1] - op.ptr[r]; };
-    for (std::int64_t w0 = 0; w0 < n; w0 += std::max<std::int64_t>(sigma, 1)) {
-        const std::int64_t w1 = std::min(n, w0 + std::max<std::int64_t>(sigma, 1));
+    const std::int64_t win = std::max<std::int64_t>(sigma, 1);
+    for (std::int64_t w0 = 0; w0 < n; w0 += win) {
         const std::size_t first = order.size();
-        for (std::int64_t r = w0; r < w1; ++r) {
+        for (std::int64_t i = w0; i < std::min(n, w0 + win); ++i) {
+            const std::int64_t r = map ? rows[i] : i;
+            if (r < 0 || r >= n) throw std::invalid_argument("apxchol_amd: sliced ELL row order");
             if (len(r) > long_len) {
                 if (!map) throw std::invalid_argument("apxchol_amd: sliced ELL without a row map has a long row");
                 s.longrows.push_back(static_cast<std::int32_t>(r));
@@ -357,16 +361,16 @@ sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::
                 order.push_back(static_cast<std::int32_t>(r));
             }
         }
-        if (map)
+        if (sigma > 1)
             std::stable_sort(order.begin() + static_cast<std::ptrdiff_t>(first), order.end(),
                              [&](std::int32_t a, std::int32_t b) { return len(a) > len(b); });
     }
-    const std::int64_t nrows = map ? static_cast<std::int64_t>(order.size()) : n;
+    const std::int64_t nrows = static_cast<std::int64_t>(order.size());
     const std::int64_t nslices = (nrows + 63) / 64;
     s.soff.assign(static_cast<std::size_t>(nslices) + 1, 0);
     for (std::int64_t q = 0; q < nslices; ++q) {
         std::int64_t w = 0;
-        for (std::int64_t l = q * 64; l < std::min(nrows, q * 64 + 64); ++l) w = std::max(w, len(map ? order[l] : l));
+        for (std::int64_t l = q * 64; l < std::min(nrows, q * 64 + 64); ++l) w = std::max(w, len(order[l]));
         const std::uint64_t next = static_cast<std::uint64_t>(s.soff[q]) + static_cast<std::uint64_t>(w);
         if (next > UINT32_MAX / 64) throw std::runtime_error("apxchol_amd: sliced ELL too large");
         s.soff[q + 1] = static_cast<std::uint32_t>(next);
@@ -381,7 +385,7 @@ sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::
         for (std::int64_t l = 0; l < 64; ++l) {
             const std::int64_t pos = q * 64 + l;
             if (pos >= nrows) continue;
-            const std::int64_t r = map ? order[pos] : pos;
+            const std::int64_t r = order[pos];
             if (map) s.rowmap[pos] = static_cast<std::int32_t>(r);
             const std::int64_t b = op.ptr[r], e = op.ptr[r + 1];
             for (std::int64_t k = 0; k < w; ++k) {

@@ -80,9 +80,10 @@ permuted_operator build_permuted_operator(const csc_view& A, const std::vector<n
 
 // Sliced-ELL form of the operator (device.hpp's sell_desc): rows with more
 // than long_len entries are listed in longrows; the others are packed 64 to a
-// slice, in row order (sigma <= 1: slice s = rows [64s, 64s+64), no row map --
-// only valid without long rows) or sorted by decreasing length (stable) inside
-// windows of sigma rows, with a row map.
+// slice in the order `rows` gives (a permutation of the n rows; with sigma > 1
+// stably sorted by decreasing length inside windows of sigma of them) and a
+// row map records it. Empty `rows`: slice s = rows [64s, 64s+64), no map (only
+// valid without long rows and sigma <= 1).
 struct sell_layout {
     std::vector<std::uint32_t> soff;
     std::vector<std::int32_t> idx;
@@ -92,9 +93,10 @@ struct sell_layout {
     std::vector<std::int32_t> longrows;
     std::uint64_t slots = 0;              // padded entries of the slices
 };
-// Padded entries of the identity slicing (no map), or UINT64_MAX when a row is
-// longer than long_len.
-std::uint64_t sell_identity_slots(const permuted_operator& op, std::int64_t long_len);
-sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::int64_t sigma);
+// Padded entries of the unsorted slicing of `rows` (empty: 0..n-1) without
+// the rows longer than long_len.
+std::uint64_t sell_slots(const permuted_operator& op, std::int64_t long_len, const std::vector<std::int32_t>& rows);
+sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::int64_t sigma,
+                       const std::vector<std::int32_t>& rows);
 
 }  // namespace apxchol_amd

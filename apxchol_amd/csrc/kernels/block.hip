@@ -242,7 +242,7 @@ __device__ __forceinline__ void col_block_partials(double (&v)[N], int kp, doubl
 }
 
 // Stage 2 (one workgroup) over the partials of nb blocks: thread (j, c) sums
-// blocks j, j + rows, ... of column c (all loads in flight at once), then
+// blocks j, j + rows, ... of column c (eight loads in flight at a time), then
 // column c's `rows` sums are added in j order. Thread c < kp returns true
 // with the totals of column c in v.
 template <int N>
@@ -253,7 +253,20 @@ __device__ __forceinline__ bool col_finish_sum(double (&v)[N], int kp, const dou
     double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] = 0.0;
-    for (int b = j; b < nb; b += rows)
+    // Eight blocks' loads in flight at once, added in block order.
+    int b = j;
+    for (; b + 7 * rows < nb; b += 8 * rows) {
+        double g[8][N];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < N; ++k) g[u][k] = part[(static_cast<size_t>(b + u * rows) * N + k) * kp + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc[k] += g[u][k];
+    }
+    for (; b < nb; b += rows)
 #pragma unroll
         for (int k = 0; k < N; ++k) acc[k] += part[(static_cast<size_t>(b) * N + k) * kp + c];
 #pragma unroll
@@ -320,6 +333,106 @@ inline int spmm_grid(long long n, int kp, int G) {
     const long long q = U > kBlock ? U / kBlock : 1;
     g = (g + q - 1) / q * q;
     return static_cast<int>(g);
+}
+
+// Y = A P for kp columns over the sliced-ELL form (sell_desc): wave q of a
+// slice covers its rows [q 64/kp, (q+1) 64/kp) for all kp columns (thread:
+// row lane t / kp, column t % kp), so a row's entry loads are shared by its kp
+// threads and the gathers P[j][0..kp) are contiguous. A long row is walked by
+// one wave per column (lane-strided + butterfly). Every column's sum is
+// spmv_sell's for that row (same entries, padding and order), so each column
+// gets the single sliced-ELL SpMV's bits. Items (slice waves, then long-row
+// waves) are split into XCD bands as in spmv_sell. With want_dot, the
+// per-column partials of p.Ap over the slices go to part.
+template <class VT, bool MAP>
+__global__ __launch_bounds__(kBlock) void spmm_sell_k(sell_desc a, int kp, const double* __restrict__ x,
+                                                      double* __restrict__ y, const batch_state* bs, double* part,
+                                                      int want_dot) {
+    if (bs->active == 0) return;
+    __shared__ double sh[kBlock];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = lane & (kp - 1);
+    const int band = static_cast<int>(blockIdx.x) % kXcds;
+    const int per = static_cast<int>(gridDim.x) / kXcds;
+    const long long items = (static_cast<long long>(a.nslices) + a.nlong) * kp;
+    const long long i0 = items * band / kXcds, i1 = items * (band + 1) / kXcds;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) / kXcds * kWavesPerBlock +
+                                                    static_cast<int>(threadIdx.x) / kWave);
+    const int* __restrict__ sidx = a.idx;
+    const VT* __restrict__ sval = static_cast<const VT*>(a.val);
+    const long long slice_items = static_cast<long long>(a.nslices) * kp;
+    double acc = 0.0;
+    for (long long it = i0 + wave; it < i1; it += static_cast<long long>(per) * kWavesPerBlock) {
+        if (it < slice_items) {
+            const int sl = static_cast<int>(it / kp), q = static_cast<int>(it % kp);
+            const int l = q * (kWave / kp) + lane / kp;
+            const unsigned c0 = a.soff[sl], c1 = a.soff[sl + 1];
+            const int w = static_cast<int>(c1 - c0);
+            const size_t base = static_cast<size_t>(c0) * kWave + l;
+            const int row = MAP ? a.rowmap[static_cast<size_t>(sl) * kWave + l] : sl * kWave + l;
+            double s = 0.0;
+            for (int k = 0; k < w; k += 8) {
+                int j[8];
+                double v[8], g[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = k + u < w;
+                    j[u] = ok ? sidx[base + static_cast<size_t>(k + u) * kWave] : 0;
+                    v[u] = ok ? static_cast<double>(sval[base + static_cast<size_t>(k + u) * kWave]) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) g[u] = k + u < w ? x[static_cast<size_t>(j[u]) * kp + c] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (k + u < w) s = fma(v[u], g[u], s);
+            }
+            if (row >= 0 && row < a.n) {
+                const size_t o = static_cast<size_t>(row) * kp + c;
+                y[o] = s;
+                acc = fma(x[o], s, acc);
+            }
+        } else {
+            const long long li = it - slice_items;
+            const int row = a.longrows[li / kp];
+            const int col = static_cast<int>(li % kp);
+            const unsigned b = a.ptr[row], e = a.ptr[row + 1];
+            const VT* __restrict__ cval = static_cast<const VT*>(a.cval);
+            double s = 0.0;
+            for (unsigned p = b + lane; p < e; p += kWave)
+                s = fma(static_cast<double>(cval[p]), x[static_cast<size_t>(a.cidx[p]) * kp + col], s);
+            s = wave_sum(s);
+            // The long rows' share of p.Ap comes from spmm_long_dot_k (its
+            // waves sit in column-dependent workgroups).
+            if (lane == 0) y[static_cast<size_t>(row) * kp + col] = s;
+        }
+    }
+    if (!want_dot) return;
+    double v[1] = {acc};
+    col_block_partials<1>(v, kp, part, sh);
+}
+
+// p.Ap over the long rows of the sliced-ELL form, per column in long-row
+// order, stored as one more block of partials (block nb, after
+// spmm_sell_k's): a column's total does not depend on its slot.
+__global__ __launch_bounds__(kBlock) void spmm_long_dot_k(sell_desc a, int kp, const double* __restrict__ x,
+                                                          const double* __restrict__ y, const batch_state* bs,
+                                                          double* part, int nb) {
+    if (bs->active == 0) return;
+    __shared__ double sh[kBlock];
+    const int c = threadIdx.x & (kp - 1);
+    const int j = threadIdx.x / kp, rows = kBlock / kp;
+    double s = 0.0;
+    for (int i = j; i < a.nlong; i += rows) {
+        const size_t o = static_cast<size_t>(a.longrows[i]) * kp + c;
+        s = fma(x[o], y[o], s);
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < kp) {
+        double t = sh[c];
+        for (int r = 1; r < rows; ++r) t += sh[r * kp + c];
+        part[static_cast<size_t>(nb) * kp + c] = t;
+    }
 }
 
 // ---- vector passes ---------------------------------------------------------

@@ -10,6 +10,7 @@
 namespace apxchol_amd {
 
 inline constexpr int kBlock = 256;          // threads per workgroup of the vector kernels
+inline constexpr int kXcds = 8;             // MI355X: accelerator dies, each with its own L2
 inline constexpr int kWavesPerBlock = kBlock / kWave;
 
 // Butterfly over the whole wave: every lane ends with the same total.
@@ -64,7 +65,20 @@ __device__ __forceinline__ bool finish_sum(double (&v)[N], const double* part, i
     double acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] = 0.0;
-    for (int b = static_cast<int>(threadIdx.x); b < nb; b += kBlock)
+    // Eight blocks' loads in flight at once, added in block order.
+    int b = static_cast<int>(threadIdx.x);
+    for (; b + 7 * kBlock < nb; b += 8 * kBlock) {
+        double g[8][N];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < N; ++k) g[u][k] = part[(b + u * kBlock) * N + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc[k] += g[u][k];
+    }
+    for (; b < nb; b += kBlock)
 #pragma unroll
         for (int k = 0; k < N; ++k) acc[k] += part[b * N + k];
     block_sum<N>(acc, sh);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
@@ -188,7 +189,8 @@ void launch_spmv_sell(const sell_desc& a, bool val32, const double* x, double* y
     const long long items = static_cast<long long>(a.nslices) + a.nlong;
     if (items <= 0) return;
     long long g = (items + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (g > vec_block_cap()) g = vec_block_cap();
+    if (g > a.max_blocks) g = a.max_blocks;
+    g = std::max<long long>(kXcds, g / kXcds * kXcds);   // whole bands (spmv_sell)
     const int w = want_dot ? 1 : 0;
     auto go = [&](auto kernel) {
         launch_timed(hook, 4, kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, a, x, y, st, part, w);
@@ -388,6 +390,34 @@ void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx
                                                 part, want_dot ? 1 : 0);
     check_launch("spmm_pAp_k");
     if (want_dot) finish_k(kBFinPAp, n, kp, part, g, st, bs, s);
+}
+
+void launch_spmm_sell_k(const sell_desc& a, bool val32, int kp, const double* x, double* y, pcg_state* st,
+                        batch_state* bs, double* part, bool want_dot, hipStream_t s) {
+    check_kp(kp);
+    const long long items = (static_cast<long long>(a.nslices) + a.nlong) * kp;
+    if (items <= 0) return;
+    long long g = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (g > vec_block_cap()) g = vec_block_cap();
+    g = std::max<long long>(kXcds, g / kXcds * kXcds);
+    const int w = want_dot ? 1 : 0;
+    const dim3 grid(static_cast<unsigned>(g));
+    if (val32) {
+        if (a.rowmap) spmm_sell_k<float, true><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+        else spmm_sell_k<float, false><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+    } else {
+        if (a.rowmap) spmm_sell_k<double, true><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+        else spmm_sell_k<double, false><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+    }
+    check_launch("spmm_sell_k");
+    if (!want_dot) return;
+    int nb = static_cast<int>(g);
+    if (a.nlong > 0) {
+        spmm_long_dot_k<<<1, kBlock, 0, s>>>(a, kp, x, y, bs, part, nb);
+        check_launch("spmm_long_dot_k");
+        ++nb;
+    }
+    finish_k(kBFinPAp, a.n, kp, part, nb, st, bs, s);
 }
 
 void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const double* B, double* r, double* x,

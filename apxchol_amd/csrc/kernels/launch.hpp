@@ -121,6 +121,9 @@ void launch_spmm_k(int n, int kp, int lanes, const unsigned* ptr, const int* idx
                    const double* x, double* y, pcg_state* st, batch_state* bs, double* part, bool want_dot, hipStream_t s);
 void launch_scatter_init_k(int n, int k, int kp, const unsigned* perm, const double* B, double* r, double* x,
                            hipStream_t s);
+// The blocked SpMM over the sliced-ELL form (each column = spmv_sell's bits).
+void launch_spmm_sell_k(const sell_desc& a, bool val32, int kp, const double* x, double* y, pcg_state* st,
+                        batch_state* bs, double* part, bool want_dot, hipStream_t s);
 void launch_init_residual_k(int n, int kp, const double* r, pcg_state* st, batch_state* bs, double* part, hipStream_t s);
 void launch_update_xr_k(int n, int kp, double* x, const double* p, double* r, const double* ap, pcg_state* st,
                         batch_state* bs, double* part, hipStream_t s);

@@ -57,14 +57,23 @@ __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* _
     if (st && st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * kWavesPerBlock +
-                                                    static_cast<int>(threadIdx.x) / kWave);
-    const int waves = static_cast<int>(gridDim.x) * kWavesPerBlock;
+    // XCD-aware split: the dispatcher places workgroup b on XCD b % kXcds, so
+    // the items (slices, then long rows) are cut into kXcds contiguous bands
+    // and band x is shared by the workgroups b = x (mod kXcds). With slices in
+    // vertex order each XCD's L2 then serves one band of rows and the x
+    // entries they gather. (The grid is a multiple of kXcds; the split only
+    // places work, every item is done once whatever the placement.)
+    const int band = static_cast<int>(blockIdx.x) % kXcds;
+    const int per = static_cast<int>(gridDim.x) / kXcds;
     const int items = a.nslices + a.nlong;
+    const int i0 = static_cast<int>(static_cast<long long>(items) * band / kXcds);
+    const int i1 = static_cast<int>(static_cast<long long>(items) * (band + 1) / kXcds);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) / kXcds * kWavesPerBlock +
+                                                    static_cast<int>(threadIdx.x) / kWave);
     const int* __restrict__ sidx = a.idx;
     const VT* __restrict__ sval = static_cast<const VT*>(a.val);
     double acc = 0.0;
-    for (int it = wave; it < items; it += waves) {
+    for (int it = i0 + wave; it < i1; it += per * kWavesPerBlock) {
         if (it < a.nslices) {
             const unsigned c0 = a.soff[it], c1 = a.soff[it + 1];
             const int w = static_cast<int>(c1 - c0);

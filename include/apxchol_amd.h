@@ -222,6 +222,10 @@ apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rh
  * pcg_cuda::spmv_pAp (include/apxchol/solver/pcg_cuda_kernels.h:75-93) without
  * the dot product. d_x, d_y: n doubles. */
 apx_status apx_solver_spmv(apx_solver_t s, const double* d_x, double* d_y, void* stream);
+/* Y = L X for k columns (column c at d_X + c*n / d_Y + c*n) with the blocked
+ * SpMM (no counterpart in the reference); every column gets apx_solver_spmv's
+ * bits. */
+apx_status apx_solver_spmv_batch(apx_solver_t s, int32_t k, const double* d_X, double* d_Y, void* stream);
 
 /* Kernel timing with HIP events on the launching stream (no reference
  * counterpart; the reference's APXCHOL_PCG_TRACE per-component ms/iter,

@@ -63,6 +63,29 @@ def test_apply_batch_bits_equal_single_apply(gpu, name, precision, fp16):
     s.close()
 
 
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("fmt", ["auto", "csr"])
+@pytest.mark.parametrize("k", [5, 70])
+def test_spmv_batch_bits_equal_single_spmv(gpu, name, fmt, k, monkeypatch):
+    """The blocked SpMM gives every column the single SpMV's bits, over the
+    sliced-ELL form (slices, and wave-walked long rows on R-MAT) or the CSR
+    lanes alike; k = 70 runs as blocks of 64 and 6."""
+    import torch
+    A, F = case(name)
+    print(f"[case] spmv_batch {name} {fmt} k={k}", flush=True)
+    if fmt != "auto":
+        monkeypatch.setenv("APXCHOL_AMD_SPMV", fmt)
+    s = ax.Solver(A, factor=F, precision="fp32")
+    X = torch.tensor(np.random.default_rng(13).standard_normal((k, A.shape[0])), device=gpu)
+    Y = s.spmv_batch(X).cpu().numpy()
+    for c in range(k):
+        y = s.spmv(X[c].clone()).cpu().numpy()
+        assert np.array_equal(bits(y), bits(Y[c])), c
+    Yr = (A @ X.cpu().numpy().T).T
+    assert np.abs(Y - Yr).max() <= 1e-12 * np.abs(Yr).max()
+    s.close()
+
+
 @pytest.mark.parametrize("name", ["grid2d_64", "rmat_12"])
 def test_apply_batch_centring_application(gpu, name):
     import torch
