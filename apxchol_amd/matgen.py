@@ -158,9 +158,9 @@ def workload(name: str):
 
 
 # The serving mix: 8 resident factors -- 2-D grids of 1e4 to 1e6 vertices,
-# 3-D grids up to 96^3, R-MAT graphs of scale 16 to 18.
-SERVING_FACTORS = ("grid2d_100", "grid2d_316", "grid2d_1000", "grid3d_48", "grid3d_96", "rmat_16", "rmat_17",
-                   "rmat_18")
+# 3-D grids up to 96^3, R-MAT graphs of scale 16 to 20.
+SERVING_FACTORS = ("grid2d_100", "grid2d_316", "grid2d_1000", "grid3d_48", "grid3d_96", "rmat_16", "rmat_18",
+                   "rmat_20")
 
 
 def request_rhs(L: sp.csc_matrix, sddm: bool, seed: int) -> np.ndarray:

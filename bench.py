@@ -10,6 +10,13 @@ fp64-sweep path is timed beside it. Every solve is graded by the true residual
 ||b - A x|| / ||b|| recomputed in fp64 on the host, and the first solution is
 compared with the CPU oracle's fp64 solve of the same system.
 
+Beside the headline, blocked k-right-hand-side solves (k = 32, fp32 sweeps,
+autotuned SpMV form and block width) are timed on the same system, against
+its plain fp64 single-vector rate, and on the 64^3 anisotropic grid
+(weights 1/10/100 x exp(N(0,0.5))); the serving mix runs 8 resident factors
+(2-D grids 1e4..1e6, 3-D grids to 96^3, R-MAT scales 16..20) under Poisson
+arrivals and reports p50 / p99 latency and solves/s.
+
 The timed solves replay each PCG iteration as a captured HIP graph. After
 them, a timing pass of --timing-solves solves launches eagerly with HIP events
 around every kernel on the solve's stream; the roofline object covers the
@@ -102,6 +109,55 @@ def run_serving(args, ax, mg, np) -> dict:
             "worst_true_residual_over_tol": worst, "memory_used_bytes": st["memory_used"]}
 
 
+def run_batched(args, ax, mg, np, torch, dev, name, A, F, plain_rate, seed0) -> dict:
+    """One blocked PCG of k right-hand sides (fp32 sweeps, graph-replayed
+    iterations) after the autotuner has picked the SpMV form and block width;
+    N(0,1) right-hand sides of seeds seed0 .. seed0+k-1, projected onto the
+    range where the matrix is a Laplacian. Without plain_rate the plain fp64
+    single-vector solve of the same system (seed 0 right-hand side) is timed
+    for the ratio. Every column is graded by its true residual."""
+    if A is None:
+        A, _ = mg.workload(name)
+        F = ax.factorize(A, seed=0)
+    n = A.shape[0]
+    k = args.batch_k
+    B = np.stack([mg.request_rhs(A, F.sddm, seed0 + c) for c in range(k)])
+    Bd = torch.tensor(B, device=dev)
+    s = ax.Solver(A, factor=F, precision="fp32", iters_per_check=args.iters_per_check)
+    tune = s.autotune(max_block_width=64)
+    s.solve_batch(Bd)
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Xb, rb = s.solve_batch(Bd)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    Xh = Xb.cpu().numpy()
+    worst = max(float(np.linalg.norm(B[c] - A @ Xh[c]) / np.linalg.norm(B[c])) for c in range(k))
+    s.close()
+    if plain_rate is None:
+        p = ax.Solver(A, factor=F, precision="fp64", iters_per_check=args.iters_per_check)
+        bd = torch.tensor(mg.request_rhs(A, F.sddm, 0), device=dev)
+        xd = torch.empty_like(bd)
+        p.solve(bd, out=xd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            p.solve(bd, out=xd)
+        torch.cuda.synchronize()
+        plain_rate = 3 / (time.perf_counter() - t0)
+        p.close()
+    rate = k / tb
+    print(f"[bench] blocked {name} k={k}: {rate:.1f} solves/s ({rate / plain_rate:.1f}x the fp64 single-vector rate)",
+          file=sys.stderr, flush=True)
+    return {"n": n, "k": k, "solves_per_s": rate, "ms_per_block": tb * 1e3,
+            "iterations_max": max(r.iterations for r in rb), "worst_true_residual": worst,
+            "fp64_single_solves_per_s": plain_rate, "over_fp64_single": rate / plain_rate,
+            "autotune": {"spmv": "sell" if tune["spmv_sell"] else f"csr lanes {tune['spmv_lanes']}",
+                         "block_width": tune["block_width"]}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,9 +168,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--timing-solves", type=int, default=2)
-    ap.add_argument("--batch-k", type=int, default=32, help="right-hand sides of the blocked measurement (0: skip)")
+    ap.add_argument("--batch-k", type=int, default=32, help="right-hand sides of the blocked measurements (0: skip)")
+    ap.add_argument("--batch-workload", default="aniso3d_64",
+                    help="second blocked workload (default: the 64^3 anisotropic grid; empty: none)")
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
-    ap.add_argument("--serving-qps", type=float, default=200.0)
+    ap.add_argument("--serving-qps", type=float, default=50.0)
     ap.add_argument("--serving-slots", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibration of the HBM byte counters)")
@@ -223,30 +281,15 @@ def main() -> int:
         ms, cnt = tm[k]
         sweep_gbps[k] = (lb[di] + tb[di]) / (ms / 1e3 / max(cnt, 1)) / 1e9 if cnt else 0.0
 
-    # Blocked k-RHS solves of the same system (mixed), against the plain
-    # single-vector fp64 path.
+    # Blocked k-RHS solves (mixed): the 128^3 system itself, against its plain
+    # single-vector fp64 rate, and the 64^3 anisotropic block workload.
     batched = None
     if args.batch_k > 0:
-        s = ax.Solver(A, factor=F, precision="fp32", iters_per_check=args.iters_per_check)
-        rng = np.random.default_rng(100)
-        B = torch.tensor(rng.standard_normal((args.batch_k, n)), device=dev)
-        s.solve_batch(B)
-        torch.cuda.synchronize()
-        reps = 2
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            Xb, rb = s.solve_batch(B)
-        torch.cuda.synchronize()
-        tb = (time.perf_counter() - t0) / reps
-        Xh = Xb.cpu().numpy()
-        Bh = B.cpu().numpy()
-        worst = max(float(np.linalg.norm(Bh[c] - A @ Xh[c]) / np.linalg.norm(Bh[c])) for c in range(args.batch_k))
-        s.close()
         plain_rate = (args.steps / plain["total"]) if plain else None
-        batched = {"k": args.batch_k, "solves_per_s": args.batch_k / tb, "ms_per_block": tb * 1e3,
-                   "iterations_max": max(r.iterations for r in rb), "worst_true_residual": worst,
-                   "over_fp64_single": (args.batch_k / tb) / plain_rate if plain_rate else None}
-        print(f"[bench] blocked k={args.batch_k}: {args.batch_k / tb:.1f} solves/s", file=sys.stderr, flush=True)
+        batched = {args.workload: run_batched(args, ax, mg, np, torch, dev, args.workload, A, F, plain_rate, 100)}
+        if args.batch_workload and args.batch_workload != args.workload:
+            batched[args.batch_workload] = run_batched(args, ax, mg, np, torch, dev, args.batch_workload, None, None,
+                                                       None, 0)
 
     serving = run_serving(args, ax, mg, np) if args.serving_requests > 0 else None
 
