@@ -85,6 +85,7 @@ solve_options to_opts(const apx_options* o) {
     so.relabel = o->relabel != 0;
     so.lane_batches = o->lane_batches;
     so.depth_levels = o->depth_levels != 0;
+    so.tail_batches = o->tail_batches;
     return so;
 }
 
@@ -124,6 +125,7 @@ void apx_options_default(apx_options* o) {
     o->relabel = 1;
     o->lane_batches = 0;
     o->depth_levels = 1;
+    o->tail_batches = 0;
 }
 
 const char* apx_last_error(void) { return g_err.c_str(); }

@@ -108,7 +108,8 @@ struct batch_state {
 // chain order); `idx` is the dependency's position minus the run's first.
 inline constexpr int kTailPairs = 4;
 inline constexpr int kTailBlock = 1024;                 // threads of the tail workgroup
-inline constexpr int kTailWaves = kTailBlock / kWave;   // max batches of a tail level
+inline constexpr int kTailWaves = kTailBlock / kWave;   // waves of the tail workgroup
+inline constexpr int kTailMaxBatches = 4 * kTailWaves;  // max batches of a tail level
 struct alignas(16) tail_lane {
     std::int32_t row;    // output row the lane writes, or -1
     std::int32_t slot;   // the row's LDS slot (position minus the run's first)

@@ -493,7 +493,7 @@ struct gpu_solver::device_state {
     // widest block seen) and the sweep tables per block width.
     struct block_tables {
         dbuf f_pos, f_mask, f_col0, b_pos, b_mask, b_col0, f_lf_dev, b_lf_dev;
-        std::vector<int> f_lf, b_lf, f_lpos, b_lpos;
+        std::vector<int> f_lf, b_lf, f_lpos, b_lpos, f_lgmax, b_lgmax;
         std::vector<int> f_seg[3], b_seg[3];   // l0, l1, tail flag
         block_sched f, b;
     };
@@ -801,13 +801,18 @@ void gpu_solver::setup(const csc_view& L) {
     // exceeded the spin limit (reported, not hung), so it is kept well below.
     d.grid = opts_.sweep_blocks > 0 ? opts_.sweep_blocks : env_int("APXCHOL_AMD_SWEEP_BLOCKS", 0);
     if (d.grid <= 0) d.grid = std::max(1, sweep_dataflow_grid(d.prec, d.fp16) / 4);
-    // Hybrid segments: a level with at least tail_rows rows or more batches
-    // than the tail workgroup has waves is launched on its own; every maximal
-    // run of the other (small) levels becomes one tail segment.
-    const int tail_rows = std::min(opts_.tail_rows > 0 ? opts_.tail_rows : env_int("APXCHOL_AMD_TAIL_ROWS", 1024),
+    // Hybrid segments: a level with at least tail_rows rows or more than
+    // level_tail_batches batches is launched on its own; every maximal run of the
+    // other (small) levels becomes one tail segment.
+    const int tail_rows = std::min(opts_.tail_rows > 0 ? opts_.tail_rows : env_int("APXCHOL_AMD_TAIL_ROWS", 2048),
                                    sweep_tail_max_rows(d.prec));
     // A tail run's outputs live in LDS: a run longer than that holds is split.
     const int tail_cap = sweep_tail_max_rows(d.prec);
+    // Batches a tail level may hold (up to kTailMaxBatches: each of the
+    // workgroup's waves takes every kTailWaves-th batch).
+    const int level_tail_batches = std::clamp(opts_.tail_batches > 0 ? opts_.tail_batches
+                                                              : env_flag("APXCHOL_AMD_TAIL_BATCHES", 2 * kTailWaves),
+                                        1, kTailMaxBatches);
     // Launched levels of at most lane_max batches (and no row longer than a
     // wave's chunk) read the static lane layout.
     const int lane_max = opts_.lane_batches < 0 ? 0
@@ -822,7 +827,7 @@ void gpu_solver::setup(const csc_view& L) {
         const auto pos = [&](int q) { return static_cast<int>(bt.pos[lf[q]]); };
         const auto rows = [&](int q) { return pos(q + 1) - pos(q); };
         // A tail level has at most one batch per wave of the tail workgroup.
-        const auto small = [&](int q) { return rows(q) < tail_rows && lf[q + 1] - lf[q] <= kTailWaves; };
+        const auto small = [&](int q) { return rows(q) < tail_rows && lf[q + 1] - lf[q] <= level_tail_batches; };
         for (int l = 0; l < L;) {
             if (!small(l)) {
                 seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
@@ -1237,6 +1242,20 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
             lpos.resize(L + 1);
             for (std::size_t l = 0; l <= L; ++l)
                 lpos[l] = static_cast<int>(backward ? T_.m - T_.level_ptr[L - l] : T_.level_ptr[l]);
+            // Lanes of each level's widest row (the per-(row, column) level
+            // kernel takes levels up to 8; 0 marks a row past 64 lanes).
+            std::vector<int>& lg = backward ? t->b_lgmax : t->f_lgmax;
+            lg.assign(L, 0);
+            const auto& rptr = backward ? T_.b_ptr : T_.f_ptr;
+            for (std::size_t l = 0; l < L; ++l) {
+                int gm = 1;
+                for (int q = lpos[l]; q < lpos[l + 1]; ++q) {
+                    const std::uint32_t row = backward ? T_.m - 1 - static_cast<std::uint32_t>(q) : static_cast<std::uint32_t>(q);
+                    const std::uint32_t len = rptr[row + 1] - rptr[row];
+                    gm = len > static_cast<std::uint32_t>(kWave * kEpl) ? 1 << 30 : std::max(gm, lanes_for_host(len));
+                }
+                lg[l] = gm > kWave ? 0 : gm;
+            }
             // Segments: runs of levels of at most kTailWaves waves whose rows
             // fit the LDS go to one workgroup; the others are launched.
             std::vector<int>(&seg)[3] = backward ? t->b_seg : t->f_seg;
@@ -1268,6 +1287,7 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
             sc.level_first = lf.data();
             sc.level_first_dev = lfd.as<int>();
             sc.level_pos = lpos.data();
+            sc.level_gmax = env_flag("APXCHOL_AMD_BLOCK_ROWS", 1) != 0 ? lg.data() : nullptr;
             sc.nlevels = static_cast<int>(lf.size()) - 1;
             sc.seg_l0 = seg[0].data();
             sc.seg_l1 = seg[1].data();

@@ -106,6 +106,73 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
     out[o] = (r - part) / static_cast<TO>(d.diag[row]);
 }
 
+// One level of a blocked sweep with one thread per (row, column) (rows of at
+// most GMAX lanes, i.e. <= 4 GMAX entries): thread t takes sweep position
+// p0 + t / kp and column t % kp, runs the row's G lane chains itself (lane s:
+// entries s + k G, fma from 0, padding slots fma(0, 0, .)) and then the xor
+// butterfly's tree as lane 0 sees it (step off < G: v[s] += v[s + off] for
+// s < off) -- the lane kernels' arithmetic, so the same bits, without the
+// wave table, idle lanes or shuffles. The right-hand side, diagonal and shift
+// loads are issued with the row pointers, so a level is three dependent
+// loads (pointers, entries, gathered values).
+template <class TI, class TO, bool F16, int GMAX>
+__global__ __launch_bounds__(kBlock) void sweep_block_rows(sweep_desc d, int p0, int p1, int kp,
+                                                           const TI* __restrict__ rhs, TO* out, const pcg_state* st,
+                                                           const batch_state* bs) {
+    if (bs && bs->active == 0) return;
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long long q0 = t / kp;
+    if (q0 >= p1 - p0) return;
+    const int pos = p0 + static_cast<int>(q0);
+    const int c = static_cast<int>(t & (kp - 1));
+    const int row = d.backward ? d.m - 1 - pos : pos;
+    const size_t o = static_cast<size_t>(row) * kp + c;
+    const double shift = (!d.backward && st) ? st[c].shift : 0.0;
+    const double rin = static_cast<double>(rhs[o]);
+    const float dg = d.diag[row];
+    const double sc = (d.backward && d.inv_scale) ? static_cast<double>(d.inv_scale[row]) : 0.0;
+    const unsigned rb = d.ptr[row];
+    const unsigned len = d.ptr[row + 1] - rb;
+    const int G = lanes_for(len);
+    int jj[GMAX][kEpl];
+    float vv[GMAX][kEpl];
+    TO xv[GMAX][kEpl];
+#pragma unroll
+    for (int s = 0; s < GMAX; ++s)
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) {
+            const unsigned q = static_cast<unsigned>(s + k * G);
+            const bool valid = s < G && q < len;
+            load_entry<F16>(d, valid ? rb + q : (len > 0 ? rb : 0u), jj[s][k], vv[s][k]);
+            if (!valid) vv[s][k] = 0.0f;
+        }
+#pragma unroll
+    for (int s = 0; s < GMAX; ++s)
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) {
+            const bool valid = s < G && static_cast<unsigned>(s + k * G) < len;
+            xv[s][k] = valid ? out[static_cast<size_t>(jj[s][k]) * kp + c] : static_cast<TO>(0);
+        }
+    TO v[GMAX];
+#pragma unroll
+    for (int s = 0; s < GMAX; ++s) {
+        TO part = 0;
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[s][k]), xv[s][k], part);
+        v[s] = part;
+    }
+#pragma unroll
+    for (int off = GMAX / 2; off >= 1; off >>= 1)
+        if (off < G)
+#pragma unroll
+            for (int s = 0; s < off; ++s) v[s] += v[s + off];
+    TO r;
+    if (!d.backward) r = static_cast<TO>(rin - shift);
+    else if (d.inv_scale) r = static_cast<TO>(rin * (sc * sc));
+    else r = static_cast<TO>(rin);
+    out[o] = (r - v[0]) / static_cast<TO>(dg);
+}
+
 // A run of small levels of a blocked sweep, walked by ONE workgroup of
 // kTailBlock threads: levels [l0, l1) of the wave table (each at most
 // kTailWaves waves), sweep positions [p0, p1). The run's outputs live in LDS

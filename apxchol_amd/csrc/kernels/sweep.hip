@@ -435,6 +435,9 @@ using vec2 = T __attribute__((ext_vector_type(2)));
 // (clang read element 0 for .y).
 __device__ __forceinline__ float as_float(int v) { return __builtin_bit_cast(float, v); }
 
+// Overflow entries of a lane fetched ahead with its item (long rows).
+inline constexpr int kTailOvf = 8;
+
 // One lane's share of one tail batch, as whole 16-byte vectors: each load
 // lands in one register tuple that is consumed in place.
 template <class TO>
@@ -442,6 +445,9 @@ struct tail_item {
     i32x4 ln;                // tail_lane
     i32x4 e0, e1, e2, e3;    // tail_pair x kTailPairs
     vec2<TO> v;              // tail_val
+    int o;                   // the batch's first overflow row (tail_desc.ovf)
+    bool has_ov;             // ov holds the lane's first overflow entries
+    entry32 ov[kTailOvf];    // (valid below the lane's overflow count)
 };
 
 __device__ __forceinline__ size_t tail_at(int t, int k) {
@@ -458,6 +464,20 @@ __device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<T
     it.e2 = ent[tail_at(t, 2)];
     it.e3 = ent[tail_at(t, 3)];
     it.v = reinterpret_cast<const vec2<TO>*>(tv)[i];
+    it.o = td.ovf[t];
+    it.has_ov = false;
+}
+
+// The first kTailOvf overflow entries of an item whose static part has
+// arrived, fetched ahead of its step (a lane loads only its own entries).
+template <class TO>
+__device__ __forceinline__ void tail_fetch_ovf(const tail_desc& td, tail_item<TO>& it) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nov = (it.ln.z >> 8) - 2 * kTailPairs;
+#pragma unroll
+    for (int u = 0; u < kTailOvf; ++u)
+        if (u < nov) it.ov[u] = td.ovf_ent[(static_cast<size_t>(it.o) + static_cast<size_t>(u)) * kWave + lane];
+    it.has_ov = true;
 }
 
 template <class TO>
@@ -481,12 +501,22 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
     part = tail_pair_fma(it.e2, lds, part);
     part = tail_pair_fma(it.e3, lds, part);
     if (__builtin_expect(c > 2 * kTailPairs, 0)) {
-        // A row longer than 64 * 2 * kTailPairs: the lane's further entries.
-        // Eight entries' loads are issued together, then consumed in chain
-        // order (the same fmas as one at a time).
-        const size_t o = static_cast<size_t>(td.ovf[t]);
+        // A row longer than 64 * 2 * kTailPairs: the lane's further entries,
+        // the first kTailOvf of them fetched ahead (tail_fetch_ovf), the rest
+        // eight loads at a time; consumed in chain order (the same fmas as
+        // one at a time).
+        const size_t o = static_cast<size_t>(it.o);
         const int nov = c - 2 * kTailPairs;
         int s = 0;
+        if (it.has_ov) {
+            TO g[kTailOvf];
+#pragma unroll
+            for (int u = 0; u < kTailOvf; ++u) g[u] = u < nov ? lds[it.ov[u].idx] : static_cast<TO>(0);
+#pragma unroll
+            for (int u = 0; u < kTailOvf; ++u)
+                if (u < nov) part = fma(static_cast<TO>(it.ov[u].val), g[u], part);
+            s = kTailOvf;
+        }
         for (; s + 8 <= nov; s += 8) {
             entry32 e[8];
 #pragma unroll
@@ -510,10 +540,11 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
     if (it.ln.x >= 0) lds[it.ln.y] = (it.v.y - part) / static_cast<TO>(as_float(it.ln.w));
 }
 
-// base: the run's first tail batch; levels [l0, l1) of the run, each of at
-// most kTailWaves batches; zslot: the run's row count (its LDS holds
-// zslot + 1 values); output row of slot s: row0 + s (forward) or row0 - s
-// (back). Outputs reach HBM in one coalesced copy at the end.
+// base: the run's first tail batch; levels [l0, l1) of the run (a level may
+// hold up to kTailMaxBatches batches: wave w takes batches w, w + kTailWaves,
+// ... of it); zslot: the run's row count (its LDS holds zslot + 1 values);
+// output row of slot s: row0 + s (forward) or row0 - s (back). Outputs reach
+// HBM in one coalesced copy at the end.
 template <class TO>
 __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, const int* __restrict__ level_first,
                                                          int l0, int l1, int zslot, int row0, int backward, TO* out,
@@ -526,24 +557,36 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     // The zero slot past the run's outputs, read by padding entries; every
     // wave clears it before its own first read (all writers store 0).
     lds[zslot] = TO(0);
-    // Wave w takes batch w of every level. The item of level l+1 is fetched
-    // before level l is computed, so its loads are in flight during that
-    // level; every fetch is issued (past the run's end it re-reads the last
-    // batch), so the compiler's memory counters see the same loads on every
-    // path. (readfirstlane: the wave index is wave-uniform, so the level
-    // bookkeeping stays scalar.)
+    // Wave w walks its batches in order (level by level); the item of its
+    // next batch -- the next one of this level, else batch w of the next
+    // level -- is fetched before the current one is computed, so its loads
+    // are in flight meanwhile and across the level barrier. Every fetch is
+    // issued (clamped to the run's last batch), so the compiler's memory
+    // counters see the same loads on every path. (readfirstlane: the wave
+    // index is wave-uniform, so the level bookkeeping stays scalar.)
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
     int lb = 0, le = level_first[l0 + 1] - bfirst;
     tail_item<TO> cur;
     tail_fetch<TO>(td, tv, min(base + w, last), cur);
+    tail_fetch_ovf<TO>(td, cur);
     for (int l = l0; l < l1; ++l) {
-        const int nb = le, ne = level_first[l + 2 <= l1 ? l + 2 : l1] - bfirst;
-        tail_item<TO> nxt;
-        tail_fetch<TO>(td, tv, min(base + nb + w, last), nxt);
-        if (lb + w < le) tail_step<TO>(td, base + lb + w, cur, lds);
+        const int ne = level_first[l + 2 <= l1 ? l + 2 : l1] - bfirst;
+        int j = w;
+        // No batch in this level: fetch the next level's (its overflow
+        // entries are then loaded at its step, not here before the barrier).
+        if (lb + j >= le) tail_fetch<TO>(td, tv, min(base + le + w, last), cur);
+        while (lb + j < le) {
+            const int jn = j + kTailWaves;
+            const int tn = lb + jn < le ? base + lb + jn : base + le + w;
+            tail_item<TO> nxt;
+            tail_fetch<TO>(td, tv, min(tn, last), nxt);
+            tail_step<TO>(td, base + lb + j, cur, lds);
+            tail_fetch_ovf<TO>(td, nxt);
+            cur = nxt;
+            j = jn;
+        }
         lds_barrier();
-        cur = nxt;
-        lb = nb;
+        lb = le;
         le = ne;
     }
     for (int q = static_cast<int>(threadIdx.x); q < zslot; q += kTailBlock) out[backward ? row0 - q : row0 + q] = lds[q];

@@ -53,6 +53,8 @@ typedef struct {
                                 layout (0 = default 4096, < 0 = never: batch tables only) */
     int32_t depth_levels;    /* 1 (default, with relabel): sweep levels = topological depth of the
                                 dependency graph; 0: the elimination rounds (round-as-level) */
+    int32_t tail_batches;    /* hybrid: a tail level holds at most this many batches (0 = 32, max 64;
+                                each wave of the tail workgroup takes every 16th) */
 } apx_options;
 
 /* Fill `o` with the defaults above. */

@@ -101,6 +101,7 @@ SCHEDULE_CONFIGS = [
     dict(schedule="levels", lane_batches=-1),    # every level from the batch tables
     dict(schedule="hybrid", lane_batches=-1),
     dict(schedule="levels", lane_batches=1),     # lane layout for one-batch levels only
+    dict(schedule="hybrid", tail_batches=64, tail_rows=4096),   # tail waves take several batches a level
 ]
 
 
@@ -121,7 +122,7 @@ def test_schedules_bit_identical(gpu, name, precision, fp16):
         if cfg["schedule"] == "dataflow" or cfg.get("lane_batches", 0) < 0:
             assert lanes == 0, cfg
         elif cfg.get("lane_batches", 0) == 0:
-            assert lanes > 0, cfg
+            assert lanes > 0 or sum(s.info["level_launches"]) == 0, cfg
         m = s.info["m"]
         x = torch.tensor(np.random.default_rng(3).standard_normal(m), device=gpu)
         y = torch.empty_like(x)
