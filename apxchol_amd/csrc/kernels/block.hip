@@ -56,21 +56,8 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
     TO part = 0;
     if (__builtin_expect(len > static_cast<unsigned>(kWave * kEpl), 0)) {
         // Long row (G = 64, one column per wave): sweep_batch's chunked walk.
-        for (unsigned c0 = 0; c0 < len; c0 += kWave * kEpl) {
-            int jj[kEpl];
-            float vv[kEpl];
-            TO xv[kEpl];
-#pragma unroll
-            for (int k = 0; k < kEpl; ++k) {
-                const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
-                const bool valid = q < len;
-                load_entry<F16>(d, valid ? rb + q : rb + c0, jj[k], vv[k]);
-                if (!valid) vv[k] = 0.0f;
-                xv[k] = valid ? out[static_cast<size_t>(jj[k]) * kp + c] : static_cast<TO>(0);
-            }
-#pragma unroll
-            for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
-        }
+        part = long_row_chain<F16, TO>(d, rb, len, [&](int j) { return out[static_cast<size_t>(j) * kp + c]; },
+                                       [](int) { return true; });
     } else {
         int jj[kEpl];
         float vv[kEpl];
@@ -219,21 +206,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, con
             };
             TO part = 0;
             if (__builtin_expect(len > static_cast<unsigned>(kWave * kEpl), 0)) {
-                for (unsigned c0 = 0; c0 < len; c0 += kWave * kEpl) {
-                    int jj[kEpl];
-                    float vv[kEpl];
-                    TO xv[kEpl];
-#pragma unroll
-                    for (int k = 0; k < kEpl; ++k) {
-                        const unsigned q = c0 + static_cast<unsigned>(lane + k * kWave);
-                        const bool valid = q < len;
-                        load_entry<F16>(d, valid ? rb + q : rb + c0, jj[k], vv[k]);
-                        if (!valid) vv[k] = 0.0f;
-                        xv[k] = valid ? value(jj[k]) : static_cast<TO>(0);
-                    }
-#pragma unroll
-                    for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
-                }
+                part = long_row_chain<F16, TO>(d, rb, len, value, [](int) { return true; });
             } else {
                 int jj[kEpl];
                 float vv[kEpl];

@@ -91,6 +91,39 @@ __device__ __forceinline__ void gather(const TO* out, const int (&jj)[kEpl], TO 
     for (int k = 0; k < kEpl; ++k) xv[k] = ((valid >> k) & 1u) ? g[k] : static_cast<TO>(0);
 }
 
+// A long row's chain as sweep_batch walks it (lane l: entries l, l + 64,
+// l + 128, ... in order, fma from 0), kLongUnroll chunks of 64 kEpl entries
+// per round: a round's entry loads, then its gathers, are in flight together
+// (the order of the fmas is the chunk-by-chunk one). value(j) reads column
+// j; an entry with use(j) false adds nothing, and neither does a slot past
+// the row's end (fma(0, 0, part) = part: a chain never holds -0).
+inline constexpr int kLongUnroll = 4;
+template <bool F16, class TO, class V, class U>
+__device__ __forceinline__ TO long_row_chain(const sweep_desc& d, unsigned rb, unsigned len, V value, U use) {
+    const int lane = threadIdx.x & (kWave - 1);
+    constexpr int R = kLongUnroll * kEpl;
+    TO part = 0;
+    for (unsigned c0 = 0; c0 < len; c0 += static_cast<unsigned>(kWave * R)) {
+        int jj[R];
+        float vv[R];
+        TO xv[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const unsigned q = c0 + static_cast<unsigned>(lane + i * kWave);
+            load_entry<F16>(d, q < len ? rb + q : rb, jj[i], vv[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const bool ok = c0 + static_cast<unsigned>(lane + i * kWave) < len && use(jj[i]);
+            xv[i] = ok ? value(jj[i]) : static_cast<TO>(0);
+            if (!ok) vv[i] = 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) part = fma(static_cast<TO>(vv[i]), xv[i], part);
+    }
+    return part;
+}
+
 // Publish a row: an agent-scope atomic store when other waves poll for it,
 // a plain store when readers are ordered behind a launch or barrier.
 template <class TO, bool WAIT>
@@ -144,7 +177,9 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         // A long row alone in its batch (G = 64): walk it in chunks of
         // 64 * kEpl entries; each lane accumulates its entries chunk by chunk.
         AT part = 0;
-        for (unsigned c0 = 0; c0 < len; c0 += kWave * kEpl) {
+        if constexpr (!WAIT)
+            part = long_row_chain<F16, AT>(d, rb, len, [&](int j) { return out[j]; }, [](int) { return true; });
+        for (unsigned c0 = 0; WAIT && c0 < len; c0 += kWave * kEpl) {
             int jj[kEpl];
             float vv[kEpl];
             AT xv[kEpl];
@@ -394,30 +429,26 @@ __global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst,
     if (b >= blast) return;
     const lane_row x = map_lane(d, b);
     TO part = 0;
-    const bool longrow = x.len > static_cast<unsigned>(kWave * kEpl);
-    const int G = longrow ? kWave : x.G;
-    const int sub = longrow ? lane : x.sub;
-    const bool act = longrow ? true : x.active;
-    if (p0 > 0) {
-        for (unsigned c0 = 0; c0 < (longrow ? x.len : 1u); c0 += kWave * kEpl) {
-            int jj[kEpl];
-            float vv[kEpl];
-            TO g[kEpl];
-            unsigned use = 0;
+    const auto before = [&](int j) { return (d.backward ? d.m - 1 - j : j) < p0; };
+    if (p0 > 0 && x.len > static_cast<unsigned>(kWave * kEpl)) {
+        part = long_row_chain<F16, TO>(d, x.rb, x.len, [&](int j) { return out[j]; }, before);
+    } else if (p0 > 0) {
+        int jj[kEpl];
+        float vv[kEpl];
+        TO g[kEpl];
+        unsigned use = 0;
 #pragma unroll
-            for (int k = 0; k < kEpl; ++k) {
-                const unsigned q = (longrow ? c0 : 0u) + static_cast<unsigned>(sub + k * G);
-                const bool v = act && q < x.len;
-                load_entry<F16>(d, v ? x.rb + q : (x.len > 0 ? x.rb : 0u), jj[k], vv[k]);
-                const int pj = d.backward ? d.m - 1 - jj[k] : jj[k];
-                use |= (v && pj < p0) ? (1u << k) : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
-#pragma unroll
-            for (int k = 0; k < kEpl; ++k)
-                if ((use >> k) & 1u) part = fma(static_cast<TO>(vv[k]), g[k], part);
+        for (int k = 0; k < kEpl; ++k) {
+            const unsigned q = static_cast<unsigned>(x.sub + k * x.G);
+            const bool v = x.active && q < x.len;
+            load_entry<F16>(d, v ? x.rb + q : (x.len > 0 ? x.rb : 0u), jj[k], vv[k]);
+            use |= (v && before(jj[k])) ? (1u << k) : 0u;
         }
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k)
+            if ((use >> k) & 1u) part = fma(static_cast<TO>(vv[k]), g[k], part);
     }
     TO r = 0;
     if (x.active && x.sub == 0) r = row_rhs<TI, TO>(d, rhs, x.row, shift);
