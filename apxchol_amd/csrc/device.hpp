@@ -47,8 +47,8 @@ struct sweep_desc {
 };
 
 // Sliced-ELL form of the PCG operator (the single-vector SpMV). Slice s holds
-// 64 rows (lane = row rowmap[s*64 + lane], or s*64 + lane without a map; -1 or
-// >= n: none) and soff[s+1] - soff[s] entry columns:
+// 64 rows (lane = row rowmap[s*64 + lane]; -1: none) and soff[s+1] - soff[s]
+// entry columns:
 //   idx/val[(soff[s] + k)*64 + lane]   entry k of the lane's row, CSR order
 //                                      (padding: the row itself / 0, value 0)
 // Rows too long for a slice are listed in longrows and read from the CSR
@@ -61,7 +61,7 @@ struct sell_desc {
     const std::uint32_t* soff = nullptr;
     const std::int32_t* idx = nullptr;
     const void* val = nullptr;              // fp64, or fp32 when every value is fp32-exact
-    const std::int32_t* rowmap = nullptr;   // null: identity
+    const std::int32_t* rowmap = nullptr;
     const std::int32_t* longrows = nullptr;
     const std::uint32_t* ptr = nullptr;
     const std::int32_t* cidx = nullptr;

@@ -750,7 +750,7 @@ void gpu_solver::setup(const csc_view& L) {
                 d.sell.soff = d.sell_soff.as<std::uint32_t>();
                 d.sell.idx = d.sell_idx.as<std::int32_t>();
                 d.sell.val = d.sell_val.get();
-                d.sell.rowmap = sl.rowmap.empty() ? nullptr : d.sell_map.as<std::int32_t>();
+                d.sell.rowmap = d.sell_map.as<std::int32_t>();
                 d.sell.longrows = d.sell_long.as<std::int32_t>();
                 d.sell.ptr = d.op_ptr.as<std::uint32_t>();
                 d.sell.cidx = d.op_idx.as<std::int32_t>();

@@ -342,9 +342,7 @@ sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::
                        const std::vector<std::int32_t>& rows) {
     sell_layout s;
     const std::int64_t n = op.n;
-    const bool map = !rows.empty();
-    if (map && static_cast<std::int64_t>(rows.size()) != n) throw std::invalid_argument("apxchol_amd: sliced ELL row order");
-    if (!map && sigma > 1) throw std::invalid_argument("apxchol_amd: sliced ELL sorting needs a row map");
+    if (static_cast<std::int64_t>(rows.size()) != n) throw std::invalid_argument("apxchol_amd: sliced ELL row order");
     auto len = [&](std::int64_t r) { return op.ptr[r + 1] - op.ptr[r]; };
     std::vector<std::int32_t> order;   // short rows in slice order
     order.reserve(static_cast<std::size_t>(n));
@@ -352,10 +350,9 @@ sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::
     for (std::int64_t w0 = 0; w0 < n; w0 += win) {
         const std::size_t first = order.size();
         for (std::int64_t i = w0; i < std::min(n, w0 + win); ++i) {
-            const std::int64_t r = map ? rows[i] : i;
+            const std::int64_t r = rows[i];
             if (r < 0 || r >= n) throw std::invalid_argument("apxchol_amd: sliced ELL row order");
             if (len(r) > long_len) {
-                if (!map) throw std::invalid_argument("apxchol_amd: sliced ELL without a row map has a long row");
                 s.longrows.push_back(static_cast<std::int32_t>(r));
             } else {
                 order.push_back(static_cast<std::int32_t>(r));
@@ -379,14 +376,14 @@ sell_layout build_sell(const permuted_operator& op, std::int64_t long_len, std::
     const bool f32 = op.fp32_exact;
     s.idx.assign(s.slots, 0);
     if (f32) s.val32.assign(s.slots, 0.0f); else s.val.assign(s.slots, 0.0);
-    if (map) s.rowmap.assign(static_cast<std::size_t>(nslices) * 64, -1);
+    s.rowmap.assign(static_cast<std::size_t>(nslices) * 64, -1);
     for (std::int64_t q = 0; q < nslices; ++q) {
         const std::int64_t w = s.soff[q + 1] - s.soff[q];
         for (std::int64_t l = 0; l < 64; ++l) {
             const std::int64_t pos = q * 64 + l;
             if (pos >= nrows) continue;
             const std::int64_t r = order[pos];
-            if (map) s.rowmap[pos] = static_cast<std::int32_t>(r);
+            s.rowmap[pos] = static_cast<std::int32_t>(r);
             const std::int64_t b = op.ptr[r], e = op.ptr[r + 1];
             for (std::int64_t k = 0; k < w; ++k) {
                 const std::size_t at = (static_cast<std::size_t>(s.soff[q]) + static_cast<std::size_t>(k)) * 64 +

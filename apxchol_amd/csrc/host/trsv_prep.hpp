@@ -82,14 +82,13 @@ permuted_operator build_permuted_operator(const csc_view& A, const std::vector<n
 // than long_len entries are listed in longrows; the others are packed 64 to a
 // slice in the order `rows` gives (a permutation of the n rows; with sigma > 1
 // stably sorted by decreasing length inside windows of sigma of them) and a
-// row map records it. Empty `rows`: slice s = rows [64s, 64s+64), no map (only
-// valid without long rows and sigma <= 1).
+// row map records it.
 struct sell_layout {
     std::vector<std::uint32_t> soff;
     std::vector<std::int32_t> idx;
     std::vector<double> val;
     std::vector<float> val32;
-    std::vector<std::int32_t> rowmap;     // empty: identity
+    std::vector<std::int32_t> rowmap;
     std::vector<std::int32_t> longrows;
     std::uint64_t slots = 0;              // padded entries of the slices
 };

@@ -384,7 +384,7 @@ inline int spmm_grid(long long n, int kp, int G) {
 // gets the single sliced-ELL SpMV's bits. Items (slice waves, then long-row
 // waves) are split into XCD bands as in spmv_sell. With want_dot, the
 // per-column partials of p.Ap over the slices go to part.
-template <class VT, bool MAP>
+template <class VT>
 __global__ __launch_bounds__(kBlock) void spmm_sell_k(sell_desc a, int kp, const double* __restrict__ x,
                                                       double* __restrict__ y, const batch_state* bs, double* part,
                                                       int want_dot) {
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void spmm_sell_k(sell_desc a, int kp, const
             const unsigned c0 = a.soff[sl], c1 = a.soff[sl + 1];
             const int w = static_cast<int>(c1 - c0);
             const size_t base = static_cast<size_t>(c0) * kWave + l;
-            const int row = MAP ? a.rowmap[static_cast<size_t>(sl) * kWave + l] : sl * kWave + l;
+            const int row = a.rowmap[static_cast<size_t>(sl) * kWave + l];
             double s = 0.0;
             for (int k = 0; k < w; k += 8) {
                 int j[8];

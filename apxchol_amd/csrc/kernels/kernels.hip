@@ -195,8 +195,9 @@ void launch_spmv_sell(const sell_desc& a, bool val32, const double* x, double* y
     auto go = [&](auto kernel) {
         launch_timed(hook, 4, kernel, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, s, a, x, y, st, part, w);
     };
-    if (val32) { if (a.rowmap) go(spmv_sell<float, true>); else go(spmv_sell<float, false>); }
-    else { if (a.rowmap) go(spmv_sell<double, true>); else go(spmv_sell<double, false>); }
+    if (!a.rowmap) throw std::invalid_argument("apxchol_amd: sliced ELL without a row map");
+    if (val32) go(spmv_sell<float>);
+    else go(spmv_sell<double>);
     check_launch("spmv_sell");
     if (want_dot) finish(kFinPAp, a.n, part, static_cast<int>(g), st, s);
 }
@@ -416,13 +417,9 @@ void launch_spmm_sell_k(const sell_desc& a, bool val32, int kp, const double* x,
     g = std::max<long long>(kXcds, g / kXcds * kXcds);
     const int w = want_dot ? 1 : 0;
     const dim3 grid(static_cast<unsigned>(g));
-    if (val32) {
-        if (a.rowmap) spmm_sell_k<float, true><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
-        else spmm_sell_k<float, false><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
-    } else {
-        if (a.rowmap) spmm_sell_k<double, true><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
-        else spmm_sell_k<double, false><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
-    }
+    if (!a.rowmap) throw std::invalid_argument("apxchol_amd: sliced ELL without a row map");
+    if (val32) spmm_sell_k<float><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+    else spmm_sell_k<double><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
     check_launch("spmm_sell_k");
     if (!want_dot) return;
     int nb = static_cast<int>(g);

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
 // walked by a whole wave from the CSR arrays (lane-strided + butterfly). A
 // row's sum is its entries in CSR order (padding adds fma(0, x[row], s) = s).
 // With want_dot, the partials of x.y go to part as in spmv_pAp.
-template <class VT, bool MAP>
+template <class VT>
 __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* __restrict__ x, double* __restrict__ y,
                                                     const pcg_state* st, double* part, int want_dot) {
     if (st && st->done) return;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* _
             const unsigned c0 = a.soff[it], c1 = a.soff[it + 1];
             const int w = static_cast<int>(c1 - c0);
             const size_t base = static_cast<size_t>(c0) * kWave + lane;
-            const int row = MAP ? a.rowmap[static_cast<size_t>(it) * kWave + lane] : it * kWave + lane;
+            const int row = a.rowmap[static_cast<size_t>(it) * kWave + lane];
             // Columns in groups of eight (the last group predicated): all of a
             // group's loads, then its gathers, are in flight together.
             double s = 0.0;

@@ -466,9 +466,6 @@ using vec2 = T __attribute__((ext_vector_type(2)));
 // (clang read element 0 for .y).
 __device__ __forceinline__ float as_float(int v) { return __builtin_bit_cast(float, v); }
 
-// Overflow entries of a lane fetched ahead with its item (long rows).
-inline constexpr int kTailOvf = 8;
-
 // One lane's share of one tail batch, as whole 16-byte vectors: each load
 // lands in one register tuple that is consumed in place.
 template <class TO>
@@ -477,8 +474,6 @@ struct tail_item {
     i32x4 e0, e1, e2, e3;    // tail_pair x kTailPairs
     vec2<TO> v;              // tail_val
     int o;                   // the batch's first overflow row (tail_desc.ovf)
-    bool has_ov;             // ov holds the lane's first overflow entries
-    entry32 ov[kTailOvf];    // (valid below the lane's overflow count)
 };
 
 __device__ __forceinline__ size_t tail_at(int t, int k) {
@@ -496,19 +491,6 @@ __device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<T
     it.e3 = ent[tail_at(t, 3)];
     it.v = reinterpret_cast<const vec2<TO>*>(tv)[i];
     it.o = td.ovf[t];
-    it.has_ov = false;
-}
-
-// The first kTailOvf overflow entries of an item whose static part has
-// arrived, fetched ahead of its step (a lane loads only its own entries).
-template <class TO>
-__device__ __forceinline__ void tail_fetch_ovf(const tail_desc& td, tail_item<TO>& it) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int nov = (it.ln.z >> 8) - 2 * kTailPairs;
-#pragma unroll
-    for (int u = 0; u < kTailOvf; ++u)
-        if (u < nov) it.ov[u] = td.ovf_ent[(static_cast<size_t>(it.o) + static_cast<size_t>(u)) * kWave + lane];
-    it.has_ov = true;
 }
 
 template <class TO>
@@ -526,41 +508,41 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
     const int lane = threadIdx.x & (kWave - 1);
     const int c = it.ln.z >> 8;
     const int G = it.ln.z & 0xff;
+    // A row longer than 64 * 2 * kTailPairs has further entries per lane,
+    // read eight at a time from the overflow table: the first group's loads
+    // are issued before the item's fmas, each next group's before the
+    // current one is consumed. The fmas run in chain order (item, then
+    // overflow).
+    const int nov = c - 2 * kTailPairs;
+    const size_t o = static_cast<size_t>(it.o);
+    int s = 0;
+    entry32 e[8];
+    if (__builtin_expect(s < nov, 0))
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (s + u < nov) e[u] = td.ovf_ent[(o + static_cast<size_t>(s + u)) * kWave + lane];
     TO part = it.v.x;
     part = tail_pair_fma(it.e0, lds, part);
     part = tail_pair_fma(it.e1, lds, part);
     part = tail_pair_fma(it.e2, lds, part);
     part = tail_pair_fma(it.e3, lds, part);
-    if (__builtin_expect(c > 2 * kTailPairs, 0)) {
-        // A row longer than 64 * 2 * kTailPairs: the lane's further entries,
-        // the first kTailOvf of them fetched ahead (tail_fetch_ovf), the rest
-        // eight loads at a time; consumed in chain order (the same fmas as
-        // one at a time).
-        const size_t o = static_cast<size_t>(it.o);
-        const int nov = c - 2 * kTailPairs;
-        int s = 0;
-        if (it.has_ov) {
-            TO g[kTailOvf];
+    if (__builtin_expect(nov > 0, 0)) {
+        while (s < nov) {
+            const int sn = s + 8;
+            entry32 nx[8];
+            if (sn < nov)
 #pragma unroll
-            for (int u = 0; u < kTailOvf; ++u) g[u] = u < nov ? lds[it.ov[u].idx] : static_cast<TO>(0);
-#pragma unroll
-            for (int u = 0; u < kTailOvf; ++u)
-                if (u < nov) part = fma(static_cast<TO>(it.ov[u].val), g[u], part);
-            s = kTailOvf;
-        }
-        for (; s + 8 <= nov; s += 8) {
-            entry32 e[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) e[u] = td.ovf_ent[(o + static_cast<size_t>(s + u)) * kWave + lane];
+                for (int u = 0; u < 8; ++u)
+                    if (sn + u < nov) nx[u] = td.ovf_ent[(o + static_cast<size_t>(sn + u)) * kWave + lane];
             TO g[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) g[u] = lds[e[u].idx];
+            for (int u = 0; u < 8; ++u) g[u] = s + u < nov ? lds[e[u].idx] : static_cast<TO>(0);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) part = fma(static_cast<TO>(e[u].val), g[u], part);
-        }
-        for (; s < nov; ++s) {
-            const entry32 e = td.ovf_ent[(o + static_cast<size_t>(s)) * kWave + lane];
-            part = fma(static_cast<TO>(e.val), lds[e.idx], part);
+            for (int u = 0; u < 8; ++u)
+                if (s + u < nov) part = fma(static_cast<TO>(e[u].val), g[u], part);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) e[u] = nx[u];
+            s = sn;
         }
     }
 #pragma unroll
@@ -599,12 +581,10 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     int lb = 0, le = level_first[l0 + 1] - bfirst;
     tail_item<TO> cur;
     tail_fetch<TO>(td, tv, min(base + w, last), cur);
-    tail_fetch_ovf<TO>(td, cur);
     for (int l = l0; l < l1; ++l) {
         const int ne = level_first[l + 2 <= l1 ? l + 2 : l1] - bfirst;
         int j = w;
-        // No batch in this level: fetch the next level's (its overflow
-        // entries are then loaded at its step, not here before the barrier).
+        // No batch in this level: fetch the next level's.
         if (lb + j >= le) tail_fetch<TO>(td, tv, min(base + le + w, last), cur);
         while (lb + j < le) {
             const int jn = j + kTailWaves;
@@ -612,7 +592,6 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
             tail_item<TO> nxt;
             tail_fetch<TO>(td, tv, min(tn, last), nxt);
             tail_step<TO>(td, base + lb + j, cur, lds);
-            tail_fetch_ovf<TO>(td, nxt);
             cur = nxt;
             j = jn;
         }
