@@ -110,6 +110,7 @@ inline constexpr int kTailPairs = 4;
 inline constexpr int kTailBlock = 1024;                 // threads of the tail workgroup
 inline constexpr int kTailWaves = kTailBlock / kWave;   // waves of the tail workgroup
 inline constexpr int kTailMaxBatches = 4 * kTailWaves;  // max batches of a tail level
+inline constexpr int kBlockTailMaxWaves = 4 * kTailWaves;  // max table waves of a blocked tail level
 struct alignas(16) tail_lane {
     std::int32_t row;    // output row the lane writes, or -1
     std::int32_t slot;   // the row's LDS slot (position minus the run's first)

@@ -1256,14 +1256,19 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
                 }
                 lg[l] = gm > kWave ? 0 : gm;
             }
-            // Segments: runs of levels of at most kTailWaves waves whose rows
-            // fit the LDS go to one workgroup; the others are launched.
+            // Segments: runs of levels of at most block_tail_waves table
+            // waves whose rows fit the LDS go to one workgroup; the others
+            // are launched.
             std::vector<int>(&seg)[3] = backward ? t->b_seg : t->f_seg;
             const int cap = sweep_block_tail_max_rows(dev_->prec, kp);
             const bool tails = opts_.schedule == sweep_schedule::hybrid && cap > 0;
+            // (16 by default: wider levels -- long rows times the block's
+            // columns -- ran slower walked by one workgroup than launched.)
+            const int block_tail_waves =
+                std::clamp(env_flag("APXCHOL_AMD_BLOCK_TAIL_WAVES", kTailWaves), 1, kBlockTailMaxWaves);
             const int nl = static_cast<int>(L);
             for (int l = 0; l < nl;) {
-                auto small = [&](int q) { return tails && lf[q + 1] - lf[q] <= kTailWaves; };
+                auto small = [&](int q) { return tails && lf[q + 1] - lf[q] <= block_tail_waves; };
                 if (!small(l)) {
                     seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
                     ++l;

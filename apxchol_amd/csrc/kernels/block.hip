@@ -53,6 +53,14 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
     const int c = mask ? sub0 / G : wcol0[w] + lane / G;
     const int s = sub0 & (G - 1);
     const bool active = mask ? sub0 < G * kp : true;
+    // The writer lane's right-hand side, diagonal and shift, loaded with the
+    // entries (independent of the gathers).
+    const bool writer = active && s == 0;
+    const size_t o = static_cast<size_t>(row) * kp + c;
+    const double rin = writer ? static_cast<double>(rhs[o]) : 0.0;
+    const float dg = writer ? d.diag[row] : 1.0f;
+    const double shift = (writer && !d.backward && st) ? st[c].shift : 0.0;
+    const double sc = (writer && d.backward && d.inv_scale) ? static_cast<double>(d.inv_scale[row]) : 0.0;
     TO part = 0;
     if (__builtin_expect(len > static_cast<unsigned>(kWave * kEpl), 0)) {
         // Long row (G = 64, one column per wave): sweep_batch's chunked walk.
@@ -78,19 +86,12 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
         const TO o = __shfl_xor(part, off, kWave);
         if (off < G) part += o;
     }
-    if (!active || s != 0) return;
-    const size_t o = static_cast<size_t>(row) * kp + c;
+    if (!writer) return;
     TO r;
-    if (!d.backward) {
-        const double shift = st ? st[c].shift : 0.0;
-        r = static_cast<TO>(static_cast<double>(rhs[o]) - shift);
-    } else if (d.inv_scale) {
-        const double sc = static_cast<double>(d.inv_scale[row]);
-        r = static_cast<TO>(static_cast<double>(rhs[o]) * (sc * sc));
-    } else {
-        r = static_cast<TO>(rhs[o]);
-    }
-    out[o] = (r - part) / static_cast<TO>(d.diag[row]);
+    if (!d.backward) r = static_cast<TO>(rin - shift);
+    else if (d.inv_scale) r = static_cast<TO>(rin * (sc * sc));
+    else r = static_cast<TO>(rin);
+    out[o] = (r - part) / static_cast<TO>(dg);
 }
 
 // One level of a blocked sweep with one thread per (row, column) (rows of at
@@ -162,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void sweep_block_rows(sweep_desc d, int p0,
 
 // A run of small levels of a blocked sweep, walked by ONE workgroup of
 // kTailBlock threads: levels [l0, l1) of the wave table (each at most
-// kTailWaves waves), sweep positions [p0, p1). The run's outputs live in LDS
+// kBlockTailMaxWaves table waves; wave w of the workgroup takes table waves
+// w, w + kTailWaves, ...), sweep positions [p0, p1). The run's outputs live in LDS
 // ([p1 - p0][kp]) as well as in `out`; a dependency inside the run is read
 // from LDS after the level barrier, one before the run from `out` (final
 // when the kernel starts). The per-(row, column) arithmetic is sweep_block's.
@@ -179,8 +181,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, con
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
     for (int l = l0; l < l1; ++l) {
-        const int w = level_first[l] + wv;
-        if (w < level_first[l + 1]) {
+        for (int w = level_first[l] + wv; w < level_first[l + 1]; w += kTailWaves) {
             const unsigned p = wpos[w];
             const unsigned long long mask = wmask[w];
             int pos, sub0;

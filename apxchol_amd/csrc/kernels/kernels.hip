@@ -350,7 +350,7 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
             const size_t lds = static_cast<size_t>(p1 - p0) * b.kp * sizeof(TO);
             if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: blocked tail run exceeds LDS");
             for (int l = l0; l < l1; ++l)
-                if (b.level_first[l + 1] - b.level_first[l] > kTailWaves)
+                if (b.level_first[l + 1] - b.level_first[l] > kBlockTailMaxWaves)
                     throw std::runtime_error("apxchol_amd: blocked tail level wider than the workgroup");
             sweep_block_tail<TI, TO, F16><<<1, kTailBlock, lds, s>>>(
                 d, b.pos, b.mask, b.col0, b.level_first_dev, l0, l1, p0, b.kp, static_cast<const TI*>(rhs),
