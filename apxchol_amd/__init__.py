@@ -18,7 +18,7 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["Solver", "Factor", "factorize", "solve", "SolveResult", "load_library", "ApxError"]
+__all__ = ["Solver", "Factor", "factorize", "laplacian", "solve", "SolveResult", "load_library", "ApxError"]
 
 _LIB_PATH = Path(__file__).resolve().parent / "libapxchol_amd.so"
 _lib = None
@@ -98,6 +98,19 @@ _SIGS = {
     "apx_solver_set_timing": ([P, I32], ctypes.c_int),
     "apx_solver_take_timing": ([P, P, P], ctypes.c_int),
     "apx_solve": ([I64, P, P, P, P, P, P, P], ctypes.c_int),
+    "apx_mtx_read": ([ctypes.c_char_p, P, P], ctypes.c_int),
+    "apx_matrix_create": ([I64, P, P, P, P], ctypes.c_int),
+    "apx_matrix_size": ([P, P, P, P], ctypes.c_int),
+    "apx_matrix_export": ([P, P, P, P], ctypes.c_int),
+    "apx_matrix_destroy": ([P], None),
+    "apx_mtx_write": ([ctypes.c_char_p, P, I32], ctypes.c_int),
+    "apx_mtx_read_vector": ([ctypes.c_char_p, P, I64, P], ctypes.c_int),
+    "apx_mtx_write_vector": ([ctypes.c_char_p, P, I64], ctypes.c_int),
+    "apx_input_scan_get": ([P, P], ctypes.c_int),
+    "apx_input_resolve": ([I32, P, I32, P, ctypes.c_char_p, I64], ctypes.c_int),
+    "apx_adjacency_to_laplacian": ([P, P], ctypes.c_int),
+    "apx_input_describe": ([I32, P, ctypes.c_char_p, ctypes.c_char_p, I64], ctypes.c_int),
+    "apx_input_diagnose": ([I32, P, ctypes.c_char_p, I64], ctypes.c_int),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -248,6 +261,15 @@ class Factor:
 
 def factorize(A, *, seed: int = 42) -> Factor:
     return Factor(A, seed=seed)
+
+
+def laplacian(A):
+    """The graph Laplacian L = D - |A| of a weighted ADJACENCY matrix, as the
+    reference package's ``apxchol.laplacian`` (python/apxchol/__init__.py:50-100):
+    self-loops and explicit zeros dropped, duplicates summed, every vertex
+    with an explicit diagonal entry (0 for an isolated one). Host-only."""
+    from .mtx import adjacency_to_laplacian
+    return adjacency_to_laplacian(A)
 
 
 class Solver:

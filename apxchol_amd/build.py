@@ -23,7 +23,8 @@ LIB = ROOT / "libapxchol_amd.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host/factorize.cpp", "host/trsv_prep.cpp", "host/gpu_solver.cpp", "host/server.cpp", "capi.cpp"]
+HOST_SOURCES = ["host/factorize.cpp", "host/trsv_prep.cpp", "host/gpu_solver.cpp", "host/server.cpp", "host/mtx.cpp",
+                "capi.cpp"]
 KERNEL_SOURCES = ["kernels/kernels.hip"]
 
 HOST_FLAGS = ["-std=c++20", "-O3", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",

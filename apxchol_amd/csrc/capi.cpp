@@ -3,13 +3,16 @@
 // thread-local message.
 #include "../../include/apxchol_amd.h"
 
+#include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
 
 #include "host/factor.hpp"
 #include "host/gpu_solver.hpp"
+#include "host/mtx.hpp"
 #include "host/server.hpp"
 #include "host/trsv_prep.hpp"
 
@@ -23,6 +26,9 @@ struct apx_solver_s {
 };
 struct apx_server_s {
     std::unique_ptr<solve_server> s;
+};
+struct apx_matrix_s {
+    csc_matrix A;
 };
 
 namespace {
@@ -475,6 +481,151 @@ apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* i
         if (vals) std::memcpy(vals, F.vals.data(), F.vals.size() * sizeof(float));
         if (perm) std::memcpy(perm, F.perm.data(), F.perm.size() * sizeof(uint32_t));
     });
+}
+
+// ---- MatrixMarket files and input interpretation ----
+
+namespace {
+input_scan to_scan(const apx_input_scan* s) {
+    need(s, "scan");
+    input_scan r;
+    r.n = s->n; r.offdiag_nnz = s->offdiag_nnz; r.offdiag_pos = s->offdiag_pos; r.offdiag_neg = s->offdiag_neg;
+    r.upper_nnz = s->upper_nnz; r.lower_nnz = s->lower_nnz; r.diag_pos = s->diag_pos;
+    r.deficient_rows = s->deficient_rows; r.worst_deficit = s->worst_deficit; r.worst_row = s->worst_row;
+    r.excess_rows = s->excess_rows;
+    return r;
+}
+input_kind to_kind(int32_t k) {
+    if (k < 0 || k > 2) throw std::invalid_argument("input kind must be 0 (automatic), 1 (laplacian) or 2 (adjacency)");
+    return static_cast<input_kind>(k);
+}
+void copy_text(const std::string& t, char* out, int64_t len) {
+    if (!out || len <= 0) return;
+    const std::size_t k = std::min<std::size_t>(t.size(), static_cast<std::size_t>(len - 1));
+    std::memcpy(out, t.data(), k);
+    out[k] = '\0';
+}
+}  // namespace
+
+apx_status apx_mtx_read(const char* path, apx_matrix_t* out, int32_t* pattern_field) {
+    return guarded([&] {
+        need(path, "path");
+        need(out, "out");
+        mtx_header h;
+        auto m = std::make_unique<apx_matrix_s>();
+        m->A = read_mtx(path, &h);
+        if (pattern_field) *pattern_field = h.pattern ? 1 : 0;
+        *out = m.release();
+    });
+}
+
+apx_status apx_matrix_create(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
+                             apx_matrix_t* out) {
+    return guarded([&] {
+        need(out, "out");
+        const csc_view v = make_view(n, colptr, rowidx, vals);
+        auto m = std::make_unique<apx_matrix_s>();
+        m->A.rows = m->A.cols = n;
+        m->A.colptr.assign(v.colptr, v.colptr + n + 1);
+        if (m->A.colptr[0] != 0) throw std::invalid_argument("colptr[0] must be 0");
+        for (int64_t j = 0; j < n; ++j)
+            if (m->A.colptr[j + 1] < m->A.colptr[j]) throw std::invalid_argument("colptr must be non-decreasing");
+        const int64_t nnz = m->A.colptr[n];
+        m->A.rowidx.assign(v.rowidx, v.rowidx + nnz);
+        m->A.vals.assign(v.vals, v.vals + nnz);
+        for (const int32_t i : m->A.rowidx)
+            if (i < 0 || i >= n) throw std::invalid_argument("row index out of range");
+        *out = m.release();
+    });
+}
+
+apx_status apx_matrix_size(apx_matrix_t m, int64_t* rows, int64_t* cols, int64_t* nnz) {
+    return guarded([&] {
+        need(m, "matrix");
+        if (rows) *rows = m->A.rows;
+        if (cols) *cols = m->A.cols;
+        if (nnz) *nnz = static_cast<int64_t>(m->A.rowidx.size());
+    });
+}
+
+apx_status apx_matrix_export(apx_matrix_t m, int64_t* colptr, int32_t* rowidx, double* vals) {
+    return guarded([&] {
+        need(m, "matrix");
+        if (colptr) std::memcpy(colptr, m->A.colptr.data(), m->A.colptr.size() * sizeof(int64_t));
+        if (rowidx) std::memcpy(rowidx, m->A.rowidx.data(), m->A.rowidx.size() * sizeof(int32_t));
+        if (vals) std::memcpy(vals, m->A.vals.data(), m->A.vals.size() * sizeof(double));
+    });
+}
+
+void apx_matrix_destroy(apx_matrix_t m) { delete m; }
+
+apx_status apx_mtx_write(const char* path, apx_matrix_t m, int32_t symmetric) {
+    return guarded([&] {
+        need(path, "path");
+        need(m, "matrix");
+        write_mtx(path, m->A, symmetric != 0);
+    });
+}
+
+apx_status apx_mtx_read_vector(const char* path, double* x, int64_t cap, int64_t* n) {
+    return guarded([&] {
+        need(path, "path");
+        need(n, "n");
+        const std::vector<double> v = read_mtx_vector(path);
+        *n = static_cast<int64_t>(v.size());
+        if (x && cap >= *n && !v.empty()) std::memcpy(x, v.data(), v.size() * sizeof(double));
+    });
+}
+
+apx_status apx_mtx_write_vector(const char* path, const double* x, int64_t n) {
+    return guarded([&] {
+        need(path, "path");
+        if (n < 0) throw std::invalid_argument("n must be >= 0");
+        if (n > 0) need(x, "x");
+        write_mtx_vector(path, x, n);
+    });
+}
+
+apx_status apx_input_scan_get(apx_matrix_t m, apx_input_scan* s) {
+    return guarded([&] {
+        need(m, "matrix");
+        need(s, "scan");
+        if (m->A.rows != m->A.cols) throw std::invalid_argument("matrix must be square");
+        const input_scan r = scan_input(m->A.view());
+        s->n = r.n; s->offdiag_nnz = r.offdiag_nnz; s->offdiag_pos = r.offdiag_pos; s->offdiag_neg = r.offdiag_neg;
+        s->upper_nnz = r.upper_nnz; s->lower_nnz = r.lower_nnz; s->diag_pos = r.diag_pos;
+        s->deficient_rows = r.deficient_rows; s->worst_deficit = r.worst_deficit; s->worst_row = r.worst_row;
+        s->excess_rows = r.excess_rows;
+    });
+}
+
+apx_status apx_input_resolve(int32_t requested, const apx_input_scan* s, int32_t pattern_field, int32_t* kind,
+                             char* reason, int64_t reason_len) {
+    return guarded([&] {
+        need(kind, "kind");
+        std::string why;
+        *kind = static_cast<int32_t>(resolve_input_kind(to_kind(requested), to_scan(s), pattern_field != 0, why));
+        copy_text(why, reason, reason_len);
+    });
+}
+
+apx_status apx_adjacency_to_laplacian(apx_matrix_t a, apx_matrix_t* out) {
+    return guarded([&] {
+        need(a, "matrix");
+        need(out, "out");
+        if (a->A.rows != a->A.cols) throw std::invalid_argument("matrix must be square");
+        auto m = std::make_unique<apx_matrix_s>();
+        m->A = adjacency_to_laplacian(a->A.view());
+        *out = m.release();
+    });
+}
+
+apx_status apx_input_describe(int32_t kind, const apx_input_scan* s, const char* reason, char* out, int64_t out_len) {
+    return guarded([&] { copy_text(describe_input(to_kind(kind), to_scan(s), reason ? reason : ""), out, out_len); });
+}
+
+apx_status apx_input_diagnose(int32_t kind, const apx_input_scan* s, char* out, int64_t out_len) {
+    return guarded([&] { copy_text(diagnose_failure(to_kind(kind), to_scan(s)), out, out_len); });
 }
 
 apx_status apx_solve(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, const double* b,

@@ -249,6 +249,48 @@ apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* i
 apx_status apx_solve(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, const double* b,
                      double* x, const apx_options* opts, apx_solve_info* info);
 
+/* ---- MatrixMarket files and the reading of an input matrix -----------------
+ * The reference CLI's input layer (src/main.cpp:14-98 with the
+ * fast_matrix_market reader; src/mtx_input.h:36-120): read a .mtx file, decide
+ * whether it holds an assembled Laplacian/SDDM operator or a graph adjacency
+ * matrix, and assemble L = D - A for the latter. Host-only. */
+typedef struct apx_matrix_s* apx_matrix_t; /* host CSC matrix, rows ascending in each column */
+/* Reads a `matrix coordinate` file (real / integer / pattern, general /
+ * symmetric; symmetric files mirrored, duplicates summed). pattern_field: 1 for
+ * a `pattern` file (may be null). */
+apx_status apx_mtx_read(const char* path, apx_matrix_t* out, int32_t* pattern_field);
+apx_status apx_matrix_create(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
+                             apx_matrix_t* out);
+apx_status apx_matrix_size(apx_matrix_t m, int64_t* rows, int64_t* cols, int64_t* nnz);
+apx_status apx_matrix_export(apx_matrix_t m, int64_t* colptr, int32_t* rowidx, double* vals);
+void apx_matrix_destroy(apx_matrix_t m);
+/* Writes m (symmetric = 1: lower triangle with a `symmetric` header), 17 digits. */
+apx_status apx_mtx_write(const char* path, apx_matrix_t m, int32_t symmetric);
+/* Dense vectors (`matrix array` n x 1, or coordinate n x 1). Reading: *n gets
+ * the length; x (capacity cap) is filled when cap >= *n. */
+apx_status apx_mtx_read_vector(const char* path, double* x, int64_t cap, int64_t* n);
+apx_status apx_mtx_write_vector(const char* path, const double* x, int64_t n);
+
+/* input_scan (src/mtx_input.h:46-73). */
+typedef struct {
+    int64_t n, offdiag_nnz, offdiag_pos, offdiag_neg, upper_nnz, lower_nnz, diag_pos, deficient_rows;
+    double worst_deficit;
+    int64_t worst_row, excess_rows;
+} apx_input_scan;
+/* Input kinds: 0 automatic, 1 laplacian (assembled operator), 2 adjacency. */
+apx_status apx_input_scan_get(apx_matrix_t m, apx_input_scan* s);
+/* resolve_input_kind (mtx_input.h:76-90): *kind = 1 or 2 and the reason
+ * (truncated to reason_len bytes, NUL included); a one-sided or ambiguous
+ * matrix returns APX_ERR_RUNTIME with the explanation in apx_last_error(). */
+apx_status apx_input_resolve(int32_t requested, const apx_input_scan* s, int32_t pattern_field, int32_t* kind,
+                             char* reason, int64_t reason_len);
+/* adjacency_to_laplacian (mtx_input.h:92-97): *out = D - |A| with self-loops
+ * dropped and an explicit diagonal entry for every vertex. */
+apx_status apx_adjacency_to_laplacian(apx_matrix_t a, apx_matrix_t* out);
+/* describe_input / diagnose_failure (mtx_input.h:99-120), truncated to out_len. */
+apx_status apx_input_describe(int32_t kind, const apx_input_scan* s, const char* reason, char* out, int64_t out_len);
+apx_status apx_input_diagnose(int32_t kind, const apx_input_scan* s, char* out, int64_t out_len);
+
 #ifdef __cplusplus
 }
 #endif
