@@ -172,7 +172,7 @@ def main() -> int:
     ap.add_argument("--batch-workload", default="aniso3d_64",
                     help="second blocked workload (default: the 64^3 anisotropic grid; empty: none)")
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
-    ap.add_argument("--serving-qps", type=float, default=50.0)
+    ap.add_argument("--serving-qps", type=float, default=25.0)
     ap.add_argument("--serving-slots", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibration of the HBM byte counters)")
