@@ -43,7 +43,8 @@ class Options(ctypes.Structure):
                 ("precision", I32), ("fp16_storage", I32), ("schedule", I32), ("drop_rel", F64),
                 ("keep_factor", I32), ("iters_per_check", I32), ("sweep_chunk", I32), ("sweep_blocks", I32),
                 ("tail_rows", I32), ("graph", I32), ("relabel", I32),
-                ("lane_batches", I32), ("depth_levels", I32), ("tail_batches", I32)]
+                ("lane_batches", I32), ("depth_levels", I32), ("tail_batches", I32),
+                ("tail_cut", I32)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -173,7 +174,7 @@ def _csc(A):
 def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
              schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=4, sweep_chunk=0,
              sweep_blocks=0, tail_rows=0, graph=True, relabel=True, lane_batches=0,
-             depth_levels=True, tail_batches=0) -> Options:
+             depth_levels=True, tail_batches=0, tail_cut=0) -> Options:
     o = Options()
     load_library().apx_options_default(ctypes.byref(o))
     o.tol = float(tol)
@@ -194,6 +195,7 @@ def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp
     o.lane_batches = int(lane_batches)
     o.depth_levels = 1 if depth_levels else 0
     o.tail_batches = int(tail_batches)
+    o.tail_cut = int(tail_cut)
     return o
 
 

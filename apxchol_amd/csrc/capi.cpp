@@ -92,6 +92,7 @@ solve_options to_opts(const apx_options* o) {
     so.lane_batches = o->lane_batches;
     so.depth_levels = o->depth_levels != 0;
     so.tail_batches = o->tail_batches;
+    so.tail_cut = o->tail_cut;
     return so;
 }
 
@@ -132,6 +133,7 @@ void apx_options_default(apx_options* o) {
     o->lane_batches = 0;
     o->depth_levels = 1;
     o->tail_batches = 0;
+    o->tail_cut = 0;
 }
 
 const char* apx_last_error(void) { return g_err.c_str(); }

@@ -123,11 +123,22 @@ struct alignas(16) tail_pair {
     std::int32_t i1;
     float v1;
 };
+// The walk of a run: for every tail batch (index inside its run) the same
+// wave's next batch in walk order (or -1) and the batch's level inside the
+// run, its wave, and whether it is that wave's first batch.
+inline constexpr int kTailWalkWaveShift = 20;
+inline constexpr int kTailWalkFirst = 1 << 24;
+inline constexpr int kTailWalkMaxLevels = 1 << kTailWalkWaveShift;
+struct alignas(8) tail_walk {
+    std::int32_t next;
+    std::int32_t info;   // level | wave << kTailWalkWaveShift | kTailWalkFirst
+};
 struct tail_desc {
     const tail_lane* lane = nullptr;
     const tail_pair* ent = nullptr;
     const std::int32_t* ovf = nullptr;
     const entry32* ovf_ent = nullptr;
+    const tail_walk* walk = nullptr;
 };
 // Static lane layout of a launched level (sweep_lanes): everything a lane of
 // a batch needs sits at an address computed from (lane batch, lane), so one

@@ -418,6 +418,24 @@ int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, 
     return base;
 }
 
+void append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt) {
+    const int b0 = level_first[l], nb = level_first[e] - b0;
+    if (nb <= 0 || e - l > kTailWalkMaxLevels) throw std::runtime_error("apxchol_amd: invalid tail walk");
+    const std::size_t at = tt.walk.size();
+    tt.walk.resize(at + static_cast<std::size_t>(nb));
+    int prev[kTailWaves];
+    std::fill(prev, prev + kTailWaves, -1);
+    for (int q = l; q < e; ++q)
+        for (int b = level_first[q]; b < level_first[q + 1]; ++b) {
+            const int i = b - b0, w = (b - level_first[q]) % kTailWaves;
+            tail_walk& x = tt.walk[at + static_cast<std::size_t>(i)];
+            x.next = -1;
+            x.info = (q - l) | (w << kTailWalkWaveShift) | (prev[w] < 0 ? kTailWalkFirst : 0);
+            if (prev[w] >= 0) tt.walk[at + static_cast<std::size_t>(prev[w])].next = i;
+            prev[w] = i;
+        }
+}
+
 long long append_lane_level(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl,
                             lane_tables& lt) {
     const auto& idx = backward ? T.b_idx : T.f_idx;
@@ -466,7 +484,7 @@ struct gpu_solver::device_state {
     dbuf f_pos, f_mask, b_pos, b_mask;
     std::vector<int> f_levels, b_levels;
     dbuf f_levels_dev, b_levels_dev, tail_val;
-    dbuf f_tlane, f_tent, f_tovf, f_tovf_ent, b_tlane, b_tent, b_tovf, b_tovf_ent;
+    dbuf f_tlane, f_tent, f_tovf, f_tovf_ent, f_twalk, b_tlane, b_tent, b_tovf, b_tovf_ent, b_twalk;
     dbuf f_lrec, f_lent, b_lrec, b_lent;
     std::vector<int> f_level_lb, b_level_lb;   // first lane batch of each level, or -1
     std::vector<int> f_seg[6], b_seg[6];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch
@@ -806,20 +824,25 @@ void gpu_solver::setup(const csc_view& L) {
     // other (small) levels becomes one tail segment.
     const int tail_rows = std::min(opts_.tail_rows > 0 ? opts_.tail_rows : env_int("APXCHOL_AMD_TAIL_ROWS", 2048),
                                    sweep_tail_max_rows(d.prec));
-    // A tail run's outputs live in LDS: a run longer than that holds is split.
-    const int tail_cap = sweep_tail_max_rows(d.prec);
+    // A tail run's outputs (and its level table) live in LDS: a run longer
+    // than that holds is split.
+    const vec_prec tail_prec = d.prec;
     // Batches a tail level may hold (up to kTailMaxBatches: each of the
     // workgroup's waves takes every kTailWaves-th batch).
     const int level_tail_batches = std::clamp(opts_.tail_batches > 0 ? opts_.tail_batches
                                                               : env_flag("APXCHOL_AMD_TAIL_BATCHES", 2 * kTailWaves),
                                         1, kTailMaxBatches);
+    // A run ends before a level whose longest lane would read at least this
+    // many overflow groups inside the run (0: runs end only at launched levels
+    // and at the LDS capacity).
+    const int tail_cut = opts_.tail_cut != 0 ? std::max(opts_.tail_cut, 0) : env_flag("APXCHOL_AMD_TAIL_CUT", 1);
     // Launched levels of at most lane_max batches (and no row longer than a
     // wave's chunk) read the static lane layout.
     const int lane_max = opts_.lane_batches < 0 ? 0
                          : opts_.lane_batches > 0 ? opts_.lane_batches
                                                   : env_int("APXCHOL_AMD_LANE_BATCHES", 4096);
     auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
-                          dbuf& lf_dev, std::vector<int> (&seg)[6], dbuf* const (&tbuf)[4], std::vector<int>& level_lb,
+                          dbuf& lf_dev, std::vector<int> (&seg)[6], dbuf* const (&tbuf)[5], std::vector<int>& level_lb,
                           dbuf& lrec, dbuf& lent) {
         lf_dev.upload(lf);
         tail_tables tt;
@@ -828,6 +851,31 @@ void gpu_solver::setup(const csc_view& L) {
         const auto rows = [&](int q) { return pos(q + 1) - pos(q); };
         // A tail level has at most one batch per wave of the tail workgroup.
         const auto small = [&](int q) { return rows(q) < tail_rows && lf[q + 1] - lf[q] <= level_tail_batches; };
+        // Overflow groups of level q inside a run starting at position p0: a
+        // long row's lane holds every 64th entry, and the tail reads a lane's
+        // entries past the first 2 * kTailPairs from HBM in dependent groups
+        // of 8 while one wave walks the row. A level whose longest such chain
+        // has at least tail_cut groups starts a new run instead: its entries
+        // then point before the run and are summed by the coupling launch,
+        // which spreads the rows over the whole device.
+        const auto& sptr = backward ? T_.b_ptr : T_.f_ptr;
+        const auto& sidx = backward ? T_.b_idx : T_.f_idx;
+        const auto ovf_groups = [&](int q, int p0) {
+            int worst = 0;
+            for (int t = pos(q); t < pos(q + 1); ++t) {
+                const int row = backward ? m - 1 - t : t;
+                const std::uint32_t b0 = sptr[row], b1 = sptr[row + 1];
+                if (b1 - b0 <= static_cast<std::uint32_t>(kWave * 2 * kTailPairs)) continue;
+                // Entries are in ascending sweep position: the inside ones are a suffix.
+                const auto first = std::partition_point(sidx.begin() + b0, sidx.begin() + b1, [&](std::int32_t j) {
+                    return (backward ? m - 1 - j : j) < p0;
+                });
+                const std::uint32_t inside = static_cast<std::uint32_t>(sidx.begin() + b1 - first);
+                const int per_lane = static_cast<int>((inside + kWave - 1) / kWave);
+                worst = std::max(worst, (per_lane - 2 * kTailPairs + 7) / 8);
+            }
+            return worst;
+        };
         for (int l = 0; l < L;) {
             if (!small(l)) {
                 seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
@@ -836,26 +884,34 @@ void gpu_solver::setup(const csc_view& L) {
                 continue;
             }
             int e = l;
-            while (e < L && small(e) && pos(e + 1) - pos(l) <= tail_cap) ++e;
+            while (e < L && small(e) && sweep_tail_fits(tail_prec, pos(e + 1) - pos(l), lf[e + 1] - lf[l], e + 1 - l) &&
+                   (e == l || tail_cut <= 0 || ovf_groups(e, pos(l)) < tail_cut))
+                ++e;
             if (e == l) ++e;   // a single small level always fits (rows < tail_rows <= cap)
             seg[0].push_back(l); seg[1].push_back(e); seg[2].push_back(1);
             seg[3].push_back(pos(l)); seg[4].push_back(pos(e));
-            seg[5].push_back(opts_.schedule == sweep_schedule::hybrid ? append_tail_run(T_, bt, backward, lf[l], lf[e], tt)
-                                                                      : -1);
+            if (opts_.schedule == sweep_schedule::hybrid) {
+                seg[5].push_back(append_tail_run(T_, bt, backward, lf[l], lf[e], tt));
+                append_tail_walk(lf, l, e, tt);
+            } else {
+                seg[5].push_back(-1);
+            }
             l = e;
         }
         const std::size_t tb = static_cast<std::size_t>(tt.batches);
         if (tt.lane.size() != tb * kWave || tt.ent.size() != tb * kTailPairs * kWave || tt.ovf.size() != tb ||
-            tt.ovf_ent.size() % kWave != 0)
+            tt.ovf_ent.size() % kWave != 0 || tt.walk.size() != tb)
             throw std::runtime_error("apxchol_amd: inconsistent tail layout");
         tbuf[0]->upload(tt.lane);
         tbuf[1]->upload(tt.ent);
         tbuf[2]->upload(tt.ovf);
         tbuf[3]->upload(tt.ovf_ent);
+        tbuf[4]->upload(tt.walk);
         sc.tail.lane = tbuf[0]->as<tail_lane>();
         sc.tail.ent = tbuf[1]->as<tail_pair>();
         sc.tail.ovf = tbuf[2]->as<std::int32_t>();
         sc.tail.ovf_ent = tbuf[3]->as<entry32>();
+        sc.tail.walk = tbuf[4]->as<tail_walk>();
         sc.tail_batches = tt.batches;
         sc.seg_base = seg[5].data();
         sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
@@ -931,8 +987,8 @@ void gpu_solver::setup(const csc_view& L) {
         sc.nseg = static_cast<int>(seg[0].size());
         sc.grid = d.grid;
     };
-    dbuf* f_tb[4] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent};
-    dbuf* b_tb[4] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent};
+    dbuf* f_tb[5] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk};
+    dbuf* b_tb[5] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk};
     make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb, d.f_level_lb, d.f_lrec, d.f_lent);
     make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb, d.b_level_lb, d.b_lrec, d.b_lent);
     // Coupling output, shared by the two directions (their sweeps never overlap).
@@ -964,8 +1020,8 @@ void gpu_solver::setup(const csc_view& L) {
     for (const dbuf* b : {&d.f_ptr, &d.f_ent, &d.f_idx, &d.f_h, &d.b_ptr, &d.b_ent, &d.b_idx, &d.b_h, &d.diag,
                           &d.inv_scale, &d.f_pos, &d.f_mask, &d.b_pos, &d.b_mask, &d.perm, &d.op_ptr, &d.op_idx,
                           &d.op_val, &d.r, &d.x, &d.p, &d.ap, &d.rp, &d.y, &d.z, &d.xout, &d.f_lrec, &d.f_lent,
-                          &d.b_lrec, &d.b_lent, &d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.b_tlane,
-                          &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.tail_val, &d.relabel, &d.sell_soff,
+                          &d.b_lrec, &d.b_lent, &d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk,
+                          &d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk, &d.tail_val, &d.relabel, &d.sell_soff,
                           &d.sell_idx, &d.sell_val, &d.sell_map, &d.sell_long})
         bytes += b->bytes();
     stats_.device_bytes = bytes;

@@ -29,6 +29,8 @@ struct solve_options {
     sweep_schedule schedule = sweep_schedule::hybrid;
     int tail_rows = 0;            // hybrid: levels below this many rows go to the one-workgroup tail (0: default)
     int tail_batches = 0;         // hybrid: ... and of at most this many batches (0: APXCHOL_AMD_TAIL_BATCHES or 32; <= 64)
+    int tail_cut = 0;             // hybrid: a run ends before a level needing this many overflow groups per lane
+                                  // (0: APXCHOL_AMD_TAIL_CUT or 4; < 0: never)
     double drop_rel = -1.0;       // < 0: APXCHOL_FACTOR_DROP / the 1e-4 default
     int iters_per_check = 4;      // iterations enqueued between completion tests
     int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 8)
@@ -237,12 +239,17 @@ struct tail_tables {
     std::vector<tail_pair> ent;
     std::vector<std::int32_t> ovf;
     std::vector<entry32> ovf_ent;
+    std::vector<tail_walk> walk;   // per tail batch
     int batches = 0;
 };
 // Appends the run of batches [bf, bl) (whole levels) and returns its first
 // tail batch. Throws if a lane's chain does not have the shape the tail
 // kernel relies on (entries before the run form a prefix of the chain).
 int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl, tail_tables& tt);
+// Appends the walk entries of the run of levels [l, e) (level_first: first
+// batch of each level): wave w of the tail workgroup takes batches w, w + 16,
+// ... of every level, in level order.
+void append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt);
 
 // Static lane layout of launched levels (device.hpp's lane_rec / entry32
 // planes), level by level.

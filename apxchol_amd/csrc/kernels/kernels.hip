@@ -87,7 +87,8 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         for (int k = 0; k < sc.nseg; ++k) {
             if (sc.seg_tail[k]) {
                 const int p0 = sc.seg_p0[k], p1 = sc.seg_p1[k];
-                const size_t lds = static_cast<size_t>(p1 - p0 + 1) * sizeof(TO);
+                const int bf = sc.level_first[sc.seg_l0[k]], bl = sc.level_first[sc.seg_l1[k]];
+                const size_t lds = tail_lds_bytes(p1 - p0, bl - bf, sizeof(TO));
                 if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: tail run exceeds LDS");
                 static const bool attr = [] {
                     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TO>),
@@ -95,7 +96,6 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                     return true;
                 }();
                 (void)attr;
-                const int bf = sc.level_first[sc.seg_l0[k]], bl = sc.level_first[sc.seg_l1[k]];
                 const int base = sc.seg_base[k];
                 if (bl <= bf || base < 0 || base + (bl - bf) > sc.tail_batches ||
                     static_cast<size_t>(sc.tail_batches) * kWave * sizeof(tail_val<TO>) > sc.tail_val_bytes)
@@ -105,9 +105,8 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                 launch_timed(hook, 2, sweep_couple<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, bf, bl, p0, in,
                              static_cast<const TO*>(o), tv + static_cast<size_t>(base) * kWave, st);
                 launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
-                             base, sc.level_first_dev, sc.seg_l0[k], sc.seg_l1[k], p1 - p0,
-                             d.backward ? d.m - 1 - p0 : p0, static_cast<int>(d.backward), o,
-                             static_cast<const tail_val<TO>*>(tv), st);
+                             base, bl - bf, sc.seg_l1[k] - sc.seg_l0[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
+                             static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st);
             } else {
                 for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
             }
@@ -125,6 +124,12 @@ int sweep_dataflow_grid(vec_prec p, bool fp16) {
 
 int sweep_tail_max_rows(vec_prec p) {
     return kTailLdsBytes / static_cast<int>(p == vec_prec::fp64 ? sizeof(double) : sizeof(float)) - 1;
+}
+
+bool sweep_tail_fits(vec_prec p, int rows, int batches, int levels) {
+    return levels <= kTailWalkMaxLevels &&
+           tail_lds_bytes(rows, batches, p == vec_prec::fp64 ? sizeof(double) : sizeof(float)) <=
+               static_cast<std::size_t>(kTailLdsBytes);
 }
 
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,

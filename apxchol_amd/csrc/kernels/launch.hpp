@@ -13,6 +13,8 @@ int sweep_dataflow_grid(vec_prec p, bool fp16);
 
 // Rows a one-workgroup tail run may hold (its outputs live in LDS).
 int sweep_tail_max_rows(vec_prec p);
+// Whether a tail run of `rows` rows, `batches` batches and `levels` levels fits the tail workgroup.
+bool sweep_tail_fits(vec_prec p, int rows, int batches, int levels);
 
 // How one sweep direction is scheduled.
 //   dataflow: one persistent launch of `grid` workgroups (polling);

@@ -378,6 +378,16 @@ __global__ __launch_bounds__(kBlock) void sweep_lanes(sweep_desc d, int lb0, int
 // traffic only. The per-row arithmetic is sweep_batch's (same chain order, fma
 // and butterfly), so the result is bit-identical to the other schedules.
 inline constexpr int kTailLdsBytes = 160 * 1024;
+// LDS of a tail run of `rows` rows and `batches` batches: the run's outputs
+// and the zero slot (rounded up to 8 bytes), the run's walk (one tail_walk
+// per batch) and every wave's first batch.
+inline constexpr std::size_t tail_slot_bytes(int rows, std::size_t vsz) {
+    return ((static_cast<std::size_t>(rows) + 1) * vsz + 7) / 8 * 8;
+}
+inline constexpr std::size_t tail_lds_bytes(int rows, int batches, std::size_t vsz) {
+    return tail_slot_bytes(rows, vsz) + static_cast<std::size_t>(batches) * sizeof(tail_walk) +
+           kTailWaves * sizeof(int);
+}
 
 // Workgroup barrier ordering LDS only: outstanding global loads stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -545,8 +555,11 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
             s = sn;
         }
     }
+    // Butterfly over the lane groups; a step no lane of the batch needs
+    // (every G <= off) is skipped as a whole, which leaves every sum as is.
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
+        if (__ballot(off < G) == 0) continue;
         const TO o = __shfl_xor(part, off, kWave);
         if (off < G) part += o;
     }
@@ -559,47 +572,82 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
 // output row of slot s: row0 + s (forward) or row0 - s (back). Outputs reach
 // HBM in one coalesced copy at the end.
 template <class TO>
-__global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, const int* __restrict__ level_first,
-                                                         int l0, int l1, int zslot, int row0, int backward, TO* out,
-                                                         const tail_val<TO>* tv, const pcg_state* st) {
+__global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, int nb, int nlev, int zslot,
+                                                         int row0, int backward, TO* out, const tail_val<TO>* tv,
+                                                         const pcg_state* st) {
     if (st && st->done) return;
     extern __shared__ unsigned char tail_lds[];
     TO* lds = reinterpret_cast<TO*>(tail_lds);
-    const int bfirst = level_first[l0];
-    const int last = base + level_first[l1] - bfirst - 1;
-    // The zero slot past the run's outputs, read by padding entries; every
-    // wave clears it before its own first read (all writers store 0).
-    lds[zslot] = TO(0);
-    // Wave w walks its batches in order (level by level); the item of its
-    // next batch -- the next one of this level, else batch w of the next
-    // level -- is fetched before the current one is computed, so its loads
-    // are in flight meanwhile and across the level barrier. Every fetch is
-    // issued (clamped to the run's last batch), so the compiler's memory
-    // counters see the same loads on every path. (readfirstlane: the wave
-    // index is wave-uniform, so the level bookkeeping stays scalar.)
-    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-    int lb = 0, le = level_first[l0 + 1] - bfirst;
-    tail_item<TO> cur;
-    tail_fetch<TO>(td, tv, min(base + w, last), cur);
-    for (int l = l0; l < l1; ++l) {
-        const int ne = level_first[l + 2 <= l1 ? l + 2 : l1] - bfirst;
-        int j = w;
-        // No batch in this level: fetch the next level's.
-        if (lb + j >= le) tail_fetch<TO>(td, tv, min(base + le + w, last), cur);
-        while (lb + j < le) {
-            const int jn = j + kTailWaves;
-            const int tn = lb + jn < le ? base + lb + jn : base + le + w;
-            tail_item<TO> nxt;
-            tail_fetch<TO>(td, tv, min(tn, last), nxt);
-            tail_step<TO>(td, base + lb + j, cur, lds);
-            cur = nxt;
-            j = jn;
-        }
-        lds_barrier();
-        lb = le;
-        le = ne;
+    // Past the outputs and the zero slot: the run's walk and every wave's
+    // first batch. The walk is read from LDS, so its lookups never wait
+    // behind the item loads in flight (global loads complete in order).
+    tail_walk* walk = reinterpret_cast<tail_walk*>(tail_lds + tail_slot_bytes(zslot, sizeof(TO)));
+    int* first = reinterpret_cast<int*>(walk + nb);
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < kTailWaves) first[tid] = -1;
+    // The zero slot past the run's outputs, read by padding entries (all
+    // writers store 0).
+    if (tid == 0) lds[zslot] = TO(0);
+    lds_barrier();
+    for (int q = tid; q < nb; q += kTailBlock) {
+        const tail_walk x = td.walk[base + q];
+        walk[q] = x;
+        if (x.info & kTailWalkFirst) first[(x.info >> kTailWalkWaveShift) & (kTailWaves - 1)] = q;
     }
-    for (int q = static_cast<int>(threadIdx.x); q < zslot; q += kTailBlock) out[backward ? row0 - q : row0 + q] = lds[q];
+    lds_barrier();
+    // Wave w walks its batches in order, passing the level barriers between
+    // them. Two items alternate: after the batch in one is computed, the one
+    // two batches ahead is fetched into it, so each item's loads have the
+    // next batch's step and every barrier in between to land (no register
+    // copy of an item waits on its loads). Every fetch is issued (clamped to
+    // the run's last batch), so the compiler's memory counters see the same
+    // loads on every path. Cursors are batch indices inside the run (-1: the
+    // walk is over) and wave-uniform (readfirstlane of LDS values read at one
+    // address), so the walk stays scalar.
+    const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const auto next_of = [&](int c) { return c < 0 ? -1 : __builtin_amdgcn_readfirstlane(walk[c].next); };
+    const auto level_of = [&](int c) {
+        return c < 0 ? nlev : (__builtin_amdgcn_readfirstlane(walk[c].info) & (kTailWalkMaxLevels - 1));
+    };
+    const auto batch_of = [&](int c) { return base + (c < 0 ? nb - 1 : c); };
+    int ca = __builtin_amdgcn_readfirstlane(first[w]);
+    int cb = next_of(ca);
+    int la = level_of(ca), lb = level_of(cb);
+    tail_item<TO> ia, ib;
+    tail_fetch<TO>(td, tv, batch_of(ca), ia);
+    tail_fetch<TO>(td, tv, batch_of(cb), ib);
+    // One exit and both fetches on every pass (a finished walk stays at -1),
+    // so the item loads in flight are the same wherever the passes meet.
+    int lvl = 0;   // the level whose rows are being computed
+    while (la < nlev) {
+        while (lvl < la) {
+            lds_barrier();
+            ++lvl;
+        }
+        tail_step<TO>(td, batch_of(ca), ia, lds);
+        ca = cb;
+        la = lb;
+        cb = next_of(cb);
+        lb = level_of(cb);
+        tail_fetch<TO>(td, tv, batch_of(cb), ia);
+        if (la < nlev) {
+            while (lvl < la) {
+                lds_barrier();
+                ++lvl;
+            }
+            tail_step<TO>(td, batch_of(ca), ib, lds);
+        }
+        ca = cb;
+        la = lb;
+        cb = next_of(cb);
+        lb = level_of(cb);
+        tail_fetch<TO>(td, tv, batch_of(cb), ib);
+    }
+    while (lvl < nlev) {
+        lds_barrier();
+        ++lvl;
+    }
+    for (int q = tid; q < zslot; q += kTailBlock) out[backward ? row0 - q : row0 + q] = lds[q];
 }
 
 // Fill a vector with the pending sentinel (the state every sweep output must

@@ -55,6 +55,10 @@ typedef struct {
                                 dependency graph; 0: the elimination rounds (round-as-level) */
     int32_t tail_batches;    /* hybrid: a tail level holds at most this many batches (0 = 32, max 64;
                                 each wave of the tail workgroup takes every 16th) */
+    int32_t tail_cut;        /* hybrid: a tail run ends before a level one of whose lanes would read
+                                at least this many groups of 8 entries inside the run from HBM
+                                (rows of thousands of entries); that level's entries are then summed
+                                by the device-wide coupling launch (0 = 4, < 0 = never) */
 } apx_options;
 
 /* Fill `o` with the defaults above. */

@@ -102,6 +102,8 @@ SCHEDULE_CONFIGS = [
     dict(schedule="hybrid", lane_batches=-1),
     dict(schedule="levels", lane_batches=1),     # lane layout for one-batch levels only
     dict(schedule="hybrid", tail_batches=64, tail_rows=4096),   # tail waves take several batches a level
+    dict(schedule="hybrid", tail_cut=-1),        # tail runs end only at launched levels / LDS capacity
+    dict(schedule="hybrid", tail_cut=1),         # a run ends before any level with overflow reads
 ]
 
 
@@ -139,6 +141,22 @@ def test_schedules_bit_identical(gpu, name, precision, fp16):
     for k in range(1, len(SCHEDULE_CONFIGS)):
         assert np.array_equal(sweeps[0], sweeps[k]), SCHEDULE_CONFIGS[k]
         assert np.array_equal(applies[0], applies[k]), SCHEDULE_CONFIGS[k]
+
+
+def test_tail_cut_splits_runs_at_long_rows(gpu):
+    """The overflow cut only moves run boundaries: on a power-law factor
+    (hub rows of thousands of entries) cutting before every level with
+    overflow reads yields more tail runs and no more overflow-heavy levels,
+    with the same bits as never cutting (checked by the schedule test)."""
+    A, F, fac = case("rmat_16")
+    print("[case] tail cut rmat_16", flush=True)
+    runs = {}
+    for cut in (-1, 0, 1):
+        s = ax.Solver(A, factor=F, precision="fp32", tail_cut=cut)
+        runs[cut] = s.info["tail_runs"]
+        s.close()
+    assert runs[1][0] >= runs[0][0] >= runs[-1][0] and runs[1][1] >= runs[0][1] >= runs[-1][1], runs
+    assert sum(runs[1]) > sum(runs[-1]), runs
 
 
 @pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "aniso3d_16", "rmat_12"])
