@@ -513,6 +513,7 @@ struct gpu_solver::device_state {
         dbuf f_pos, f_mask, f_col0, b_pos, b_mask, b_col0, f_lf_dev, b_lf_dev;
         std::vector<int> f_lf, b_lf, f_lpos, b_lpos, f_lgmax, b_lgmax;
         std::vector<int> f_seg[3], b_seg[3];   // l0, l1, tail flag
+        dbuf col_tv;                            // coupling values of the column-by-column tail runs
         block_sched f, b;
     };
     struct block_bufs {
@@ -1354,6 +1355,26 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
             sc.seg_l1 = seg[1].data();
             sc.seg_tail = seg[2].data();
             sc.nseg = static_cast<int>(seg[0].size());
+        }
+        // Column-by-column tail runs: the blocked sweep follows the
+        // single-vector schedule (its launched levels as blocked level
+        // launches, its tail runs with one tail workgroup per column), so
+        // the runs' level structure and overflow cuts carry over.
+        if (opts_.schedule == sweep_schedule::hybrid && env_flag("APXCHOL_AMD_BLOCK_COL_TAIL", 1) != 0) {
+            std::size_t maxb = 0;
+            for (const sweep_sched* sc : {&dev_->f_sched, &dev_->b_sched})
+                for (int k = 0; k < sc->nseg; ++k)
+                    if (sc->seg_tail[k])
+                        maxb = std::max<std::size_t>(
+                            maxb, static_cast<std::size_t>(sc->level_first[sc->seg_l1[k]] - sc->level_first[sc->seg_l0[k]]));
+            const std::size_t tvb = dev_->prec == vec_prec::fp64 ? sizeof(tail_val<double>) : sizeof(tail_val<float>);
+            const std::size_t bytes = std::max<std::size_t>(1, maxb) * static_cast<std::size_t>(kp) * kWave * tvb;
+            t->col_tv.alloc(bytes);
+            for (block_sched* sc : {&t->f, &t->b}) {
+                sc->cols = sc == &t->f ? &dev_->f_sched : &dev_->b_sched;
+                sc->col_tail_val = t->col_tv.get();
+                sc->col_tail_val_bytes = bytes;
+            }
         }
         it = b.tables.emplace(kp, std::move(t)).first;
     }

@@ -102,11 +102,13 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                     throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
                 tail_val<TO>* tv = static_cast<tail_val<TO>*>(sc.tail_val);
                 const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
+                tv += static_cast<size_t>(base) * kWave;
                 launch_timed(hook, 2, sweep_couple<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, bf, bl, p0, in,
-                             static_cast<const TO*>(o), tv + static_cast<size_t>(base) * kWave, st);
+                             static_cast<const TO*>(o), tv, st, 1, static_cast<const batch_state*>(nullptr));
                 launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
                              base, bl - bf, sc.seg_l1[k] - sc.seg_l0[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
-                             static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st);
+                             static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st, 1,
+                             static_cast<const batch_state*>(nullptr));
             } else {
                 for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
             }
@@ -336,7 +338,39 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
         sweep_block<TI, TO, F16><<<g, kBlock, 0, s>>>(d, b.pos, b.mask, b.col0, w0, w1, b.kp,
                                                       static_cast<const TI*>(rhs), static_cast<TO*>(out), st, bs);
     };
-    if (b.nseg == 0) {
+    if (b.cols) {
+        // The single-vector schedule, column by column: its launched levels
+        // as blocked level launches, each of its tail runs as one coupling
+        // launch over (batch, column) and one tail workgroup per column.
+        const sweep_sched& sc = *b.cols;
+        static const bool attr = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TO>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
+            return true;
+        }();
+        (void)attr;
+        for (int k = 0; k < sc.nseg; ++k) {
+            if (!sc.seg_tail[k]) {
+                for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
+                continue;
+            }
+            const int p0 = sc.seg_p0[k], p1 = sc.seg_p1[k];
+            const int bf = sc.level_first[sc.seg_l0[k]], bl = sc.level_first[sc.seg_l1[k]];
+            const size_t lds = tail_lds_bytes(p1 - p0, bl - bf, sizeof(TO));
+            const int base = sc.seg_base[k];
+            if (bl <= bf || base < 0 || base + (bl - bf) > sc.tail_batches || lds > static_cast<size_t>(kTailLdsBytes) ||
+                static_cast<size_t>(bl - bf) * b.kp * kWave * sizeof(tail_val<TO>) > b.col_tail_val_bytes)
+                throw std::runtime_error("apxchol_amd: column tail run outside its tables");
+            tail_val<TO>* tv = static_cast<tail_val<TO>*>(b.col_tail_val);
+            const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
+            sweep_couple<TI, TO, F16><<<dim3(g, b.kp), kBlock, 0, s>>>(d, bf, bl, p0, static_cast<const TI*>(rhs),
+                                                                     static_cast<const TO*>(out), tv, st, b.kp, bs);
+            sweep_tail<TO><<<b.kp, kTailBlock, lds, s>>>(sc.tail, base, bl - bf, sc.seg_l1[k] - sc.seg_l0[k], p1 - p0,
+                                                         d.backward ? d.m - 1 - p0 : p0, static_cast<int>(d.backward),
+                                                         static_cast<TO*>(out), static_cast<const tail_val<TO>*>(tv),
+                                                         st, b.kp, bs);
+        }
+    } else if (b.nseg == 0) {
         for (int l = 0; l < b.nlevels; ++l) level_launch(l);
     } else {
         static const bool attr = [] {

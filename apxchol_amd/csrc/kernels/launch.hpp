@@ -114,6 +114,12 @@ struct block_sched {
     const int* seg_l1 = nullptr;
     const int* seg_tail = nullptr;
     int nseg = 0;
+    // When set: the single-vector schedule whose levels this sweep follows,
+    // its tail runs walked column by column (one tail workgroup per column)
+    // with the coupling values in col_tail_val (kp * 64 tail_val per run batch).
+    const sweep_sched* cols = nullptr;
+    void* col_tail_val = nullptr;
+    size_t col_tail_val_bytes = 0;
 };
 // One blocked sweep as one launch per level.
 void launch_sweep_k(const sweep_desc& d, vec_prec p, bool fp16, const block_sched& bsc, const void* rhs, void* out,

@@ -134,14 +134,20 @@ __device__ __forceinline__ void publish(TO* p, TO v) {
 
 // Right-hand side of `row`: forward reads r - shift (the Laplacian input
 // centring), back reads y scaled by inv_scale^2 on the fp16 storage.
+// `at` is the element of rhs that holds the row (row * k + column for a
+// block of k vectors stored row by row).
 template <class TI, class AT>
-__device__ __forceinline__ AT row_rhs(const sweep_desc& d, const TI* rhs, int row, double shift) {
-    if (!d.backward) return static_cast<AT>(static_cast<double>(rhs[row]) - shift);
+__device__ __forceinline__ AT row_rhs(const sweep_desc& d, const TI* rhs, int row, size_t at, double shift) {
+    if (!d.backward) return static_cast<AT>(static_cast<double>(rhs[at]) - shift);
     if (d.inv_scale) {
         const double r = static_cast<double>(d.inv_scale[row]);
-        return static_cast<AT>(static_cast<double>(rhs[row]) * (r * r));
+        return static_cast<AT>(static_cast<double>(rhs[at]) * (r * r));
     }
-    return static_cast<AT>(rhs[row]);
+    return static_cast<AT>(rhs[at]);
+}
+template <class TI, class AT>
+__device__ __forceinline__ AT row_rhs(const sweep_desc& d, const TI* rhs, int row, double shift) {
+    return row_rhs<TI, AT>(d, rhs, row, static_cast<size_t>(row), shift);
 }
 
 // Process batch b. Returns false if a dependency never arrived within the
@@ -427,21 +433,27 @@ __device__ __forceinline__ lane_row map_lane(const sweep_desc& d, int b) {
 // run (complete when this kernel runs), in sweep_batch's slot order, and the
 // row's right-hand side for the lane that writes it. Entries of a row are in
 // ascending sweep position, so these are a prefix of every lane's chain.
-// tv points at the run's first tail batch.
+// Column c = blockIdx.y of a block of kp vectors stored row by row (kp = 1:
+// one vector; st then holds the one solve's state, else one per column, and
+// bs, when given, stops the launch once no column is active). tv points at
+// the run's first tail batch; column c's values start nb * 64 further on per
+// column (nb = blast - bfirst).
 template <class TI, class TO, bool F16>
 __global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
                                                        const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
-                                                       const pcg_state* st) {
-    if (st && st->done) return;
-    const double shift = (st && !d.backward) ? st->shift : 0.0;
+                                                       const pcg_state* st, int kp, const batch_state* bs) {
+    if (bs ? bs->active == 0 : (st && st->done)) return;
+    const int c = static_cast<int>(blockIdx.y);
+    const double shift = (st && !d.backward) ? st[c].shift : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     const int b = bfirst + static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
     if (b >= blast) return;
     const lane_row x = map_lane(d, b);
+    const auto value = [&](int j) { return out[static_cast<size_t>(j) * kp + c]; };
     TO part = 0;
     const auto before = [&](int j) { return (d.backward ? d.m - 1 - j : j) < p0; };
     if (p0 > 0 && x.len > static_cast<unsigned>(kWave * kEpl)) {
-        part = long_row_chain<F16, TO>(d, x.rb, x.len, [&](int j) { return out[j]; }, before);
+        part = long_row_chain<F16, TO>(d, x.rb, x.len, value, before);
     } else if (p0 > 0) {
         int jj[kEpl];
         float vv[kEpl];
@@ -455,17 +467,18 @@ __global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst,
             use |= (v && before(jj[k])) ? (1u << k) : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < kEpl; ++k) g[k] = out[jj[k]];
+        for (int k = 0; k < kEpl; ++k) g[k] = value(jj[k]);
 #pragma unroll
         for (int k = 0; k < kEpl; ++k)
             if ((use >> k) & 1u) part = fma(static_cast<TO>(vv[k]), g[k], part);
     }
     TO r = 0;
-    if (x.active && x.sub == 0) r = row_rhs<TI, TO>(d, rhs, x.row, shift);
+    if (x.active && x.sub == 0)
+        r = row_rhs<TI, TO>(d, rhs, x.row, static_cast<size_t>(x.row) * kp + c, shift);
     tail_val<TO> o;
     o.part0 = part;
     o.r = r;
-    tv[static_cast<size_t>(b - bfirst) * kWave + lane] = o;
+    tv[(static_cast<size_t>(c) * (blast - bfirst) + (b - bfirst)) * kWave + lane] = o;
 }
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -490,8 +503,10 @@ __device__ __forceinline__ size_t tail_at(int t, int k) {
     return (static_cast<size_t>(t) * kTailPairs + static_cast<size_t>(k)) * kWave + (threadIdx.x & (kWave - 1));
 }
 
+// t: the tail batch; tv: its run's coupling values, read at batch q of the run.
 template <class TO>
-__device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<TO>* tv, int t, tail_item<TO>& it) {
+__device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<TO>* tv, int t, int q,
+                                           tail_item<TO>& it) {
     const size_t i = static_cast<size_t>(t) * kWave + (threadIdx.x & (kWave - 1));
     const i32x4* ent = reinterpret_cast<const i32x4*>(td.ent);
     it.ln = reinterpret_cast<const i32x4*>(td.lane)[i];
@@ -499,7 +514,7 @@ __device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<T
     it.e1 = ent[tail_at(t, 1)];
     it.e2 = ent[tail_at(t, 2)];
     it.e3 = ent[tail_at(t, 3)];
-    it.v = reinterpret_cast<const vec2<TO>*>(tv)[i];
+    it.v = reinterpret_cast<const vec2<TO>*>(tv)[static_cast<size_t>(q) * kWave + (threadIdx.x & (kWave - 1))];
     it.o = td.ovf[t];
 }
 
@@ -571,11 +586,16 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
 // ... of it); zslot: the run's row count (its LDS holds zslot + 1 values);
 // output row of slot s: row0 + s (forward) or row0 - s (back). Outputs reach
 // HBM in one coalesced copy at the end.
+// Column c = blockIdx.x of a block of kp vectors stored row by row (kp = 1:
+// one vector); tv: the run's coupling values, nb * 64 per column; bs, when
+// given, stops the launch once no column is active (else st->done does).
 template <class TO>
 __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, int nb, int nlev, int zslot,
                                                          int row0, int backward, TO* out, const tail_val<TO>* tv,
-                                                         const pcg_state* st) {
-    if (st && st->done) return;
+                                                         const pcg_state* st, int kp, const batch_state* bs) {
+    if (bs ? bs->active == 0 : (st && st->done)) return;
+    const int col = static_cast<int>(blockIdx.x);
+    tv += static_cast<size_t>(col) * nb * kWave;
     extern __shared__ unsigned char tail_lds[];
     TO* lds = reinterpret_cast<TO*>(tail_lds);
     // Past the outputs and the zero slot: the run's walk and every wave's
@@ -609,13 +629,14 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     const auto level_of = [&](int c) {
         return c < 0 ? nlev : (__builtin_amdgcn_readfirstlane(walk[c].info) & (kTailWalkMaxLevels - 1));
     };
-    const auto batch_of = [&](int c) { return base + (c < 0 ? nb - 1 : c); };
+    const auto run_batch = [&](int c) { return c < 0 ? nb - 1 : c; };
+    const auto batch_of = [&](int c) { return base + run_batch(c); };
     int ca = __builtin_amdgcn_readfirstlane(first[w]);
     int cb = next_of(ca);
     int la = level_of(ca), lb = level_of(cb);
     tail_item<TO> ia, ib;
-    tail_fetch<TO>(td, tv, batch_of(ca), ia);
-    tail_fetch<TO>(td, tv, batch_of(cb), ib);
+    tail_fetch<TO>(td, tv, batch_of(ca), run_batch(ca), ia);
+    tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ib);
     // One exit and both fetches on every pass (a finished walk stays at -1),
     // so the item loads in flight are the same wherever the passes meet.
     int lvl = 0;   // the level whose rows are being computed
@@ -629,7 +650,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
         la = lb;
         cb = next_of(cb);
         lb = level_of(cb);
-        tail_fetch<TO>(td, tv, batch_of(cb), ia);
+        tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ia);
         if (la < nlev) {
             while (lvl < la) {
                 lds_barrier();
@@ -641,13 +662,14 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
         la = lb;
         cb = next_of(cb);
         lb = level_of(cb);
-        tail_fetch<TO>(td, tv, batch_of(cb), ib);
+        tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ib);
     }
     while (lvl < nlev) {
         lds_barrier();
         ++lvl;
     }
-    for (int q = tid; q < zslot; q += kTailBlock) out[backward ? row0 - q : row0 + q] = lds[q];
+    for (int q = tid; q < zslot; q += kTailBlock)
+        out[static_cast<size_t>(backward ? row0 - q : row0 + q) * kp + col] = lds[q];
 }
 
 // Fill a vector with the pending sentinel (the state every sweep output must

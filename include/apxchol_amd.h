@@ -58,7 +58,7 @@ typedef struct {
     int32_t tail_cut;        /* hybrid: a tail run ends before a level one of whose lanes would read
                                 at least this many groups of 8 entries inside the run from HBM
                                 (rows of thousands of entries); that level's entries are then summed
-                                by the device-wide coupling launch (0 = 4, < 0 = never) */
+                                by the device-wide coupling launch (0 = 1, < 0 = never) */
 } apx_options;
 
 /* Fill `o` with the defaults above. */

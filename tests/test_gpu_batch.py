@@ -21,6 +21,7 @@ CASES = {
     "sddm3d_16": lambda: mg.add_excess(mg.grid_laplacian((16, 16, 16), "lognormal")),
     "aniso3d_16": lambda: mg.grid_laplacian((16, 16, 16), "lognormal", 0.5, seed=5, axis_weights=(1, 10, 100)),
     "rmat_12": lambda: mg.rmat_laplacian(12),
+    "rmat_14": lambda: mg.rmat_laplacian(14),   # hub rows: overflow cuts in the tail runs
 }
 _cache = {}
 
@@ -44,13 +45,18 @@ def bits(a):
     return np.ascontiguousarray(a).view(np.uint64)
 
 
-@pytest.mark.parametrize("name", ["grid2d_64", "sddm3d_16", "rmat_12"])
+@pytest.mark.parametrize("name", ["grid2d_64", "sddm3d_16", "rmat_12", "rmat_14"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("fp16", [False, True])
-def test_apply_batch_bits_equal_single_apply(gpu, name, precision, fp16):
+@pytest.mark.parametrize("col_tail", ["1", "0"])
+def test_apply_batch_bits_equal_single_apply(gpu, name, precision, fp16, col_tail, monkeypatch):
+    """Every column of a blocked application has the single application's
+    bits, with the tail runs walked column by column (one tail workgroup per
+    column, the default) or by the blocked one-workgroup tail."""
     import torch
     A, F = case(name)
-    print(f"[case] apply_batch {name} {precision} fp16={fp16}", flush=True)
+    print(f"[case] apply_batch {name} {precision} fp16={fp16} col_tail={col_tail}", flush=True)
+    monkeypatch.setenv("APXCHOL_AMD_BLOCK_COL_TAIL", col_tail)
     s = ax.Solver(A, factor=F, precision=precision, fp16_storage=fp16)
     k = 6
     R = torch.tensor(rhs_block(A, F.sddm, k, 1), device=gpu)
