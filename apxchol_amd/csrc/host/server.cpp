@@ -86,6 +86,9 @@ std::int64_t solve_server::submit(int factor, const double* b, double tol, int m
     {
         std::lock_guard<std::mutex> lk(mu_);
         if (factor < 0 || factor >= static_cast<int>(lanes_.size())) throw std::invalid_argument("server: bad factor id");
+        // A lane ended by a device failure takes no more requests.
+        if (!lanes_[factor]->error.empty())
+            throw std::runtime_error("server: factor " + std::to_string(factor) + " failed: " + lanes_[factor]->error);
         const std::int64_t n = lanes_[factor]->n;
         rq->b.assign(b, b + n);
         rq->id = next_id_++;
@@ -110,6 +113,11 @@ bool solve_server::wait(std::int64_t id, double* x, solve_result* res, double ti
     auto ready = [&] { return rq->done; };
     if (timeout_s < 0) done_cv_.wait(lk, ready);
     else if (!done_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) return false;
+    if (rq->res.status == -1) {   // its lane ended on a device failure
+        const std::string why = lanes_[rq->factor]->error;
+        requests_.erase(id);
+        throw std::runtime_error("server: request " + std::to_string(id) + " failed: " + why);
+    }
     if (x) std::copy(rq->x.begin(), rq->x.end(), x);
     if (res) *res = rq->res;
     requests_.erase(id);
@@ -199,7 +207,9 @@ void solve_server::run(lane& ln) {
             for (std::size_t j = 0; j < fin.size(); ++j) {
                 auto rq = ln.slot[fin[j]];
                 rq->x.resize(n);
-                check(hipMemcpy(rq->x.data(), dst[j], n * sizeof(double), hipMemcpyDeviceToHost), "download x");
+                check(hipMemcpyAsync(rq->x.data(), dst[j], n * sizeof(double), hipMemcpyDeviceToHost, ln.stream),
+                      "download x");
+                check(hipStreamSynchronize(ln.stream), "download x");
                 const auto& st = states[fin[j]];
                 rq->res.iterations = st.iterations;
                 rq->res.residual = st.residual;
@@ -221,9 +231,12 @@ void solve_server::run(lane& ln) {
                 }
             }
             done_cv_.notify_all();
-        } catch (const std::exception&) {
-            // A device failure ends this lane: its requests fail (status -1).
+        } catch (const std::exception& e) {
+            // A device failure ends this lane: its requests fail (status -1)
+            // and later submissions to it are refused with the reason.
             std::lock_guard<std::mutex> lk(mu_);
+            ln.error = e.what();
+            if (ln.error.empty()) ln.error = "device failure";
             auto fail = [&](std::shared_ptr<request>& rq) {
                 if (!rq) return;
                 rq->res.status = -1;

@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -80,6 +81,7 @@ private:
         void* d_x = nullptr;
         int kp = 0;
         std::int64_t n = 0;
+        std::string error;                            // set when a device failure ended the lane
     };
     void run(lane& ln);
 

@@ -74,6 +74,7 @@ def run_serving(args, ax, mg, np) -> dict:
         fids.append(srv.add(S))
         solvers.append(S)
     t_setup = time.perf_counter() - t0
+    print(f"[bench] serving: {len(mats)} factors resident ({t_setup:.1f}s)", file=sys.stderr, flush=True)
     rng = np.random.default_rng(6)
     tols = [1e-6, 1e-8, 1e-10]
     plan = []
@@ -83,15 +84,23 @@ def run_serving(args, ax, mg, np) -> dict:
     rhs = [mg.request_rhs(mats[f][1], mats[f][2], 1000 + i) for i, (f, _, _) in enumerate(plan)]
     ids = [None] * len(plan)
 
+    errors = []
+
     def produce():
-        for i, (f, tol, gap) in enumerate(plan):
-            time.sleep(gap)
-            ids[i] = srv.submit(fids[f], rhs[i], tol)
+        try:
+            for i, (f, tol, gap) in enumerate(plan):
+                time.sleep(gap)
+                ids[i] = srv.submit(fids[f], rhs[i], tol)
+        except Exception as e:   # a refused submission ends the run with its reason
+            errors.append(e)
 
     th = threading.Thread(target=produce)
     t1 = time.perf_counter()
     th.start()
     th.join()
+    if errors:
+        raise errors[0]
+    print(f"[bench] serving: {len(plan)} requests submitted", file=sys.stderr, flush=True)
     worst = 0.0
     for i, (f, tol, _) in enumerate(plan):
         x, r = srv.wait(ids[i], timeout=300)
