@@ -81,11 +81,7 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
 #pragma unroll
         for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const TO o = __shfl_xor(part, off, kWave);
-        if (off < G) part += o;
-    }
+    part = group_tree_sum(part, G);
     if (!writer) return;
     TO r;
     if (!d.backward) r = static_cast<TO>(rin - shift);
@@ -223,11 +219,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, con
 #pragma unroll
                 for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[k]), xv[k], part);
             }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const TO o = __shfl_xor(part, off, kWave);
-                if (off < G) part += o;
-            }
+            part = group_tree_sum(part, G);
             if (active && s == 0) {
                 const size_t o = static_cast<size_t>(row) * kp + c;
                 TO r;

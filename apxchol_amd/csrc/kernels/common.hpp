@@ -21,6 +21,49 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Lane i's copy of lane i + OFF of its row of 16 lanes (OFF < 16), by a DPP
+// row shift; lanes whose source lies past the row read 0.
+template <int OFF>
+__device__ __forceinline__ int dpp_from_right(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x100 + OFF, 0xF, 0xF, false);
+}
+template <int OFF>
+__device__ __forceinline__ float dpp_from_right(float v) {
+    return __builtin_bit_cast(float, dpp_from_right<OFF>(__builtin_bit_cast(int, v)));
+}
+template <int OFF>
+__device__ __forceinline__ double dpp_from_right(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = static_cast<unsigned>(dpp_from_right<OFF>(static_cast<int>(b & 0xFFFFFFFFull)));
+    const unsigned hi = static_cast<unsigned>(dpp_from_right<OFF>(static_cast<int>(b >> 32)));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// The butterfly of a lane group's partial sums as the group's first lane
+// sees it: for off = 32, 16, ..., 1 with off < G, lane s adds lane s + off
+// (groups are aligned to their power-of-two size G). Only the first lane of
+// each group holds the group's sum afterwards; its value is the xor
+// butterfly's (v[s] += v[s ^ off] gives lane s < off the same operand).
+// Steps below 16 read the partner with a DPP row shift, the two wider ones
+// with a lane permute; a step no lane needs is skipped as a whole.
+template <int OFF, class T>
+__device__ __forceinline__ T tree_step(T part, int G) {
+    if (__ballot(OFF < G) == 0) return part;
+    T o;
+    if constexpr (OFF >= 16) o = __shfl_xor(part, OFF, kWave);
+    else o = dpp_from_right<OFF>(part);
+    return OFF < G ? part + o : part;
+}
+template <class T>
+__device__ __forceinline__ T group_tree_sum(T part, int G) {
+    part = tree_step<32>(part, G);
+    part = tree_step<16>(part, G);
+    part = tree_step<8>(part, G);
+    part = tree_step<4>(part, G);
+    part = tree_step<2>(part, G);
+    return tree_step<1>(part, G);
+}
+
 // Sum over a workgroup of kBlock threads: per-wave butterfly, then wave 0..3
 // in order. Valid in thread 0 only. `sh` holds kWavesPerBlock * N values.
 template <int N>

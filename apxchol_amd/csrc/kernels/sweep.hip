@@ -213,8 +213,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
 #pragma unroll
             for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
         }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, kWave);
+        part = group_tree_sum(part, kWave);
         if (lane == 0) {
             const AT r = row_rhs<TI, AT>(d, rhs, row, shift);
             publish<TO, WAIT>(out + row, (r - part) / static_cast<AT>(d.diag[row]));
@@ -247,11 +246,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         AT part = 0;
 #pragma unroll
         for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const AT o = __shfl_xor(part, off, kWave);
-            if (off < G) part += o;
-        }
+        part = group_tree_sum(part, G);
         if (active && sub == 0) publish<TO, WAIT>(out + row, (r - part) / dg);
         return true;
     }
@@ -265,11 +260,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         if (go)
 #pragma unroll
             for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const AT o = __shfl_xor(part, off, kWave);
-            if (off < G) part += o;
-        }
+        part = group_tree_sum(part, G);
         if (go) {
             if (sub == 0) publish<TO, WAIT>(out + row, (r - part) / dg);
             done = true;
@@ -365,11 +356,7 @@ __global__ __launch_bounds__(kBlock) void sweep_lanes(sweep_desc d, int lb0, int
     AT part = 0;
 #pragma unroll
     for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(e[k].val), xv[k], part);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const AT o = __shfl_xor(part, off, kWave);
-        if (off < G) part += o;
-    }
+    part = group_tree_sum(part, G);
     if (head) out[row] = (r - part) / dg;
 }
 
@@ -570,14 +557,7 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
             s = sn;
         }
     }
-    // Butterfly over the lane groups; a step no lane of the batch needs
-    // (every G <= off) is skipped as a whole, which leaves every sum as is.
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        if (__ballot(off < G) == 0) continue;
-        const TO o = __shfl_xor(part, off, kWave);
-        if (off < G) part += o;
-    }
+    part = group_tree_sum(part, G);
     if (it.ln.x >= 0) lds[it.ln.y] = (it.v.y - part) / static_cast<TO>(as_float(it.ln.w));
 }
 
