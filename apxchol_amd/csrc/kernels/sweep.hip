@@ -372,14 +372,15 @@ __global__ __launch_bounds__(kBlock) void sweep_lanes(sweep_desc d, int lb0, int
 // and butterfly), so the result is bit-identical to the other schedules.
 inline constexpr int kTailLdsBytes = 160 * 1024;
 // LDS of a tail run of `rows` rows and `batches` batches: the run's outputs
-// and the zero slot (rounded up to 8 bytes), the run's walk (one tail_walk
-// per batch) and every wave's first batch.
+// and the zero slot (rounded up to 8 bytes), the run's walk (per batch: the
+// same wave's next batch and that batch's level) and every wave's first
+// batch and its level.
 inline constexpr std::size_t tail_slot_bytes(int rows, std::size_t vsz) {
     return ((static_cast<std::size_t>(rows) + 1) * vsz + 7) / 8 * 8;
 }
 inline constexpr std::size_t tail_lds_bytes(int rows, int batches, std::size_t vsz) {
-    return tail_slot_bytes(rows, vsz) + static_cast<std::size_t>(batches) * sizeof(tail_walk) +
-           kTailWaves * sizeof(int);
+    return tail_slot_bytes(rows, vsz) + static_cast<std::size_t>(batches) * 2 * sizeof(int) +
+           kTailWaves * 2 * sizeof(int);
 }
 
 // Workgroup barrier ordering LDS only: outstanding global loads stay in flight.
@@ -578,21 +579,25 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     tv += static_cast<size_t>(col) * nb * kWave;
     extern __shared__ unsigned char tail_lds[];
     TO* lds = reinterpret_cast<TO*>(tail_lds);
-    // Past the outputs and the zero slot: the run's walk and every wave's
-    // first batch. The walk is read from LDS, so its lookups never wait
-    // behind the item loads in flight (global loads complete in order).
-    tail_walk* walk = reinterpret_cast<tail_walk*>(tail_lds + tail_slot_bytes(zslot, sizeof(TO)));
-    int* first = reinterpret_cast<int*>(walk + nb);
+    // Past the outputs and the zero slot: the run's walk, as (next batch of
+    // the wave, its level) pairs, and every wave's first batch and level.
+    // The walk is read from LDS, so its lookups never wait behind the item
+    // loads in flight (global loads complete in order), and one read gives
+    // both halves of a step.
+    int2* walk = reinterpret_cast<int2*>(tail_lds + tail_slot_bytes(zslot, sizeof(TO)));
+    int2* first = walk + nb;
     const int tid = static_cast<int>(threadIdx.x);
-    if (tid < kTailWaves) first[tid] = -1;
+    if (tid < kTailWaves) first[tid] = make_int2(-1, nlev);
     // The zero slot past the run's outputs, read by padding entries (all
     // writers store 0).
     if (tid == 0) lds[zslot] = TO(0);
     lds_barrier();
     for (int q = tid; q < nb; q += kTailBlock) {
         const tail_walk x = td.walk[base + q];
-        walk[q] = x;
-        if (x.info & kTailWalkFirst) first[(x.info >> kTailWalkWaveShift) & (kTailWaves - 1)] = q;
+        const int nl = x.next < 0 ? nlev : (td.walk[base + x.next].info & (kTailWalkMaxLevels - 1));
+        walk[q] = make_int2(x.next, nl);
+        if (x.info & kTailWalkFirst)
+            first[(x.info >> kTailWalkWaveShift) & (kTailWaves - 1)] = make_int2(q, x.info & (kTailWalkMaxLevels - 1));
     }
     lds_barrier();
     // Wave w walks its batches in order, passing the level barriers between
@@ -605,44 +610,45 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     // walk is over) and wave-uniform (readfirstlane of LDS values read at one
     // address), so the walk stays scalar.
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const auto next_of = [&](int c) { return c < 0 ? -1 : __builtin_amdgcn_readfirstlane(walk[c].next); };
-    const auto level_of = [&](int c) {
-        return c < 0 ? nlev : (__builtin_amdgcn_readfirstlane(walk[c].info) & (kTailWalkMaxLevels - 1));
+    // (next batch, its level) after batch c (c = -1: the walk is over).
+    const auto step_of = [&](int c) {
+        if (c < 0) return make_int2(-1, nlev);
+        const int2 e = walk[c];
+        return make_int2(__builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y));
     };
     const auto run_batch = [&](int c) { return c < 0 ? nb - 1 : c; };
     const auto batch_of = [&](int c) { return base + run_batch(c); };
-    int ca = __builtin_amdgcn_readfirstlane(first[w]);
-    int cb = next_of(ca);
-    int la = level_of(ca), lb = level_of(cb);
+    int2 f = first[w];
+    int ca = __builtin_amdgcn_readfirstlane(f.x), la = __builtin_amdgcn_readfirstlane(f.y);
+    int2 nb2 = step_of(ca);
+    int cb = nb2.x, lb = nb2.y;
     tail_item<TO> ia, ib;
     tail_fetch<TO>(td, tv, batch_of(ca), run_batch(ca), ia);
     tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ib);
-    // One exit and both fetches on every pass (a finished walk stays at -1),
-    // so the item loads in flight are the same wherever the passes meet.
     int lvl = 0;   // the level whose rows are being computed
-    while (la < nlev) {
-        while (lvl < la) {
-            lds_barrier();
-            ++lvl;
-        }
-        tail_step<TO>(td, batch_of(ca), ia, lds);
-        ca = cb;
-        la = lb;
-        cb = next_of(cb);
-        lb = level_of(cb);
-        tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ia);
+    // Compute the batch in `it` (cursor a), shift the cursors and fetch the
+    // batch two ahead into `it` (its walk entry is read before the step, so
+    // the lookup overlaps the step). One exit and every fetch on every pass
+    // (a finished walk stays at -1), so the item loads in flight are the
+    // same wherever the passes meet.
+    const auto turn = [&](tail_item<TO>& it) {
+        const int2 nx = step_of(cb);
         if (la < nlev) {
             while (lvl < la) {
                 lds_barrier();
                 ++lvl;
             }
-            tail_step<TO>(td, batch_of(ca), ib, lds);
+            tail_step<TO>(td, batch_of(ca), it, lds);
         }
         ca = cb;
         la = lb;
-        cb = next_of(cb);
-        lb = level_of(cb);
-        tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ib);
+        cb = nx.x;
+        lb = nx.y;
+        tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), it);
+    };
+    while (la < nlev) {
+        turn(ia);
+        turn(ib);
     }
     while (lvl < nlev) {
         lds_barrier();
