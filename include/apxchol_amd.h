@@ -205,7 +205,10 @@ apx_status apx_server_add(apx_server_t srv, apx_solver_t solver, int32_t* factor
 apx_status apx_server_submit(apx_server_t srv, int32_t factor, const double* b, double tol, int32_t max_iter,
                              int64_t* request_id);
 /* Waits up to timeout_s seconds (< 0: no limit); *ready = 1 and x (n host
- * doubles) / info filled when the request is done (it is then forgotten). */
+ * doubles) / info filled when the request is done (it is then forgotten).
+ * A request whose factor lane ended on a device error returns
+ * APX_ERR_RUNTIME with that error; submissions to such a lane are refused
+ * the same way. */
 apx_status apx_server_wait(apx_server_t srv, int64_t request_id, double* x, apx_solve_info* info, double timeout_s,
                            int32_t* ready);
 apx_status apx_server_get_stats(apx_server_t srv, apx_server_stats* st);
