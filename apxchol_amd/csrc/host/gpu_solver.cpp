@@ -1300,7 +1300,7 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
             for (std::size_t l = 0; l <= L; ++l)
                 lpos[l] = static_cast<int>(backward ? T_.m - T_.level_ptr[L - l] : T_.level_ptr[l]);
             // Lanes of each level's widest row (the per-(row, column) level
-            // kernel takes levels up to 8; 0 marks a row past 64 lanes).
+            // kernel takes levels up to 16; 0 marks a row past 64 lanes).
             std::vector<int>& lg = backward ? t->b_lgmax : t->f_lgmax;
             lg.assign(L, 0);
             const auto& rptr = backward ? T_.b_ptr : T_.f_ptr;
@@ -1350,6 +1350,7 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
             sc.level_first_dev = lfd.as<int>();
             sc.level_pos = lpos.data();
             sc.level_gmax = env_flag("APXCHOL_AMD_BLOCK_ROWS", 1) != 0 ? lg.data() : nullptr;
+            sc.rows_max_g = env_flag("APXCHOL_AMD_BLOCK_ROWS_G", 16) >= 16 ? 16 : 8;
             sc.nlevels = static_cast<int>(lf.size()) - 1;
             sc.seg_l0 = seg[0].data();
             sc.seg_l1 = seg[1].data();

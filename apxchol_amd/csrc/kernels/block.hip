@@ -118,32 +118,40 @@ __global__ __launch_bounds__(kBlock) void sweep_block_rows(sweep_desc d, int p0,
     const unsigned rb = d.ptr[row];
     const unsigned len = d.ptr[row + 1] - rb;
     const int G = lanes_for(len);
-    int jj[GMAX][kEpl];
-    float vv[GMAX][kEpl];
-    TO xv[GMAX][kEpl];
-#pragma unroll
-    for (int s = 0; s < GMAX; ++s)
-#pragma unroll
-        for (int k = 0; k < kEpl; ++k) {
-            const unsigned q = static_cast<unsigned>(s + k * G);
-            const bool valid = s < G && q < len;
-            load_entry<F16>(d, valid ? rb + q : (len > 0 ? rb : 0u), jj[s][k], vv[s][k]);
-            if (!valid) vv[s][k] = 0.0f;
-        }
-#pragma unroll
-    for (int s = 0; s < GMAX; ++s)
-#pragma unroll
-        for (int k = 0; k < kEpl; ++k) {
-            const bool valid = s < G && static_cast<unsigned>(s + k * G) < len;
-            xv[s][k] = valid ? out[static_cast<size_t>(jj[s][k]) * kp + c] : static_cast<TO>(0);
-        }
+    // Lane chains in chunks of CH (all of them up to 8 lanes): a chunk's
+    // entry loads, then its gathers, are in flight together.
+    constexpr int CH = GMAX < 8 ? GMAX : 8;
     TO v[GMAX];
 #pragma unroll
-    for (int s = 0; s < GMAX; ++s) {
-        TO part = 0;
+    for (int s0 = 0; s0 < GMAX; s0 += CH) {
+        int jj[CH][kEpl];
+        float vv[CH][kEpl];
+        TO xv[CH][kEpl];
 #pragma unroll
-        for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[s][k]), xv[s][k], part);
-        v[s] = part;
+        for (int u = 0; u < CH; ++u)
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) {
+                const int s = s0 + u;
+                const unsigned q = static_cast<unsigned>(s + k * G);
+                const bool valid = s < G && q < len;
+                load_entry<F16>(d, valid ? rb + q : (len > 0 ? rb : 0u), jj[u][k], vv[u][k]);
+                if (!valid) vv[u][k] = 0.0f;
+            }
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) {
+                const int s = s0 + u;
+                const bool valid = s < G && static_cast<unsigned>(s + k * G) < len;
+                xv[u][k] = valid ? out[static_cast<size_t>(jj[u][k]) * kp + c] : static_cast<TO>(0);
+            }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            TO part = 0;
+#pragma unroll
+            for (int k = 0; k < kEpl; ++k) part = fma(static_cast<TO>(vv[u][k]), xv[u][k], part);
+            v[s0 + u] = part;
+        }
     }
 #pragma unroll
     for (int off = GMAX / 2; off >= 1; off >>= 1)

@@ -321,7 +321,7 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
         const int w0 = b.level_first[l], w1 = b.level_first[l + 1];
         if (w1 <= w0) return;
         const int gmax = b.level_gmax ? b.level_gmax[l] : 0;
-        if (gmax >= 1 && gmax <= 8) {
+        if (gmax >= 1 && gmax <= b.rows_max_g) {
             // One thread per (row, column).
             const int p0 = b.level_pos[l], p1 = b.level_pos[l + 1];
             const long long threads = static_cast<long long>(p1 - p0) * b.kp;
@@ -331,7 +331,8 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
             if (gmax == 1) sweep_block_rows<TI, TO, F16, 1><<<g, kBlock, 0, s>>>(d, p0, p1, b.kp, in, o, st, bs);
             else if (gmax == 2) sweep_block_rows<TI, TO, F16, 2><<<g, kBlock, 0, s>>>(d, p0, p1, b.kp, in, o, st, bs);
             else if (gmax == 4) sweep_block_rows<TI, TO, F16, 4><<<g, kBlock, 0, s>>>(d, p0, p1, b.kp, in, o, st, bs);
-            else sweep_block_rows<TI, TO, F16, 8><<<g, kBlock, 0, s>>>(d, p0, p1, b.kp, in, o, st, bs);
+            else if (gmax == 8) sweep_block_rows<TI, TO, F16, 8><<<g, kBlock, 0, s>>>(d, p0, p1, b.kp, in, o, st, bs);
+            else sweep_block_rows<TI, TO, F16, 16><<<g, kBlock, 0, s>>>(d, p0, p1, b.kp, in, o, st, bs);
             return;
         }
         const int g = (w1 - w0 + kWavesPerBlock - 1) / kWavesPerBlock;

@@ -107,6 +107,8 @@ struct block_sched {
     const int* level_first_dev = nullptr;   // device copy (tail runs)
     const int* level_pos = nullptr;         // host: first sweep position of each level (nlevels + 1)
     const int* level_gmax = nullptr;        // host: lanes of the level's widest row (0: a row past 64 lanes)
+    int rows_max_g = 16;                    // levels whose widest row has at most this many lanes (1..16)
+                                            // run one thread per (row, column)
     int nlevels = 0;
     // Segments: levels [seg_l0, seg_l1), tail flag (one workgroup walks the
     // run with its outputs in LDS) or one launch per level.
