@@ -17,7 +17,7 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def gpu():
     """The HIP device as torch sees it; the library must be built -- a missing
-    library on a GPU box is a failure, not a skip."""
+    library on a machine with a GPU is a failure, not a skip."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
