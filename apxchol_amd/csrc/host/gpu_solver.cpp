@@ -831,7 +831,7 @@ void gpu_solver::setup(const csc_view& L) {
     // Batches a tail level may hold (up to kTailMaxBatches: each of the
     // workgroup's waves takes every kTailWaves-th batch).
     const int level_tail_batches = std::clamp(opts_.tail_batches > 0 ? opts_.tail_batches
-                                                              : env_flag("APXCHOL_AMD_TAIL_BATCHES", 2 * kTailWaves),
+                                                              : env_flag("APXCHOL_AMD_TAIL_BATCHES", 3 * kTailWaves),
                                         1, kTailMaxBatches);
     // A run ends before a level whose longest lane would read at least this
     // many overflow groups inside the run (0: runs end only at launched levels

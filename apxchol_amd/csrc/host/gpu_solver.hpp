@@ -28,7 +28,7 @@ struct solve_options {
     bool fp16_storage = false;
     sweep_schedule schedule = sweep_schedule::hybrid;
     int tail_rows = 0;            // hybrid: levels below this many rows go to the one-workgroup tail (0: default)
-    int tail_batches = 0;         // hybrid: ... and of at most this many batches (0: APXCHOL_AMD_TAIL_BATCHES or 32; <= 64)
+    int tail_batches = 0;         // hybrid: ... and of at most this many batches (0: APXCHOL_AMD_TAIL_BATCHES or 48; <= 64)
     int tail_cut = 0;             // hybrid: a run ends before a level needing this many overflow groups per lane
                                   // (0: APXCHOL_AMD_TAIL_CUT or 1; < 0: never)
     double drop_rel = -1.0;       // < 0: APXCHOL_FACTOR_DROP / the 1e-4 default

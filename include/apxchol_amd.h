@@ -53,7 +53,7 @@ typedef struct {
                                 layout (0 = default 4096, < 0 = never: batch tables only) */
     int32_t depth_levels;    /* 1 (default, with relabel): sweep levels = topological depth of the
                                 dependency graph; 0: the elimination rounds (round-as-level) */
-    int32_t tail_batches;    /* hybrid: a tail level holds at most this many batches (0 = 32, max 64;
+    int32_t tail_batches;    /* hybrid: a tail level holds at most this many batches (0 = 48, max 64;
                                 each wave of the tail workgroup takes every 16th) */
     int32_t tail_cut;        /* hybrid: a tail run ends before a level one of whose lanes would read
                                 at least this many groups of 8 entries inside the run from HBM
