@@ -172,7 +172,7 @@ def _csc(A):
 
 
 def _options(tol=1e-8, maxiter=200, stagnation_window=50, seed=42, precision="fp64", fp16_storage=False,
-             schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=4, sweep_chunk=0,
+             schedule="hybrid", drop_rel=-1.0, keep_factor=False, iters_per_check=2, sweep_chunk=0,
              sweep_blocks=0, tail_rows=0, graph=True, relabel=True, lane_batches=0,
              depth_levels=True, tail_batches=0, tail_cut=0) -> Options:
     o = Options()

@@ -124,7 +124,7 @@ void apx_options_default(apx_options* o) {
     o->schedule = 2;
     o->drop_rel = -1.0;
     o->keep_factor = 0;
-    o->iters_per_check = 4;
+    o->iters_per_check = 2;
     o->sweep_chunk = 0;
     o->sweep_blocks = 0;
     o->tail_rows = 0;

@@ -41,7 +41,7 @@ typedef struct {
                                 big levels launched, runs of small levels walked by one workgroup */
     double drop_rel;         /* < 0: APXCHOL_FACTOR_DROP or 1e-4; 0 disables the drop */
     int32_t keep_factor;     /* keep the host factor values for apx_solver_export_factor */
-    int32_t iters_per_check; /* PCG iterations enqueued between completion tests (default 4) */
+    int32_t iters_per_check; /* PCG iterations enqueued between completion tests (default 2) */
     int32_t sweep_chunk;     /* batches claimed per dataflow ticket (0 = default) */
     int32_t sweep_blocks;    /* dataflow workgroups / level-launch cap (0 = the device's resident grid) */
     int32_t tail_rows;       /* hybrid: levels with fewer rows go to the one-workgroup tail (0 = default) */
