@@ -562,11 +562,12 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
     if (it.ln.x >= 0) lds[it.ln.y] = (it.v.y - part) / static_cast<TO>(as_float(it.ln.w));
 }
 
-// base: the run's first tail batch; levels [l0, l1) of the run (a level may
-// hold up to kTailMaxBatches batches: wave w takes batches w, w + kTailWaves,
-// ... of it); zslot: the run's row count (its LDS holds zslot + 1 values);
-// output row of slot s: row0 + s (forward) or row0 - s (back). Outputs reach
-// HBM in one coalesced copy at the end.
+// base: the run's first tail batch; nb: its batches; nlev: its levels (a
+// level may hold up to kTailMaxBatches batches: wave w takes batches w,
+// w + kTailWaves, ... of it, as the run's walk table lists them); zslot: the
+// run's row count (its LDS holds zslot + 1 values); output row of slot s:
+// row0 + s (forward) or row0 - s (back). Outputs reach HBM in one coalesced
+// copy at the end.
 // Column c = blockIdx.x of a block of kp vectors stored row by row (kp = 1:
 // one vector); tv: the run's coupling values, nb * 64 per column; bs, when
 // given, stops the launch once no column is active (else st->done does).
@@ -588,8 +589,8 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     int2* first = walk + nb;
     const int tid = static_cast<int>(threadIdx.x);
     if (tid < kTailWaves) first[tid] = make_int2(-1, nlev);
-    // The zero slot past the run's outputs, read by padding entries (all
-    // writers store 0).
+    // The zero slot past the run's outputs, read by padding entries; written
+    // once, before the barrier that precedes the walk.
     if (tid == 0) lds[zslot] = TO(0);
     lds_barrier();
     for (int q = tid; q < nb; q += kTailBlock) {
