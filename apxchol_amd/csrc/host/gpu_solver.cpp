@@ -825,7 +825,7 @@ void gpu_solver::setup(const csc_view& L) {
     // other (small) levels becomes one tail segment.
     const int tail_rows = std::min(opts_.tail_rows > 0 ? opts_.tail_rows : env_int("APXCHOL_AMD_TAIL_ROWS", 2048),
                                    sweep_tail_max_rows(d.prec));
-    // A tail run's outputs (and its level table) live in LDS: a run longer
+    // A tail run's outputs (and its walk table) live in LDS: a run longer
     // than that holds is split.
     const vec_prec tail_prec = d.prec;
     // Batches a tail level may hold (up to kTailMaxBatches: each of the
