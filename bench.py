@@ -300,7 +300,16 @@ def main() -> int:
             batched[args.batch_workload] = run_batched(args, ax, mg, np, torch, dev, args.batch_workload, None, None,
                                                        None, 0)
 
-    serving = run_serving(args, ax, mg, np) if args.serving_requests > 0 else None
+    # The serving mix is reported beside the headline: a failure there is
+    # recorded in its section (with the server's reason) and does not void
+    # the measured single-solve line.
+    serving = None
+    if args.serving_requests > 0:
+        try:
+            serving = run_serving(args, ax, mg, np)
+        except Exception as e:
+            print(f"[bench] serving failed: {e}", file=sys.stderr, flush=True)
+            serving = {"error": str(e)}
 
     cpu = None
     x_cpu_err = None
