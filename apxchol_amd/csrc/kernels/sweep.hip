@@ -314,7 +314,6 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
     const double shift = (st && !d.backward) ? st->shift : 0.0;
     const int waves = static_cast<int>(gridDim.x) * kWavesPerBlock;
     const int w = static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
-    bool ok = true;
     for (int b = b0 + w; b < b1; b += waves) sweep_batch<TI, TO, F16, false>(d, b, rhs, out, shift, err);
 }
 
@@ -323,12 +322,12 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
 // state, so the completion test does not add a dependent load; the per-row
 // arithmetic is sweep_batch's (same slots, fma chain and butterfly), so the
 // bits equal the other schedules'.
-template <class TI, class TO, bool F16>
-__global__ __launch_bounds__(kBlock) void sweep_lanes(sweep_desc d, int lb0, int nlb, const lane_rec* __restrict__ rec,
-                                                      const entry32* __restrict__ ent, const TI* __restrict__ rhs,
-                                                      TO* out, const pcg_state* st) {
+template <class TI, class TO, bool F16, int BS = kBlock>
+__global__ __launch_bounds__(BS) void sweep_lanes(sweep_desc d, int lb0, int nlb, const lane_rec* __restrict__ rec,
+                                                  const entry32* __restrict__ ent, const TI* __restrict__ rhs,
+                                                  TO* out, const pcg_state* st) {
     using AT = TO;
-    const int w = static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    const int w = static_cast<int>(blockIdx.x) * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
     if (w >= nlb) return;
     const int lane = threadIdx.x & (kWave - 1);
     const size_t b = static_cast<size_t>(lb0 + w);
