@@ -103,10 +103,10 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                     static_cast<size_t>(sc.tail_batches) * kWave * sizeof(tail_val<TO>) > sc.tail_val_bytes)
                     throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
                 tail_val<TO>* tv = static_cast<tail_val<TO>*>(sc.tail_val);
-                const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
                 tv += static_cast<size_t>(base) * kWave;
-                launch_timed(hook, 2, sweep_couple<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, bf, bl, p0, in,
-                             static_cast<const TO*>(o), tv, st, 1, static_cast<const batch_state*>(nullptr));
+                // One wave per workgroup (a batch each), as the lane levels.
+                launch_timed(hook, 2, sweep_couple<TI, TO, F16, kWave>, dim3(bl - bf), dim3(kWave), 0, s, d, bf, bl, p0,
+                             in, static_cast<const TO*>(o), tv, st, 1, static_cast<const batch_state*>(nullptr));
                 launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
                              base, bl - bf, sc.seg_l1[k] - sc.seg_l0[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
                              static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st, 1,

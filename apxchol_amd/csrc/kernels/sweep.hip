@@ -425,15 +425,15 @@ __device__ __forceinline__ lane_row map_lane(const sweep_desc& d, int b) {
 // bs, when given, stops the launch once no column is active). tv points at
 // the run's first tail batch; column c's values start nb * 64 further on per
 // column (nb = blast - bfirst).
-template <class TI, class TO, bool F16>
-__global__ __launch_bounds__(kBlock) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
-                                                       const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
-                                                       const pcg_state* st, int kp, const batch_state* bs) {
+template <class TI, class TO, bool F16, int BS = kBlock>
+__global__ __launch_bounds__(BS) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
+                                                   const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
+                                                   const pcg_state* st, int kp, const batch_state* bs) {
     if (bs ? bs->active == 0 : (st && st->done)) return;
     const int c = static_cast<int>(blockIdx.y);
     const double shift = (st && !d.backward) ? st[c].shift : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
-    const int b = bfirst + static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    const int b = bfirst + static_cast<int>(blockIdx.x) * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
     if (b >= blast) return;
     const lane_row x = map_lane(d, b);
     const auto value = [&](int j) { return out[static_cast<size_t>(j) * kp + c]; };
