@@ -69,9 +69,7 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         if (lb >= 0) {
             if (static_cast<long long>(lb) + (b1 - b0) > sc.lane_batches)
                 throw std::runtime_error("apxchol_amd: level outside the lane layout");
-            // One wave per workgroup: spreads a level's waves over the most
-            // CUs (measured faster than 4, 8 or 16 waves per workgroup).
-            launch_timed(hook, 0, sweep_lanes<TI, TO, F16, kWave>, dim3(b1 - b0), dim3(kWave), 0, s, d, lb, b1 - b0,
+            launch_timed(hook, 0, sweep_lanes<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, lb, b1 - b0,
                          sc.lane_rec_dev, sc.lane_ent_dev, in, o, st);
             return;
         }
