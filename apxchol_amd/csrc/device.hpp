@@ -124,14 +124,17 @@ struct alignas(16) tail_pair {
     float v1;
 };
 // The walk of a run: for every tail batch (index inside its run) the same
-// wave's next batch in walk order (or -1) and the batch's level inside the
-// run, its wave, and whether it is that wave's first batch.
+// wave's next batch in walk order (or -1) and the batch's barrier phase inside
+// the run, its wave, and whether it is that wave's first batch. Phases follow
+// the levels, except that a level of one batch that follows another level of
+// one batch shares its phase: both are wave 0's, whose LDS writes and reads
+// complete in issue order, so no workgroup barrier separates them.
 inline constexpr int kTailWalkWaveShift = 20;
 inline constexpr int kTailWalkFirst = 1 << 24;
 inline constexpr int kTailWalkMaxLevels = 1 << kTailWalkWaveShift;
 struct alignas(8) tail_walk {
     std::int32_t next;
-    std::int32_t info;   // level | wave << kTailWalkWaveShift | kTailWalkFirst
+    std::int32_t info;   // phase | wave << kTailWalkWaveShift | kTailWalkFirst
 };
 struct tail_desc {
     const tail_lane* lane = nullptr;

@@ -418,22 +418,29 @@ int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, 
     return base;
 }
 
-void append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt) {
+int append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt, bool merge) {
     const int b0 = level_first[l], nb = level_first[e] - b0;
     if (nb <= 0 || e - l > kTailWalkMaxLevels) throw std::runtime_error("apxchol_amd: invalid tail walk");
     const std::size_t at = tt.walk.size();
     tt.walk.resize(at + static_cast<std::size_t>(nb));
     int prev[kTailWaves];
     std::fill(prev, prev + kTailWaves, -1);
-    for (int q = l; q < e; ++q)
+    const auto single = [&](int q) { return level_first[q + 1] - level_first[q] == 1; };
+    int phase = -1;
+    for (int q = l; q < e; ++q) {
+        // Every level of a phase after its first holds one batch, and so does
+        // the first: the whole phase is wave 0's.
+        if (q == l || !merge || !(single(q) && single(q - 1))) ++phase;
         for (int b = level_first[q]; b < level_first[q + 1]; ++b) {
             const int i = b - b0, w = (b - level_first[q]) % kTailWaves;
             tail_walk& x = tt.walk[at + static_cast<std::size_t>(i)];
             x.next = -1;
-            x.info = (q - l) | (w << kTailWalkWaveShift) | (prev[w] < 0 ? kTailWalkFirst : 0);
+            x.info = phase | (w << kTailWalkWaveShift) | (prev[w] < 0 ? kTailWalkFirst : 0);
             if (prev[w] >= 0) tt.walk[at + static_cast<std::size_t>(prev[w])].next = i;
             prev[w] = i;
         }
+    }
+    return phase + 1;
 }
 
 long long append_lane_level(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl,
@@ -487,7 +494,7 @@ struct gpu_solver::device_state {
     dbuf f_tlane, f_tent, f_tovf, f_tovf_ent, f_twalk, b_tlane, b_tent, b_tovf, b_tovf_ent, b_twalk;
     dbuf f_lrec, f_lent, b_lrec, b_lent;
     std::vector<int> f_level_lb, b_level_lb;   // first lane batch of each level, or -1
-    std::vector<int> f_seg[6], b_seg[6];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch
+    std::vector<int> f_seg[7], b_seg[7];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch, barrier phases
     sweep_sched f_sched, b_sched;
     dbuf perm, op_ptr, op_idx, op_val;
     bool op32 = false;
@@ -837,13 +844,16 @@ void gpu_solver::setup(const csc_view& L) {
     // many overflow groups inside the run (0: runs end only at launched levels
     // and at the LDS capacity).
     const int tail_cut = opts_.tail_cut != 0 ? std::max(opts_.tail_cut, 0) : env_flag("APXCHOL_AMD_TAIL_CUT", 1);
+    // Chains of one-batch tail levels share a barrier phase (0: one phase per
+    // level).
+    const bool merge_phases = env_flag("APXCHOL_AMD_TAIL_PHASES", 1) != 0;
     // Launched levels of at most lane_max batches (and no row longer than a
     // wave's chunk) read the static lane layout.
     const int lane_max = opts_.lane_batches < 0 ? 0
                          : opts_.lane_batches > 0 ? opts_.lane_batches
                                                   : env_int("APXCHOL_AMD_LANE_BATCHES", 4096);
     auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
-                          dbuf& lf_dev, std::vector<int> (&seg)[6], dbuf* const (&tbuf)[5], std::vector<int>& level_lb,
+                          dbuf& lf_dev, std::vector<int> (&seg)[7], dbuf* const (&tbuf)[5], std::vector<int>& level_lb,
                           dbuf& lrec, dbuf& lent) {
         lf_dev.upload(lf);
         tail_tables tt;
@@ -880,7 +890,7 @@ void gpu_solver::setup(const csc_view& L) {
         for (int l = 0; l < L;) {
             if (!small(l)) {
                 seg[0].push_back(l); seg[1].push_back(l + 1); seg[2].push_back(0);
-                seg[3].push_back(pos(l)); seg[4].push_back(pos(l + 1)); seg[5].push_back(-1);
+                seg[3].push_back(pos(l)); seg[4].push_back(pos(l + 1)); seg[5].push_back(-1); seg[6].push_back(1);
                 ++l;
                 continue;
             }
@@ -893,9 +903,10 @@ void gpu_solver::setup(const csc_view& L) {
             seg[3].push_back(pos(l)); seg[4].push_back(pos(e));
             if (opts_.schedule == sweep_schedule::hybrid) {
                 seg[5].push_back(append_tail_run(T_, bt, backward, lf[l], lf[e], tt));
-                append_tail_walk(lf, l, e, tt);
+                seg[6].push_back(append_tail_walk(lf, l, e, tt, merge_phases));
             } else {
                 seg[5].push_back(-1);
+                seg[6].push_back(e - l);
             }
             l = e;
         }
@@ -915,6 +926,7 @@ void gpu_solver::setup(const csc_view& L) {
         sc.tail.walk = tbuf[4]->as<tail_walk>();
         sc.tail_batches = tt.batches;
         sc.seg_base = seg[5].data();
+        sc.seg_phases = seg[6].data();
         sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
                                    : opts_.schedule == sweep_schedule::levels ? 1 : 2);
         // Static lane layout of the launched levels.

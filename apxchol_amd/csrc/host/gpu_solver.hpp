@@ -248,8 +248,9 @@ struct tail_tables {
 int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl, tail_tables& tt);
 // Appends the walk entries of the run of levels [l, e) (level_first: first
 // batch of each level): wave w of the tail workgroup takes batches w, w + 16,
-// ... of every level, in level order.
-void append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt);
+// ... of every level, in level order. Returns the run's barrier phases (with
+// merge, a one-batch level after another one-batch level joins its phase).
+int append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt, bool merge = true);
 
 // Static lane layout of launched levels (device.hpp's lane_rec / entry32
 // planes), level by level.

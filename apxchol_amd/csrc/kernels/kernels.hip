@@ -106,7 +106,7 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
                 launch_timed(hook, 2, sweep_couple<TI, TO, F16, kWave>, dim3(bl - bf), dim3(kWave), 0, s, d, bf, bl, p0,
                              in, static_cast<const TO*>(o), tv, st, 1, static_cast<const batch_state*>(nullptr));
                 launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
-                             base, bl - bf, sc.seg_l1[k] - sc.seg_l0[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
+                             base, bl - bf, sc.seg_phases[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
                              static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st, 1,
                              static_cast<const batch_state*>(nullptr));
             } else {
@@ -366,7 +366,7 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
             const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
             sweep_couple<TI, TO, F16><<<dim3(g, b.kp), kBlock, 0, s>>>(d, bf, bl, p0, static_cast<const TI*>(rhs),
                                                                      static_cast<const TO*>(out), tv, st, b.kp, bs);
-            sweep_tail<TO><<<b.kp, kTailBlock, lds, s>>>(sc.tail, base, bl - bf, sc.seg_l1[k] - sc.seg_l0[k], p1 - p0,
+            sweep_tail<TO><<<b.kp, kTailBlock, lds, s>>>(sc.tail, base, bl - bf, sc.seg_phases[k], p1 - p0,
                                                          d.backward ? d.m - 1 - p0 : p0, static_cast<int>(d.backward),
                                                          static_cast<TO*>(out), static_cast<const tail_val<TO>*>(tv),
                                                          st, b.kp, bs);

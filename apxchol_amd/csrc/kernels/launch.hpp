@@ -33,6 +33,7 @@ struct sweep_sched {
     const int* seg_p0 = nullptr;           // tail segments: sweep positions [p0, p1)
     const int* seg_p1 = nullptr;
     const int* seg_base = nullptr;         // tail segments: first tail batch
+    const int* seg_phases = nullptr;       // tail segments: barrier phases of the walk
     int nseg = 0;
     int grid = 0;                          // dataflow grid / cap of a level launch
     tail_desc tail;                        // static layout of the tail runs (device)

@@ -561,9 +561,11 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
     if (it.ln.x >= 0) lds[it.ln.y] = (it.v.y - part) / static_cast<TO>(as_float(it.ln.w));
 }
 
-// base: the run's first tail batch; nb: its batches; nlev: its levels (a
-// level may hold up to kTailMaxBatches batches: wave w takes batches w,
-// w + kTailWaves, ... of it, as the run's walk table lists them); zslot: the
+// base: the run's first tail batch; nb: its batches; nlev: its barrier
+// phases (a level may hold up to kTailMaxBatches batches: wave w takes
+// batches w, w + kTailWaves, ... of it, as the run's walk table lists them; a
+// phase is one level, or a chain of one-batch levels that wave 0 walks with
+// no barrier between them, its LDS accesses completing in order); zslot: the
 // run's row count (its LDS holds zslot + 1 values); output row of slot s:
 // row0 + s (forward) or row0 - s (back). Outputs reach HBM in one coalesced
 // copy at the end.
@@ -580,7 +582,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     extern __shared__ unsigned char tail_lds[];
     TO* lds = reinterpret_cast<TO*>(tail_lds);
     // Past the outputs and the zero slot: the run's walk, as (next batch of
-    // the wave, its level) pairs, and every wave's first batch and level.
+    // the wave, its phase) pairs, and every wave's first batch and phase.
     // The walk is read from LDS, so its lookups never wait behind the item
     // loads in flight (global loads complete in order), and one read gives
     // both halves of a step.
@@ -600,7 +602,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
             first[(x.info >> kTailWalkWaveShift) & (kTailWaves - 1)] = make_int2(q, x.info & (kTailWalkMaxLevels - 1));
     }
     lds_barrier();
-    // Wave w walks its batches in order, passing the level barriers between
+    // Wave w walks its batches in order, passing the phase barriers between
     // them. Two items alternate: after the batch in one is computed, the one
     // two batches ahead is fetched into it, so each item's loads have the
     // next batch's step and every barrier in between to land (no register
@@ -610,7 +612,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     // walk is over) and wave-uniform (readfirstlane of LDS values read at one
     // address), so the walk stays scalar.
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
-    // (next batch, its level) after batch c (c = -1: the walk is over).
+    // (next batch, its phase) after batch c (c = -1: the walk is over).
     const auto step_of = [&](int c) {
         if (c < 0) return make_int2(-1, nlev);
         const int2 e = walk[c];
@@ -625,7 +627,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     tail_item<TO> ia, ib;
     tail_fetch<TO>(td, tv, batch_of(ca), run_batch(ca), ia);
     tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ib);
-    int lvl = 0;   // the level whose rows are being computed
+    int lvl = 0;   // the phase whose rows are being computed
     // Compute the batch in `it` (cursor a), shift the cursors and fetch the
     // batch two ahead into `it` (its walk entry is read before the step, so
     // the lookup overlaps the step). One exit and every fetch on every pass

@@ -342,3 +342,75 @@ def test_host_factorization_through_c_abi_without_gpu():
     A, g = golden("sddm3d_6")
     F = ax.factorize(A, seed=0)
     assert F.n == A.shape[0] and F.nnz == len(g["inner"]) and F.nrounds == len(g["round_is"])
+
+
+# ---- the center-k centring of a Laplacian apply (preconditioner.h:335-396) ----
+
+def _tree_laplacian(n, seed):
+    """A random weighted spanning tree: the approximate Cholesky of a tree is
+    exact (every elimination has at most one neighbour left to sample)."""
+    rng = np.random.default_rng(seed)
+    parent = np.array([rng.integers(0, v) for v in range(1, n)])
+    child = np.arange(1, n)
+    w = np.exp(rng.standard_normal(n - 1))
+    Adj = sp.coo_matrix((np.concatenate([w, w]), (np.concatenate([parent, child]), np.concatenate([child, parent]))),
+                        shape=(n, n)).tocsc()
+    return ax.laplacian(Adj)
+
+
+@pytest.mark.parametrize("src", ["grid2d_12", "rmat_9", "tree_40"])
+def test_center_k_schedule_and_projection(src):
+    """Which applications centre, and what centring is, fixed without the
+    oracle's own formula: with K = 10 the 10th application of a run (after a
+    reset) is the centred operator C M C r (C = I - 1 1^T / n, M the
+    uncentred apply, taken from an oracle with K so large it never centres),
+    and every other one is M r bit for bit; K = 1 centres every application;
+    M grounds the last-eliminated vertex (its output entry is 0) and is
+    symmetric. A dropped mean, a centring of the input only or the output
+    only, or an off-by-one schedule fails one of these."""
+    if src.startswith("tree"):
+        A = _tree_laplacian(int(src.split("_")[1]), 3)
+        F = ax.factorize(A, seed=0)
+        fac = F.export()
+    else:
+        A, g = golden(src)
+        fac = golden_factor(g)
+    n = A.shape[0]
+    perm = np.asarray(fac[3])
+    O10 = oracle.Oracle(A, fac, False, center_k=10)
+    O1 = oracle.Oracle(A, fac, False, center_k=1)
+    Ou = oracle.Oracle(A, fac, False, center_k=1 << 30)
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(n)
+    C = lambda v: v - v.mean()
+    Mr = Ou.apply(r)
+    assert Mr[int(np.flatnonzero(perm == n - 1)[0])] == 0.0
+    cmc = C(Ou.apply(C(r)))
+    for k in range(1, 13):
+        z = O10.apply(r)
+        if k == 10:
+            assert np.linalg.norm(z - cmc) <= 1e-13 * np.linalg.norm(cmc), k
+            assert abs(z.mean()) <= 1e-14 * np.linalg.norm(z)
+        else:
+            assert np.array_equal(bits(z), bits(Mr)), k
+    for _ in range(2):
+        z = O1.apply(r)
+        assert np.linalg.norm(z - cmc) <= 1e-13 * np.linalg.norm(cmc)
+    r2 = rng.standard_normal(n)
+    assert abs(r @ Ou.apply(r2) - r2 @ Mr) <= 1e-12 * np.linalg.norm(r) * np.linalg.norm(Mr)
+
+
+@pytest.mark.parametrize("n,seed", [(30, 1), (200, 2)])
+def test_centred_apply_of_a_tree_is_the_pseudo_inverse(n, seed):
+    """Closed form: on a connected tree the factor is exact (up to its fp32
+    storage), so a centring application C M C equals the Moore-Penrose
+    pseudo-inverse of the Laplacian (numpy's pinv)."""
+    A = _tree_laplacian(n, seed)
+    F = ax.factorize(A, seed=0)
+    O1 = oracle.Oracle(A, F.export(), False, center_k=1)
+    Ap = np.linalg.pinv(A.toarray())
+    R = np.random.default_rng(seed).standard_normal((3, n))
+    for r in R:
+        z = O1.apply(r)
+        ref = Ap @ r
+        assert np.linalg.norm(z - ref) <= 1e-5 * np.linalg.norm(ref)
