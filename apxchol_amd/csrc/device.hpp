@@ -127,7 +127,7 @@ struct alignas(16) tail_pair {
 // wave's next batch in walk order (or -1) and the batch's barrier phase inside
 // the run, its wave, and whether it is that wave's first batch. Phases follow
 // the levels, except that a level of one batch that follows another level of
-// one batch shares its phase: both are wave 0's, whose LDS writes and reads
+// one batch of the same wave shares its phase: one wave's LDS writes and reads
 // complete in issue order, so no workgroup barrier separates them.
 inline constexpr int kTailWalkWaveShift = 20;
 inline constexpr int kTailWalkFirst = 1 << 24;

@@ -418,21 +418,26 @@ int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, 
     return base;
 }
 
-int append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt, bool merge) {
+int append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt, bool merge, bool rotate) {
     const int b0 = level_first[l], nb = level_first[e] - b0;
     if (nb <= 0 || e - l > kTailWalkMaxLevels) throw std::runtime_error("apxchol_amd: invalid tail walk");
     const std::size_t at = tt.walk.size();
     tt.walk.resize(at + static_cast<std::size_t>(nb));
     int prev[kTailWaves];
     std::fill(prev, prev + kTailWaves, -1);
+    // The wave of batch b of level q: with rotate, batches are dealt round
+    // robin over the whole run (wave w takes run batches w, w + 16, ...), so a
+    // wave's next batch lies levels ahead and its prefetched layout has that
+    // long to arrive; without, every level starts again at wave 0.
+    const auto wave_of = [&](int q, int b) { return (rotate ? b - b0 : b - level_first[q]) % kTailWaves; };
     const auto single = [&](int q) { return level_first[q + 1] - level_first[q] == 1; };
     int phase = -1;
     for (int q = l; q < e; ++q) {
-        // Every level of a phase after its first holds one batch, and so does
-        // the first: the whole phase is wave 0's.
-        if (q == l || !merge || !(single(q) && single(q - 1))) ++phase;
+        // Every level of a phase holds one batch, all of the same wave.
+        if (q == l || !merge || !(single(q) && single(q - 1) && wave_of(q, level_first[q]) == wave_of(q - 1, level_first[q - 1])))
+            ++phase;
         for (int b = level_first[q]; b < level_first[q + 1]; ++b) {
-            const int i = b - b0, w = (b - level_first[q]) % kTailWaves;
+            const int i = b - b0, w = wave_of(q, b);
             tail_walk& x = tt.walk[at + static_cast<std::size_t>(i)];
             x.next = -1;
             x.info = phase | (w << kTailWalkWaveShift) | (prev[w] < 0 ? kTailWalkFirst : 0);
@@ -847,6 +852,9 @@ void gpu_solver::setup(const csc_view& L) {
     // Chains of one-batch tail levels share a barrier phase (0: one phase per
     // level).
     const bool merge_phases = env_flag("APXCHOL_AMD_TAIL_PHASES", 1) != 0;
+    // Tail batches dealt to the waves round robin over the run (0: by their
+    // index inside the level).
+    const bool rotate_waves = env_flag("APXCHOL_AMD_TAIL_ROTATE", 1) != 0;
     // Launched levels of at most lane_max batches (and no row longer than a
     // wave's chunk) read the static lane layout.
     const int lane_max = opts_.lane_batches < 0 ? 0
@@ -903,7 +911,7 @@ void gpu_solver::setup(const csc_view& L) {
             seg[3].push_back(pos(l)); seg[4].push_back(pos(e));
             if (opts_.schedule == sweep_schedule::hybrid) {
                 seg[5].push_back(append_tail_run(T_, bt, backward, lf[l], lf[e], tt));
-                seg[6].push_back(append_tail_walk(lf, l, e, tt, merge_phases));
+                seg[6].push_back(append_tail_walk(lf, l, e, tt, merge_phases, rotate_waves));
             } else {
                 seg[5].push_back(-1);
                 seg[6].push_back(e - l);

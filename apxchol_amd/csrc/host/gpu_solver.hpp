@@ -247,10 +247,12 @@ struct tail_tables {
 // kernel relies on (entries before the run form a prefix of the chain).
 int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, int bf, int bl, tail_tables& tt);
 // Appends the walk entries of the run of levels [l, e) (level_first: first
-// batch of each level): wave w of the tail workgroup takes batches w, w + 16,
-// ... of every level, in level order. Returns the run's barrier phases (with
-// merge, a one-batch level after another one-batch level joins its phase).
-int append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt, bool merge = true);
+// batch of each level): with rotate, wave w of the tail workgroup takes the
+// run's batches w, w + 16, ...; without, batches w, w + 16, ... of every
+// level. Returns the run's barrier phases (with merge, a one-batch level after
+// a one-batch level of the same wave joins its phase).
+int append_tail_walk(const std::vector<int>& level_first, int l, int e, tail_tables& tt, bool merge = true,
+                     bool rotate = true);
 
 // Static lane layout of launched levels (device.hpp's lane_rec / entry32
 // planes), level by level.

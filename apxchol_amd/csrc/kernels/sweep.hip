@@ -562,10 +562,11 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
 }
 
 // base: the run's first tail batch; nb: its batches; nlev: its barrier
-// phases (a level may hold up to kTailMaxBatches batches: wave w takes
-// batches w, w + kTailWaves, ... of it, as the run's walk table lists them; a
-// phase is one level, or a chain of one-batch levels that wave 0 walks with
-// no barrier between them, its LDS accesses completing in order); zslot: the
+// phases (a level may hold up to kTailMaxBatches batches; the run's walk
+// table deals each wave its batches, in level order -- by default round
+// robin over the whole run; a phase is one level, or a chain of one-batch
+// levels that one wave walks with no barrier between them, its LDS accesses
+// completing in order); zslot: the
 // run's row count (its LDS holds zslot + 1 values); output row of slot s:
 // row0 + s (forward) or row0 - s (back). Outputs reach HBM in one coalesced
 // copy at the end.
