@@ -159,6 +159,37 @@ def test_tail_cut_splits_runs_at_long_rows(gpu):
     assert sum(runs[1]) > sum(runs[-1]), runs
 
 
+@pytest.mark.parametrize("name", ["sddm3d_16", "rmat_16"])
+@pytest.mark.parametrize("rotate,phases", [("0", "1"), ("0", "0"), ("1", "0")])
+def test_tail_dealing_and_phases_keep_the_bits(gpu, name, rotate, phases, monkeypatch):
+    """How the tail runs deal their batches to the waves (round robin over the
+    run, the default, or level by level) and whether chains of one-batch
+    levels of one wave share a barrier phase only change which wave computes a
+    row and when: both sweeps, in both precisions, keep the default's bits.
+    These factors hold long chains of one-batch levels (rmat_16: ~460 per
+    sweep), the case the shared phases exist for."""
+    import torch
+    A, F, fac = case(name)
+    print(f"[case] tail dealing {name} rotate={rotate} phases={phases}", flush=True)
+    for precision in ("fp64", "fp32"):
+        monkeypatch.delenv("APXCHOL_AMD_TAIL_ROTATE", raising=False)
+        monkeypatch.delenv("APXCHOL_AMD_TAIL_PHASES", raising=False)
+        ref = ax.Solver(A, factor=F, precision=precision)
+        monkeypatch.setenv("APXCHOL_AMD_TAIL_ROTATE", rotate)
+        monkeypatch.setenv("APXCHOL_AMD_TAIL_PHASES", phases)
+        s = ax.Solver(A, factor=F, precision=precision)
+        m = s.info["m"]
+        x = torch.tensor(np.random.default_rng(17).standard_normal(m), device=gpu)
+        for backward in (False, True):
+            a, b = torch.empty_like(x), torch.empty_like(x)
+            ref.sweep(x, a, backward=backward)
+            s.sweep(x, b, backward=backward)
+            torch.cuda.synchronize()
+            assert np.array_equal(a.cpu().numpy().view(np.int64), b.cpu().numpy().view(np.int64)), (precision, backward)
+        ref.close()
+        s.close()
+
+
 @pytest.mark.parametrize("name", ["grid2d_64", "grid2d_256", "sddm3d_16", "aniso3d_16", "rmat_12"])
 def test_apply_matches_cpu_apply(gpu, name):
     """cpu_solver::apply parity over 12 consecutive applications (the center-k
