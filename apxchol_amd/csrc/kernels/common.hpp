@@ -39,18 +39,41 @@ __device__ __forceinline__ double dpp_from_right(double v) {
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// Lane i's copy of lane i ^ OFF (OFF = 16 or 32) by gfx950's VALU row swaps:
+// permlane16_swap(x, x) leaves in its first result the even row of each row
+// pair in both rows, in its second the odd row; permlane32_swap does the same
+// for the two halves. The partner of lane i is the other row (half).
+template <int OFF>
+__device__ __forceinline__ unsigned swap_partner(unsigned v) {
+    const auto r = OFF == 16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false)
+                             : __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & OFF) ? r[0] : r[1];
+}
+template <int OFF>
+__device__ __forceinline__ float swap_partner(float v) {
+    return __builtin_bit_cast(float, swap_partner<OFF>(__builtin_bit_cast(unsigned, v)));
+}
+template <int OFF>
+__device__ __forceinline__ double swap_partner(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = swap_partner<OFF>(static_cast<unsigned>(b & 0xFFFFFFFFull));
+    const unsigned hi = swap_partner<OFF>(static_cast<unsigned>(b >> 32));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 // The butterfly of a lane group's partial sums as the group's first lane
 // sees it: for off = 32, 16, ..., 1 with off < G, lane s adds lane s + off
 // (groups are aligned to their power-of-two size G). Only the first lane of
 // each group holds the group's sum afterwards; its value is the xor
 // butterfly's (v[s] += v[s ^ off] gives lane s < off the same operand).
 // Steps below 16 read the partner with a DPP row shift, the two wider ones
-// with a lane permute; a step no lane needs is skipped as a whole.
+// with a row swap (both stay in the VALU, no LDS round trip); a step no lane
+// needs is skipped as a whole.
 template <int OFF, class T>
 __device__ __forceinline__ T tree_step(T part, int G) {
     if (__ballot(OFF < G) == 0) return part;
     T o;
-    if constexpr (OFF >= 16) o = __shfl_xor(part, OFF, kWave);
+    if constexpr (OFF >= 16) o = swap_partner<OFF>(part);
     else o = dpp_from_right<OFF>(part);
     return OFF < G ? part + o : part;
 }
