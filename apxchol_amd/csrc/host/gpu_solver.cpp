@@ -868,7 +868,8 @@ void gpu_solver::setup(const csc_view& L) {
         const int L = static_cast<int>(lf.size()) - 1;
         const auto pos = [&](int q) { return static_cast<int>(bt.pos[lf[q]]); };
         const auto rows = [&](int q) { return pos(q + 1) - pos(q); };
-        // A tail level has at most one batch per wave of the tail workgroup.
+        // A tail level holds fewer than tail_rows rows in at most
+        // level_tail_batches batches (a few per wave of the tail workgroup).
         const auto small = [&](int q) { return rows(q) < tail_rows && lf[q + 1] - lf[q] <= level_tail_batches; };
         // Overflow groups of level q inside a run starting at position p0: a
         // long row's lane holds every 64th entry, and the tail reads a lane's
