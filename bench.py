@@ -343,7 +343,8 @@ def main() -> int:
         "dtype": "fp32 sweeps + fp64 CG",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "nnz": int(A.nnz), "tol": 1e-8, "factor_seed": 0,
-                   "factor_nnz": F.nnz, "levels": info["levels"], "rhs": "N(0,1) seed 0"},
+                   "factor_nnz": F.nnz, "levels": info["levels"], "rhs": "N(0,1) seed 0",
+                   "iters_per_check": args.iters_per_check},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": per_kernel[dom]["bytes_per_launch"],
