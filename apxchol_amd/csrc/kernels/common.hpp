@@ -42,7 +42,8 @@ __device__ __forceinline__ double dpp_from_right(double v) {
 // Lane i's copy of lane i ^ OFF (OFF = 16 or 32) by gfx950's VALU row swaps:
 // permlane16_swap(x, x) leaves in its first result the even row of each row
 // pair in both rows, in its second the odd row; permlane32_swap does the same
-// for the two halves. The partner of lane i is the other row (half).
+// for the two halves. The partner of lane i is the other row (half). Lane i
+// is threadIdx.x mod 64: every kernel here runs one-dimensional workgroups.
 template <int OFF>
 __device__ __forceinline__ unsigned swap_partner(unsigned v) {
     const auto r = OFF == 16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false)
