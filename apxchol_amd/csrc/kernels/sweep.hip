@@ -630,12 +630,13 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     tail_fetch<TO>(td, tv, batch_of(cb), run_batch(cb), ib);
     int lvl = 0;   // the phase whose rows are being computed
     // Compute the batch in `it` (cursor a), shift the cursors and fetch the
-    // batch two ahead into `it` (its walk entry is read before the step, so
-    // the lookup overlaps the step). One exit and every fetch on every pass
-    // (a finished walk stays at -1), so the item loads in flight are the
-    // same wherever the passes meet.
+    // batch two ahead into `it`. The walk entry is read after the step: read
+    // before it, its result registers were shared with the overflow loads,
+    // and the wait that sharing needs sat ahead of the barrier loop, so a
+    // wave held every barrier up to its batch until its own loads landed.
+    // One exit and every fetch on every pass (a finished walk stays at -1),
+    // so the item loads in flight are the same wherever the passes meet.
     const auto turn = [&](tail_item<TO>& it) {
-        const int2 nx = step_of(cb);
         if (la < nlev) {
             while (lvl < la) {
                 lds_barrier();
@@ -643,6 +644,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
             }
             tail_step<TO>(td, batch_of(ca), it, lds);
         }
+        const int2 nx = step_of(cb);
         ca = cb;
         la = lb;
         cb = nx.x;
