@@ -54,7 +54,7 @@ typedef struct {
     int32_t depth_levels;    /* 1 (default, with relabel): sweep levels = topological depth of the
                                 dependency graph; 0: the elimination rounds (round-as-level) */
     int32_t tail_batches;    /* hybrid: a tail level holds at most this many batches (0 = 48, max 64;
-                                each wave of the tail workgroup takes every 16th) */
+                                the tail workgroup's 16 waves take a run's batches round robin) */
     int32_t tail_cut;        /* hybrid: a tail run ends before a level one of whose lanes would read
                                 at least this many groups of 8 entries inside the run from HBM
                                 (rows of thousands of entries); that level's entries are then summed

@@ -415,3 +415,22 @@ def test_centred_apply_of_a_tree_is_the_pseudo_inverse(n, seed):
         z = O1.apply(r)
         ref = Ap @ r
         assert np.linalg.norm(z - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+# ---- the C ABI's option block ---------------------------------------------------
+
+def test_options_default_fills_exactly_the_python_struct():
+    """apx_options_default writes the documented defaults and nothing past
+    the ctypes mirror of apx_options (a field added on one side only would
+    write past it or leave a field unset)."""
+    import ctypes
+    L = ax.load_library()
+    n = ctypes.sizeof(ax.Options)
+    buf = (ctypes.c_ubyte * (n + 64))(*([0xAB] * (n + 64)))
+    L.apx_options_default(ctypes.cast(buf, ctypes.c_void_p))
+    assert bytes(buf[n:]) == b"\xab" * 64
+    o = ax.Options.from_buffer(buf)
+    assert (o.tol, o.max_iter, o.stagnation_window, o.seed) == (1e-8, 200, 50, 42)
+    assert (o.precision, o.fp16_storage, o.schedule, o.drop_rel, o.keep_factor) == (0, 0, 2, -1.0, 0)
+    assert (o.iters_per_check, o.graph, o.relabel, o.depth_levels) == (2, 1, 1, 1)
+    assert (o.sweep_chunk, o.sweep_blocks, o.tail_rows, o.lane_batches, o.tail_batches, o.tail_cut) == (0,) * 6
