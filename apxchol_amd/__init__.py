@@ -49,7 +49,8 @@ class Options(ctypes.Structure):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", I64), ("residual", F64), ("lumped_offdiag", I64), ("solve_vram_mb", F64),
-                ("status", I32)]
+                ("status", I32), ("factorize_ms", F64), ("sptrsv_setup_ms", F64), ("pcg_setup_ms", F64),
+                ("pcg_ms", F64)]
 
 
 class SolverInfo(ctypes.Structure):
@@ -94,6 +95,14 @@ _SIGS = {
     "apx_solver_reset_apply_count": ([P], ctypes.c_int),
     "apx_solver_sweep": ([P, I32, P, P, P], ctypes.c_int),
     "apx_solver_spmv": ([P, P, P, P], ctypes.c_int),
+    "apx_solver_solve_LLt": ([P, P, P, P], ctypes.c_int),
+    "apx_pcg_blocks": ([I64], I32),
+    "apx_pcg_spmv_lanes_for": ([F64], I32),
+    "apx_pcg_spmv_pAp": ([I32, P, P, P, I32, P, P, P, I32, P], ctypes.c_int),
+    "apx_pcg_update_xr": ([I64, P, P, P, P, F64, P, P], ctypes.c_int),
+    "apx_pcg_dot": ([I64, P, P, P, P], ctypes.c_int),
+    "apx_pcg_update_p": ([I64, P, P, F64, P], ctypes.c_int),
+    "apx_pcg_reduce_partials": ([P, I32, P, P], ctypes.c_int),
     "apx_solver_spmv_batch": ([P, I32, P, P, P], ctypes.c_int),
     "apx_solver_export_factor": ([P, P, P, P, P], ctypes.c_int),
     "apx_solver_set_timing": ([P, I32], ctypes.c_int),
@@ -156,6 +165,40 @@ def _ptr(a) -> int:
     return a.data_ptr()  # torch tensor
 
 
+def _host_vec(a, numel: int, what: str, *, writable: bool = False) -> np.ndarray:
+    """A host vector handed to the library: float64, C-contiguous, exactly
+    `numel` elements. Inputs are converted (a copy when needed); an output
+    array must already have that form, since the library writes through its
+    pointer."""
+    if writable:
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous and a.flags.writeable):
+            raise ValueError(f"{what} must be a writable C-contiguous float64 numpy array")
+    else:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != numel:
+        raise ValueError(f"{what} holds {a.size} values, expected {numel}")
+    return a
+
+
+def _dev_vec(t, numel: int, device, what: str):
+    """A device vector handed to the library by pointer: a torch float64
+    tensor on the solver's HIP device, contiguous, with exactly `numel`
+    elements. Anything else raises ValueError before any launch: the kernels
+    index n (or k * n) doubles from the pointer."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what} must be a torch tensor on the GPU or a numpy array")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{what} must be float64, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous (a strided view would be read as the wrong elements)")
+    if t.numel() != numel:
+        raise ValueError(f"{what} holds {t.numel()} values, expected {numel}")
+    if t.device.type != "cuda" or (device is not None and t.device.index != device):
+        raise ValueError(f"{what} must live on cuda:{device}, got {t.device}")
+    return t
+
+
 def _csc(A):
     """Full-symmetric CSC arrays (int64 colptr, int32 rowidx, float64 vals)."""
     import scipy.sparse as sp
@@ -207,6 +250,30 @@ class SolveResult:
     status: int
     lumped_offdiag: int = 0
     solve_vram_mb: float = -1.0
+    # solve_result::timings: setup/factorize, setup/sptrsv_setup,
+    # setup/gpu_pcg_setup and pcg/gpu_pcg_loop in milliseconds.
+    timings: dict | None = None
+
+
+def _result(x, info) -> SolveResult:
+    return SolveResult(x, int(info.iterations), float(info.residual), int(info.status), int(info.lumped_offdiag),
+                       float(info.solve_vram_mb),
+                       {"factorize_ms": float(info.factorize_ms), "sptrsv_setup_ms": float(info.sptrsv_setup_ms),
+                        "pcg_setup_ms": float(info.pcg_setup_ms), "pcg_ms": float(info.pcg_ms)})
+
+
+def _stream(stream) -> int:
+    import torch
+    return stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+
+
+def _current_device():
+    """Index of the current HIP device as torch sees it (None without torch or a GPU)."""
+    try:
+        import torch
+        return torch.cuda.current_device() if torch.cuda.is_available() else None
+    except ImportError:
+        return None
 
 
 class Factor:
@@ -278,7 +345,9 @@ class Solver:
     """GPU approximate-Cholesky PCG solver: factor built once, solve many b.
 
     Mirrors the reference's ``apxchol.Solver`` (python/apxchol/__init__.py:111-270);
-    the solve runs on the current HIP device.
+    the solve runs on the current HIP device. Device vectors are torch float64
+    tensors on that device, contiguous, of the stated length; host vectors are
+    numpy arrays. Anything else raises ValueError before the library is called.
     """
 
     def __init__(self, A, *, seed: int = 42, factor: Factor | None = None, **opts):
@@ -295,6 +364,7 @@ class Solver:
             _check(lib.apx_solver_create_with_factor(n, cp.ctypes.data, ri.ctypes.data, v.ctypes.data, factor._h,
                                                      ctypes.byref(self._opts), ctypes.byref(h)))
         self._h = h
+        self._device = _current_device()
         self._refresh_info()
 
     def _refresh_info(self):
@@ -319,25 +389,36 @@ class Solver:
     def sddm(self) -> bool:
         return bool(self.info["sddm"])
 
+    @property
+    def m(self) -> int:
+        """Rows of the stored triangular factor L11 (n for SDDM, n - 1 for a Laplacian)."""
+        return int(self.info["m"])
+
+    def _dev(self, t, numel, what):
+        return _dev_vec(t, numel, self._device, what)
+
     def solve(self, b, tol: float | None = None, maxiter: int | None = None, x0=None, out=None,
               stream=None) -> SolveResult:
         lib = load_library()
         tol = -1.0 if tol is None else float(tol)
         maxiter = -1 if maxiter is None else int(maxiter)
         info = SolveInfo()
+        n = self._n
         if isinstance(b, np.ndarray):
-            b = np.ascontiguousarray(b, dtype=np.float64)
-            x = np.zeros(self._n) if out is None else out
-            x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+            b = _host_vec(b, n, "b")
+            x = np.zeros(n) if out is None else _host_vec(out, n, "out", writable=True)
+            x0a = None if x0 is None else _host_vec(x0, n, "x0")
             _check(lib.apx_solver_solve_host(self._h, b.ctypes.data, x.ctypes.data, _ptr(x0a), tol, maxiter,
                                              ctypes.byref(info)))
         else:
             import torch
-            x = torch.empty_like(b) if out is None else out
-            s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+            self._dev(b, n, "b")
+            x = torch.empty_like(b) if out is None else self._dev(out, n, "out")
+            if x0 is not None:
+                self._dev(x0, n, "x0")
+            s = _stream(stream)
             _check(lib.apx_solver_solve(self._h, _ptr(b), _ptr(x), _ptr(x0), tol, maxiter, ctypes.byref(info), s))
-        return SolveResult(x, int(info.iterations), float(info.residual), int(info.status),
-                           int(info.lumped_offdiag), float(info.solve_vram_mb))
+        return _result(x, info)
 
     def solve_batch(self, B, tol: float | None = None, maxiter: int | None = None, out=None,
                     stream=None) -> tuple:
@@ -347,31 +428,31 @@ class Solver:
         lib = load_library()
         tol = -1.0 if tol is None else float(tol)
         maxiter = -1 if maxiter is None else int(maxiter)
+        n = self._n
         if isinstance(B, np.ndarray):
             B = np.ascontiguousarray(B, dtype=np.float64)
+            if B.ndim != 2 or B.shape[1] != n:
+                raise ValueError(f"B must have shape (k, {n}), got {B.shape}")
             k = B.shape[0]
-            X = np.zeros_like(B) if out is None else out
+            X = np.zeros_like(B) if out is None else _host_vec(out, k * n, "out", writable=True)
             infos = (SolveInfo * max(k, 1))()
             _check(lib.apx_solver_solve_batch_host(self._h, k, B.ctypes.data, X.ctypes.data, tol, maxiter, infos))
         else:
             import torch
-            B = B.contiguous()
+            if B.dim() != 2 or B.shape[1] != n:
+                raise ValueError(f"B must have shape (k, {n}), got {tuple(B.shape)}")
             k = B.shape[0]
-            X = torch.empty_like(B) if out is None else out
+            self._dev(B, k * n, "B")
+            X = torch.empty_like(B) if out is None else self._dev(out, k * n, "out")
             infos = (SolveInfo * max(k, 1))()
-            s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-            _check(lib.apx_solver_solve_batch(self._h, k, _ptr(B), _ptr(X), tol, maxiter, infos, s))
-        res = [SolveResult(None, int(i.iterations), float(i.residual), int(i.status), int(i.lumped_offdiag),
-                           float(i.solve_vram_mb)) for i in list(infos)[:k]]
-        return X, res
+            _check(lib.apx_solver_solve_batch(self._h, k, _ptr(B), _ptr(X), tol, maxiter, infos, _stream(stream)))
+        return X, [_result(None, i) for i in list(infos)[:k]]
 
     def autotune(self, max_block_width: int = 64, stream=None) -> dict:
         """Measure and adopt the fastest SpMV lanes; record the best block
         width for blocked solves (see apx_solver_autotune)."""
-        import torch
         r = AutotuneResult()
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        _check(load_library().apx_solver_autotune(self._h, int(max_block_width), ctypes.byref(r), s))
+        _check(load_library().apx_solver_autotune(self._h, int(max_block_width), ctypes.byref(r), _stream(stream)))
         out = {"spmv_lanes": r.spmv_lanes, "block_width": r.block_width, "spmv_us": list(r.spmv_us),
                "sweep_pair_us": list(r.sweep_pair_us), "spmv_sell": r.spmv_sell, "spmv_sell_us": r.spmv_sell_us}
         self._refresh_info()
@@ -381,23 +462,26 @@ class Solver:
         """Preconditioner applied to the k rows of the device tensor R (k, n)
         at once (one application of the centring schedule)."""
         import torch
-        R = R.contiguous()
-        Z = torch.empty_like(R) if out is None else out
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        _check(load_library().apx_solver_apply_batch(self._h, R.shape[0], _ptr(R), _ptr(Z), s))
+        if R.dim() != 2 or R.shape[1] != self._n:
+            raise ValueError(f"R must have shape (k, {self._n}), got {tuple(R.shape)}")
+        k = R.shape[0]
+        self._dev(R, k * self._n, "R")
+        Z = torch.empty_like(R) if out is None else self._dev(out, k * self._n, "out")
+        _check(load_library().apx_solver_apply_batch(self._h, k, _ptr(R), _ptr(Z), _stream(stream)))
         return Z
 
     def apply(self, r, out=None, stream=None):
         lib = load_library()
+        n = self._n
         if isinstance(r, np.ndarray):
-            r = np.ascontiguousarray(r, dtype=np.float64)
-            z = np.zeros(self._n) if out is None else out
+            r = _host_vec(r, n, "r")
+            z = np.zeros(n) if out is None else _host_vec(out, n, "out", writable=True)
             _check(lib.apx_solver_apply_host(self._h, r.ctypes.data, z.ctypes.data))
             return z
         import torch
-        z = torch.empty_like(r) if out is None else out
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        _check(lib.apx_solver_apply(self._h, _ptr(r), _ptr(z), s))
+        self._dev(r, n, "r")
+        z = torch.empty_like(r) if out is None else self._dev(out, n, "out")
+        _check(lib.apx_solver_apply(self._h, _ptr(r), _ptr(z), _stream(stream)))
         return z
 
     TIMING_CLASSES = ("fwd_sweep", "bck_sweep", "spmv", "vector", "sweep_levels", "sweep_tail", "sweep_couple")
@@ -422,27 +506,36 @@ class Solver:
     def spmv(self, x, out=None, stream=None):
         """y = A x with the solver's single-vector SpMV (torch float64 CUDA tensors of length n)."""
         import torch
-        if out is None:
-            out = torch.empty_like(x)
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        _check(load_library().apx_solver_spmv(self._h, _ptr(x), _ptr(out), s))
+        self._dev(x, self._n, "x")
+        out = torch.empty_like(x) if out is None else self._dev(out, self._n, "out")
+        _check(load_library().apx_solver_spmv(self._h, _ptr(x), _ptr(out), _stream(stream)))
         return out
 
     def spmv_batch(self, X, out=None, stream=None):
         """Y = A X for the k rows of X (torch float64 CUDA tensor k x n) with the blocked SpMM."""
         import torch
-        X = X.contiguous()
-        if out is None:
-            out = torch.empty_like(X)
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        _check(load_library().apx_solver_spmv_batch(self._h, int(X.shape[0]), _ptr(X), _ptr(out), s))
+        if X.dim() != 2 or X.shape[1] != self._n:
+            raise ValueError(f"X must have shape (k, {self._n}), got {tuple(X.shape)}")
+        k = X.shape[0]
+        self._dev(X, k * self._n, "X")
+        out = torch.empty_like(X) if out is None else self._dev(out, k * self._n, "out")
+        _check(load_library().apx_solver_spmv_batch(self._h, int(k), _ptr(X), _ptr(out), _stream(stream)))
         return out
 
     def sweep(self, rhs, out, backward: bool, stream=None):
-        """One sweep in factor order (torch float64 CUDA tensors of length m)."""
+        """One sweep in factor order with fp64 vectors (torch float64 CUDA tensors of length m)."""
+        self._dev(rhs, self.m, "rhs")
+        self._dev(out, self.m, "out")
+        _check(load_library().apx_solver_sweep(self._h, int(backward), _ptr(rhs), _ptr(out), _stream(stream)))
+        return out
+
+    def solve_LLt(self, x, out=None, stream=None):
+        """y = L^-T L^-1 x on the stored factor in factor order (length m), the
+        sweeps in the solver's vector precision -- cuda_sptrsv::solve_LLt_dev."""
         import torch
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        _check(load_library().apx_solver_sweep(self._h, int(backward), _ptr(rhs), _ptr(out), s))
+        self._dev(x, self.m, "x")
+        out = torch.empty_like(x) if out is None else self._dev(out, self.m, "out")
+        _check(load_library().apx_solver_solve_LLt(self._h, _ptr(x), _ptr(out), _stream(stream)))
         return out
 
 
@@ -498,8 +591,7 @@ class Server:
         if not ready.value:
             raise TimeoutError(f"request {rid} not done")
         del self._pending[rid]
-        return x, SolveResult(x, int(info.iterations), float(info.residual), int(info.status),
-                              int(info.lumped_offdiag), float(info.solve_vram_mb))
+        return x, _result(x, info)
 
     def stats(self) -> dict:
         st = ServerStats()
@@ -517,10 +609,16 @@ class Server:
 
 
 def solve(A, b, tol: float = 1e-8, maxiter: int = 200, **opts) -> SolveResult:
-    """One-shot apxchol::solve on the GPU (factorize, upload, PCG)."""
-    with_opts = dict(opts)
-    s = Solver(A, **with_opts)
-    try:
-        return s.solve(np.asarray(b, dtype=np.float64), tol=tol, maxiter=maxiter)
-    finally:
-        s.close()
+    """One-shot apxchol::solve (src/solve.cpp:681-737) through apx_solve:
+    factorize on the host, upload, PCG on the current HIP device; b and the
+    returned x are host arrays. The result's timings hold the setup phases and
+    the PCG, as the reference's solve_result::timings."""
+    lib = load_library()
+    n, cp, ri, v = _csc(A)
+    b = _host_vec(b, n, "b")
+    o = _options(tol=tol, maxiter=maxiter, **opts)
+    x = np.zeros(n)
+    info = SolveInfo()
+    _check(lib.apx_solve(n, cp.ctypes.data, ri.ctypes.data, v.ctypes.data, b.ctypes.data, x.ctypes.data,
+                         ctypes.byref(o), ctypes.byref(info)))
+    return _result(x, info)

@@ -4,6 +4,8 @@
 #include "../../include/apxchol_amd.h"
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -15,6 +17,7 @@
 #include "host/mtx.hpp"
 #include "host/server.hpp"
 #include "host/trsv_prep.hpp"
+#include "kernels/launch.hpp"
 
 using namespace apxchol_amd;
 
@@ -84,7 +87,7 @@ solve_options to_opts(const apx_options* o) {
     so.tail_rows = o->tail_rows;
     so.drop_rel = o->drop_rel;
     so.keep_factor_values = o->keep_factor != 0;
-    so.iters_per_check = o->iters_per_check > 0 ? o->iters_per_check : 4;
+    so.iters_per_check = o->iters_per_check > 0 ? o->iters_per_check : 2;
     so.sweep_chunk = o->sweep_chunk;
     so.sweep_blocks = o->sweep_blocks;
     so.graph = o->graph != 0;
@@ -103,6 +106,10 @@ void fill_info(const solve_result& r, apx_solve_info* info) {
     info->lumped_offdiag = r.lumped_offdiag;
     info->solve_vram_mb = r.solve_vram_mb;
     info->status = r.status;
+    info->factorize_ms = r.timings.factorize_ms;
+    info->sptrsv_setup_ms = r.timings.sptrsv_setup_ms;
+    info->pcg_setup_ms = r.timings.pcg_setup_ms;
+    info->pcg_ms = r.timings.pcg_ms;
 }
 
 void need(const void* p, const char* what) {
@@ -438,6 +445,15 @@ apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rh
     });
 }
 
+apx_status apx_solver_solve_LLt(apx_solver_t s, const double* d_x, double* d_y, void* stream) {
+    return guarded([&] {
+        need(s, "solver");
+        need(d_x, "x");
+        need(d_y, "y");
+        s->s->solve_LLt_device(d_x, d_y, static_cast<hipStream_t>(stream));
+    });
+}
+
 apx_status apx_solver_spmv(apx_solver_t s, const double* d_x, double* d_y, void* stream) {
     return guarded([&] {
         need(s, "solver");
@@ -482,6 +498,78 @@ apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* i
         if (inner) std::memcpy(inner, F.inner.data(), F.inner.size() * sizeof(uint32_t));
         if (vals) std::memcpy(vals, F.vals.data(), F.vals.size() * sizeof(float));
         if (perm) std::memcpy(perm, F.perm.data(), F.perm.size() * sizeof(uint32_t));
+    });
+}
+
+// ---- stand-alone PCG pieces ----
+
+namespace {
+int piece_n(int64_t n) {
+    if (n < 0 || n > INT32_MAX) throw std::invalid_argument("n must be in [0, 2^31 - 1]");
+    return static_cast<int>(n);
+}
+}  // namespace
+
+int32_t apx_pcg_blocks(int64_t units) { return pcg_blocks(units); }
+
+int32_t apx_pcg_spmv_lanes_for(double avg_nnz_per_row) {
+    // pcg_cuda_kernels.h:64-69: the smallest power of two >= avg / sqrt(2), <= 32.
+    const double target = avg_nnz_per_row / 1.4142135623730951;
+    int32_t lanes = 1;
+    while (lanes < 32 && static_cast<double>(lanes) < target) lanes <<= 1;
+    return lanes;
+}
+
+apx_status apx_pcg_spmv_pAp(int32_t n, const int32_t* d_rowptr, const int32_t* d_colidx, const void* d_vals,
+                            int32_t vals_fp32, const double* d_x, double* d_y, double* d_part, int32_t lanes,
+                            void* stream) {
+    return guarded([&] {
+        piece_n(n);
+        if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+            throw std::invalid_argument("lanes must be a power of two in [1, 64]");
+        if (n == 0) return;
+        need(d_rowptr, "rowptr"); need(d_colidx, "colidx"); need(d_vals, "vals");
+        need(d_x, "x"); need(d_y, "y"); need(d_part, "part");
+        launch_piece_spmv_pAp(n, reinterpret_cast<const unsigned*>(d_rowptr), d_colidx, d_vals, vals_fp32 != 0, lanes,
+                              d_x, d_y, d_part, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_pcg_update_xr(int64_t n, double* d_x, const double* d_p, double* d_r, const double* d_Ap, double alpha,
+                             double* d_part, void* stream) {
+    return guarded([&] {
+        const int k = piece_n(n);
+        if (k == 0) return;
+        need(d_x, "x"); need(d_p, "p"); need(d_r, "r"); need(d_Ap, "Ap"); need(d_part, "part");
+        launch_piece_update_xr(k, d_x, d_p, d_r, d_Ap, alpha, d_part, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_pcg_dot(int64_t n, const double* d_a, const double* d_b, double* d_part, void* stream) {
+    return guarded([&] {
+        const int k = piece_n(n);
+        if (k == 0) return;
+        need(d_a, "a"); need(d_b, "b"); need(d_part, "part");
+        launch_piece_dot(k, d_a, d_b, d_part, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_pcg_update_p(int64_t n, double* d_p, const double* d_z, double beta, void* stream) {
+    return guarded([&] {
+        const int k = piece_n(n);
+        if (k == 0) return;
+        need(d_p, "p"); need(d_z, "z");
+        launch_piece_update_p(k, d_p, d_z, beta, static_cast<hipStream_t>(stream));
+    });
+}
+
+apx_status apx_pcg_reduce_partials(const double* d_part, int32_t blocks, double* d_out, void* stream) {
+    return guarded([&] {
+        if (blocks < 0 || blocks > APX_PCG_MAX_BLOCKS)
+            throw std::invalid_argument("blocks must be in [0, APX_PCG_MAX_BLOCKS]");
+        need(d_out, "out");
+        if (blocks > 0) need(d_part, "part");
+        launch_piece_reduce(d_part, blocks, d_out, static_cast<hipStream_t>(stream));
     });
 }
 

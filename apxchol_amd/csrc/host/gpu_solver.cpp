@@ -8,9 +8,12 @@
 // flag every iters_per_check iterations.
 #include "gpu_solver.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -634,9 +637,34 @@ void gpu_solver::take_timing(double* ms, std::int64_t* launches) const {
     }
 }
 
+namespace {
+// Graph captures of all solvers of the process, one at a time: several
+// serving lanes capture their first iterations from their own threads, and
+// the runtime's capture bookkeeping is process-wide state.
+std::mutex g_capture_mu;
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+}  // namespace
+
 gpu_solver::gpu_solver(const csc_view& L, const solve_options& opts) : opts_(opts) {
+    const auto t0 = std::chrono::steady_clock::now();
     F_ = factorize(L, opts_.factor_opts);
+    setup_times_.factorize_ms = ms_since(t0);
     setup(L);
+}
+
+void gpu_solver::claim_serving() const {
+    bool expect = false;
+    if (!serving_.compare_exchange_strong(expect, true))
+        throw std::invalid_argument("gpu_solver: this solver already serves a server");
+}
+
+void gpu_solver::not_serving(const char* what) const {
+    if (serving_.load())
+        throw std::invalid_argument(std::string("gpu_solver::") + what +
+                                    ": the solver is serving a server, which owns its device state until the server is destroyed");
 }
 
 gpu_solver::gpu_solver(const csc_view& L, factorization F, const solve_options& opts) : opts_(opts) {
@@ -651,6 +679,7 @@ gpu_solver::gpu_solver(const csc_view& L, factorization F, const solve_options& 
 gpu_solver::~gpu_solver() = default;
 
 void gpu_solver::setup(const csc_view& L) {
+    const auto t_setup = std::chrono::steady_clock::now();
     timer_ = std::make_unique<timer>();
     n_ = L.n;
     if (n_ > (1ll << 31) - 2) throw std::invalid_argument("gpu_solver: n exceeds the 32-bit device index range");
@@ -667,7 +696,9 @@ void gpu_solver::setup(const csc_view& L) {
         for (auto& q : sperm)
             if (q < T_.m) q = relabel_[q];
     }
+    const auto t_op = std::chrono::steady_clock::now();
     const permuted_operator op = build_permuted_operator(L, sperm);
+    setup_times_.pcg_setup_ms = ms_since(t_op);
     if (op.ptr.back() > static_cast<std::int64_t>(UINT32_MAX))
         throw std::invalid_argument("gpu_solver: operator nnz exceeds the 32-bit device index range");
     for (const std::int32_t c : op.idx)
@@ -1053,6 +1084,9 @@ void gpu_solver::setup(const csc_view& L) {
         std::vector<float>().swap(F_.vals);
     }
     hip_check(hipDeviceSynchronize(), "setup");
+    // Everything but the operator build is the SpTRSV side (preparation,
+    // layouts, uploads of both).
+    setup_times_.sptrsv_setup_ms = ms_since(t_setup) - setup_times_.pcg_setup_ms;
 }
 
 void gpu_solver::check_sweep_error() const {
@@ -1115,12 +1149,15 @@ void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
 
 solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol, int max_iter, const double* d_x0,
                                       hipStream_t s) const {
+    not_serving("solve");
+    const auto t_solve = std::chrono::steady_clock::now();
     if (tol < 0.0) tol = opts_.tol;
     if (max_iter < 0) max_iter = opts_.max_iter;
     auto& d = *dev_;
     const int n = static_cast<int>(n_);
     solve_result res;
     res.lumped_offdiag = F_.lumped_offdiag;
+    res.timings = setup_times_;
     if (n == 0) return res;
     pcg_state& h = *d.host_st;
     std::memset(&h, 0, sizeof h);
@@ -1151,6 +1188,7 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     // serves every solve); eager launches while timing.
     const bool graph = opts_.graph && !tm.on;
     if (graph && !d.iter_exec) {
+        std::lock_guard<std::mutex> capture_lock(g_capture_mu);
         if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
@@ -1185,6 +1223,7 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     res.iterations = h.iter;
     res.residual = h.residual;
     res.status = h.status;
+    res.timings.pcg_ms = ms_since(t_solve);
     std::size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess && tot >= fr) res.solve_vram_mb = static_cast<double>(tot - fr) / (1024.0 * 1024.0);
     return res;
@@ -1276,6 +1315,7 @@ void gpu_solver::spmv_block(int k, const double* d_X, double* d_Y, hipStream_t s
 }
 
 void gpu_solver::spmv_batch_device(int k, const double* d_X, double* d_Y, hipStream_t s) const {
+    not_serving("spmv_batch");
     if (k < 0) throw std::invalid_argument("gpu_solver::spmv_batch: negative column count");
     if (n_ == 0 || k == 0) return;
     for (int c0 = 0; c0 < k; c0 += 64)
@@ -1284,6 +1324,7 @@ void gpu_solver::spmv_batch_device(int k, const double* d_X, double* d_Y, hipStr
 }
 
 void gpu_solver::apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const {
+    not_serving("apply_batch");
     if (k < 0) throw std::invalid_argument("gpu_solver::apply_batch: negative column count");
     if (n_ == 0 || k == 0) return;
     // One application of the center-k schedule for all columns.
@@ -1444,6 +1485,7 @@ void gpu_solver::replay_iterations_k(int kp, int count, hipStream_t s) const {
         return;
     }
     if (b.graph_kp != kp) {
+        std::lock_guard<std::mutex> capture_lock(g_capture_mu);
         b.drop_graph();
         if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -1477,6 +1519,7 @@ void gpu_solver::fresh_state(pcg_state& h, double tol, int max_iter) const {
 // ---- autotuner ----------------------------------------------------------
 
 gpu_solver::autotune_result gpu_solver::autotune(int max_block_width, hipStream_t s) const {
+    not_serving("autotune");
     auto& d = *dev_;
     autotune_result res;
     const int n = static_cast<int>(n_);
@@ -1698,6 +1741,7 @@ std::vector<solve_result> gpu_solver::solve_block(int k, const double* d_B, doub
 
 std::vector<solve_result> gpu_solver::solve_batch_device(int k, const double* d_B, double* d_X, double tol,
                                                          int max_iter, hipStream_t s) const {
+    not_serving("solve_batch");
     if (k < 0) throw std::invalid_argument("gpu_solver::solve_batch: negative column count");
     if (tol < 0.0) tol = opts_.tol;
     if (max_iter < 0) max_iter = opts_.max_iter;
@@ -1742,6 +1786,7 @@ solve_result gpu_solver::solve(const std::vector<double>& b, double tol, int max
 }
 
 void gpu_solver::apply_device(const double* d_r, double* d_z, hipStream_t s) const {
+    not_serving("apply");
     auto& d = *dev_;
     const int n = static_cast<int>(n_);
     const int m = static_cast<int>(T_.m);
@@ -1773,6 +1818,7 @@ std::vector<double> gpu_solver::apply(const std::vector<double>& r) const {
 }
 
 void gpu_solver::sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const {
+    not_serving("sweep");
     auto& d = *dev_;
     const int m = static_cast<int>(T_.m);
     if (m == 0) return;
@@ -1791,7 +1837,26 @@ void gpu_solver::sweep_device(bool backward, const double* d_rhs, double* d_out,
     launch_permute(m, Q, d.xout.as<double>(), d_out, false, s);
 }
 
+void gpu_solver::solve_LLt_device(const double* d_x, double* d_y, hipStream_t s) const {
+    not_serving("solve_LLt");
+    auto& d = *dev_;
+    const int m = static_cast<int>(T_.m);
+    if (m == 0) return;
+    // cuda.h:736-785: cast in, forward sweep, back sweep, cast out. The casts
+    // are the permutation kernels here (factor order at the boundary, storage
+    // order inside); the forward sweep reads fp64 and writes the solver's
+    // vector type, so no separate cast pass runs.
+    const unsigned* Q = relabel_.empty() ? nullptr : d.relabel.as<unsigned>();
+    launch_permute(m, Q, d_x, d.rp.as<double>(), true, s);
+    launch_fill_pending(d.prec, d.y.get(), m, s);
+    launch_fill_pending(d.prec, d.z.get(), m, s);
+    launch_sweep(d.fwd, d.prec, d.fp16, d.rp.get(), d.y.get(), nullptr, d.err.as<int>(), d.f_sched, s);
+    launch_sweep(d.bck, d.prec, d.fp16, d.y.get(), d.z.get(), nullptr, d.err.as<int>(), d.b_sched, s);
+    launch_permute_out(d.prec, m, Q, d.z.get(), d_y, s);
+}
+
 void gpu_solver::spmv_device(const double* d_x, double* d_y, hipStream_t s) const {
+    not_serving("spmv");
     auto& d = *dev_;
     const int n = static_cast<int>(n_);
     if (n == 0) return;

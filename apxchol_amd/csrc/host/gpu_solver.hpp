@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -42,6 +43,14 @@ struct solve_options {
                                   // (0: APXCHOL_AMD_LANE_BATCHES or 4096; < 0: never)
 };
 
+// Setup / solve phases in milliseconds (host wall clock).
+struct solve_timings {
+    double factorize_ms = 0.0;      // setup/factorize (0 when the factor was handed in)
+    double sptrsv_setup_ms = 0.0;   // setup/sptrsv_setup: trsv preparation, layouts, device uploads
+    double pcg_setup_ms = 0.0;      // setup/gpu_pcg_setup: the permuted operator (CSR + sliced ELL)
+    double pcg_ms = 0.0;            // pcg/gpu_pcg_loop: the solve itself, final synchronization included
+};
+
 // solve.h:27-42. status: 0 running/none, 1 converged, 2 pAp <= 0 breakdown,
 // 3 stagnation, 4 max_iter, 5 zero right-hand side.
 struct solve_result {
@@ -51,6 +60,11 @@ struct solve_result {
     std::int64_t lumped_offdiag = 0;
     double solve_vram_mb = -1.0;
     int status = 0;
+    // solve_result::timings (the reference's checkpoint tree, solve.h:27-42;
+    // labels of src/solve.cpp:681-737 and preconditioner.h:238-286): the
+    // solver's setup phases (every solve of a solver reports the same) and
+    // this solve's PCG, host wall clock in milliseconds.
+    solve_timings timings;
 };
 
 struct solver_stats {
@@ -149,6 +163,11 @@ public:
     void apply_batch_device(int k, const double* d_R, double* d_Z, hipStream_t s) const;
     // One sweep in factor order with fp64 vectors (rhs, out: m doubles).
     void sweep_device(bool backward, const double* d_rhs, double* d_out, hipStream_t s) const;
+    // The pair y = L^-T L^-1 x of cuda_sptrsv::solve_LLt_dev (cuda.h:480-487,
+    // 736-785) in factor order, m doubles at the boundary, the two sweeps in
+    // the solver's vector precision (fp32 vectors when precision is fp32).
+    // d_x == d_y is allowed.
+    void solve_LLt_device(const double* d_x, double* d_y, hipStream_t s) const;
     // y = A x in the caller's order (n doubles each) with the single SpMV.
     void spmv_device(const double* d_x, double* d_y, hipStream_t s) const;
     // Y = A X for k columns (column c at d_X + c n) with the blocked SpMM,
@@ -166,6 +185,14 @@ public:
     void set_timing(bool on) const;
     // ms[kTimingClasses], launches[kTimingClasses] since the last call; resets them.
     void take_timing(double* ms, std::int64_t* launches) const;
+    // A solver serves one server at a time and, while it does, takes no other
+    // device call (the serving session owns its block buffers, its stream's
+    // graphs and the capture stream). claim_serving throws if it is already
+    // serving; every other device entry point throws while it is.
+    void claim_serving() const;
+    void release_serving() const { serving_.store(false); }
+    bool serving() const { return serving_.load(); }
+    const solve_timings& setup_timings() const { return setup_times_; }
     const factorization& factor() const { return F_; }
     const trsv_factor& trsv() const { return T_; }
     const solver_stats& stats() const { return stats_; }
@@ -176,6 +203,7 @@ private:
     struct timer;
     void setup(const csc_view& L);
     void check_sweep_error() const;
+    void not_serving(const char* what) const;
     // Enqueue PCG iteration `it` (SpMV, x/r update, both sweeps, the
     // preconditioned dot and the p update); step labels the timing records.
     void enqueue_iteration(int step, hipStream_t s) const;
@@ -202,6 +230,8 @@ private:
     std::int64_t n_ = 0;
     int center_k_ = 10;
     mutable std::uint64_t apply_count_ = 0;
+    mutable std::atomic<bool> serving_{false};
+    solve_timings setup_times_;
     std::unique_ptr<device_state> dev_;
     std::unique_ptr<timer> timer_;
 };

@@ -36,14 +36,25 @@ solve_server::~solve_server() {
     for (auto& l : lanes_)
         if (l->worker.joinable()) l->worker.join();
     for (auto& l : lanes_) {
+        if (l->stream) (void)hipStreamSynchronize(l->stream);
         if (l->d_b) (void)hipFree(l->d_b);
         if (l->d_x) (void)hipFree(l->d_x);
+        if (l->h_stage) (void)hipHostFree(l->h_stage);
         if (l->stream) (void)hipStreamDestroy(l->stream);
+        if (l->solver) l->solver->release_serving();
     }
 }
 
 int solve_server::add(std::shared_ptr<gpu_solver> solver) {
     if (!solver) throw std::invalid_argument("server: null solver");
+    // One server, one session per solver: a second add (to this or another
+    // server) or any other device call while it serves would share the
+    // session's block buffers and graphs.
+    solver->claim_serving();
+    struct release_on_throw {
+        const gpu_solver* s;
+        ~release_on_throw() { if (s) s->release_serving(); }
+    } guard{solver.get()};
     // slots = 0: the factor's autotuned block width (32 if not tuned).
     const int want = opts_.slots > 0 ? opts_.slots : (solver->preferred_block_width() > 0 ? solver->preferred_block_width() : 32);
     int kp = 1;
@@ -69,6 +80,12 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
     const std::size_t vb = static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * kp * sizeof(double);
     check(hipMalloc(&ln->d_b, vb), "hipMalloc");
     check(hipMalloc(&ln->d_x, vb), "hipMalloc");
+    // Pinned staging of one vector: every host <-> device copy of the lane is
+    // a DMA from / to it, never a copy from pageable memory (which the
+    // runtime stages internally and may serialize behind other streams).
+    check(hipHostMalloc(&ln->h_stage, static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * sizeof(double),
+                        hipHostMallocDefault),
+          "hipHostMalloc");
     ln->solver->session_begin(kp, ln->stream);
     lane* raw = ln.get();
     int id;
@@ -78,6 +95,7 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
         id = static_cast<int>(lanes_.size()) - 1;
     }
     raw->worker = std::thread([this, raw] { run(*raw); });
+    guard.s = nullptr;   // the lane owns the claim now; ~solve_server releases it
     return id;
 }
 
@@ -180,10 +198,12 @@ void solve_server::run(lane& ln) {
         try {
             if (!admit.empty()) {
                 std::vector<gpu_solver::session_column> cols;
+                double* stage = static_cast<double*>(ln.h_stage);
                 for (auto& [c, rq] : admit) {
                     double* dst = db + static_cast<std::size_t>(c) * n;
-                    check(hipMemcpyAsync(dst, rq->b.data(), n * sizeof(double), hipMemcpyHostToDevice, ln.stream),
-                          "upload b");
+                    std::copy(rq->b.begin(), rq->b.end(), stage);
+                    check(hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyHostToDevice, ln.stream), "upload b");
+                    check(hipStreamSynchronize(ln.stream), "upload b");   // the staging vector is reused
                     cols.push_back({c, dst, rq->tol, rq->max_iter});
                 }
                 ln.solver->session_admit(cols, ln.stream);
@@ -206,10 +226,11 @@ void solve_server::run(lane& ln) {
             std::vector<std::shared_ptr<request>> finished;
             for (std::size_t j = 0; j < fin.size(); ++j) {
                 auto rq = ln.slot[fin[j]];
-                rq->x.resize(n);
-                check(hipMemcpyAsync(rq->x.data(), dst[j], n * sizeof(double), hipMemcpyDeviceToHost, ln.stream),
+                double* stage = static_cast<double*>(ln.h_stage);
+                check(hipMemcpyAsync(stage, dst[j], n * sizeof(double), hipMemcpyDeviceToHost, ln.stream),
                       "download x");
                 check(hipStreamSynchronize(ln.stream), "download x");
+                rq->x.assign(stage, stage + n);
                 const auto& st = states[fin[j]];
                 rq->res.iterations = st.iterations;
                 rq->res.residual = st.residual;

@@ -79,6 +79,7 @@ private:
         std::vector<std::shared_ptr<request>> slot;   // request in each column
         void* d_b = nullptr;                          // slots x n staging (device)
         void* d_x = nullptr;
+        void* h_stage = nullptr;                      // n doubles, pinned host memory
         int kp = 0;
         std::int64_t n = 0;
         std::string error;                            // set when a device failure ended the lane

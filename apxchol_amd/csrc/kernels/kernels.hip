@@ -12,6 +12,7 @@
 #include "pcg.hip"
 #include "sweep.hip"
 #include "block.hip"
+#include "pieces.hip"
 
 namespace apxchol_amd {
 
@@ -540,6 +541,12 @@ void launch_permute(int m, const unsigned* Q, const double* src, double* dst, bo
     check_launch("permute_vec");
 }
 
+void launch_permute_out(vec_prec p, int m, const unsigned* Q, const void* src, double* dst, hipStream_t s) {
+    if (p == vec_prec::fp64) permute_out<double><<<vec_blocks(m), kBlock, 0, s>>>(m, Q, static_cast<const double*>(src), dst);
+    else permute_out<float><<<vec_blocks(m), kBlock, 0, s>>>(m, Q, static_cast<const float*>(src), dst);
+    check_launch("permute_out");
+}
+
 void launch_end_iteration_k(int kp, pcg_state* st, batch_state* bs, hipStream_t s) {
     end_iteration_k<<<1, 64, 0, s>>>(kp, st, bs);
     check_launch("end_iteration_k");
@@ -573,6 +580,66 @@ void launch_gather_k(vec_prec p, int n, int k, int kp, const unsigned* perm, con
     else
         gather_k<float><<<g, kBlock, 0, s>>>(n, k, kp, perm, static_cast<const float*>(x), X, st, use_mean ? 1 : 0);
     check_launch("gather_k");
+}
+
+}  // namespace apxchol_amd
+
+// ---- stand-alone PCG pieces (caller-owned partials) -------------------------
+
+namespace apxchol_amd {
+
+int pcg_blocks(long long units) { return vec_blocks(units); }
+
+namespace {
+template <class VT>
+void piece_spmv_t(int n, const unsigned* ptr, const int* idx, const VT* val, int lanes, const double* x, double* y,
+                  double* part, hipStream_t s) {
+    const int g = vec_blocks(static_cast<long long>(n) * lanes);
+    const pcg_state* none = nullptr;
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(g), dim3(kBlock), 0, s, n, ptr, idx, val, x, y, none, part, 1); };
+    switch (lanes) {
+        case 1: go(spmv_pAp<VT, 1>); break;
+        case 2: go(spmv_pAp<VT, 2>); break;
+        case 4: go(spmv_pAp<VT, 4>); break;
+        case 8: go(spmv_pAp<VT, 8>); break;
+        case 16: go(spmv_pAp<VT, 16>); break;
+        case 32: go(spmv_pAp<VT, 32>); break;
+        case 64: go(spmv_pAp<VT, 64>); break;
+        default: throw std::invalid_argument("apxchol_amd: SpMV lanes must be a power of two in [1, 64]");
+    }
+    check_launch("spmv_pAp");
+}
+}  // namespace
+
+void launch_piece_spmv_pAp(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes,
+                           const double* x, double* y, double* part, hipStream_t s) {
+    if (n <= 0) return;
+    if (val32) piece_spmv_t<float>(n, ptr, idx, static_cast<const float*>(val), lanes, x, y, part, s);
+    else piece_spmv_t<double>(n, ptr, idx, static_cast<const double*>(val), lanes, x, y, part, s);
+}
+
+void launch_piece_update_xr(int n, double* x, const double* p, double* r, const double* ap, double alpha, double* part,
+                            hipStream_t s) {
+    if (n <= 0) return;
+    piece_update_xr<<<vec_blocks(n), kBlock, 0, s>>>(n, x, p, r, ap, alpha, part);
+    check_launch("piece_update_xr");
+}
+
+void launch_piece_dot(int n, const double* a, const double* b, double* part, hipStream_t s) {
+    if (n <= 0) return;
+    piece_dot<<<vec_blocks(n), kBlock, 0, s>>>(n, a, b, part);
+    check_launch("piece_dot");
+}
+
+void launch_piece_update_p(int n, double* p, const double* z, double beta, hipStream_t s) {
+    if (n <= 0) return;
+    piece_update_p<<<vec_blocks(n), kBlock, 0, s>>>(n, p, z, beta);
+    check_launch("piece_update_p");
+}
+
+void launch_piece_reduce(const double* part, int nb, double* out, hipStream_t s) {
+    piece_reduce<<<1, kBlock, 0, s>>>(part, nb, out);
+    check_launch("piece_reduce");
 }
 
 }  // namespace apxchol_amd

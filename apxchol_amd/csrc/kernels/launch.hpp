@@ -93,7 +93,11 @@ void launch_sum_out(vec_prec p, int n, const void* a, pcg_state* st, double* par
 void launch_apply_gather(vec_prec p, int n, const unsigned* perm, const void* out, double* z, const pcg_state* st,
                          bool center, hipStream_t s);
 void launch_fill_pending(vec_prec p, void* a, int n, hipStream_t s);
+// Between factor order and storage order (Q: factor position -> storage
+// position; null: the identity): in, dst[Q[q]] = src[q]; out, dst[q] = src[Q[q]].
 void launch_permute(int m, const unsigned* Q, const double* src, double* dst, bool in, hipStream_t s);
+// dst[q] = double(src[Q[q]]) from a sweep output of the given precision.
+void launch_permute_out(vec_prec p, int m, const unsigned* Q, const void* src, double* dst, hipStream_t s);
 
 // ---- blocked solves: kp columns (a power of two <= 64), row-major blocks ----
 
@@ -156,5 +160,22 @@ void launch_gather_cols_k(int n, int kp, const unsigned* perm, int ncol, const i
                           const double* x, const pcg_state* st, bool use_mean, hipStream_t s);
 void launch_gather_k(vec_prec p, int n, int k, int kp, const unsigned* perm, const void* x, double* X,
                      const pcg_state* st, bool use_mean, hipStream_t s);
+
+// ---- stand-alone PCG pieces (pcg_cuda_kernels.h:43-93): caller-owned partials,
+// one double per producer workgroup, pcg_blocks(units) of them ----
+int pcg_blocks(long long units);
+// y = A x over a full CSR (lanes per row: a power of two <= 64) and the
+// partials of x.y, pcg_blocks(n * lanes).
+void launch_piece_spmv_pAp(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes,
+                           const double* x, double* y, double* part, hipStream_t s);
+// x += alpha p; r -= alpha Ap; partials of r.r (pcg_blocks(n)).
+void launch_piece_update_xr(int n, double* x, const double* p, double* r, const double* ap, double alpha, double* part,
+                            hipStream_t s);
+// Partials of a.b (pcg_blocks(n)).
+void launch_piece_dot(int n, const double* a, const double* b, double* part, hipStream_t s);
+// p = z + beta p.
+void launch_piece_update_p(int n, double* p, const double* z, double beta, hipStream_t s);
+// out[0] = the fixed-order sum of part[0..nb).
+void launch_piece_reduce(const double* part, int nb, double* out, hipStream_t s);
 
 }  // namespace apxchol_amd

@@ -397,14 +397,24 @@ __global__ __launch_bounds__(kBlock) void pcg_finish(int op, int n, const double
 }
 
 // Between factor order and storage order: dst[Q[q]] = src[q] (in) or
-// dst[q] = src[Q[q]] (out), q < m.
+// dst[q] = src[Q[q]] (out), q < m; Q null: the identity.
 __global__ __launch_bounds__(kBlock) void permute_vec(int m, const unsigned* __restrict__ Q,
                                                       const double* __restrict__ src, double* __restrict__ dst,
                                                       int in) {
     for (int q = blockIdx.x * kBlock + threadIdx.x; q < m; q += gridDim.x * kBlock) {
-        if (in) dst[Q[q]] = src[q];
-        else dst[q] = src[Q[q]];
+        const unsigned t = Q ? Q[q] : static_cast<unsigned>(q);
+        if (in) dst[t] = src[q];
+        else dst[q] = src[t];
     }
+}
+
+// dst[q] = double(src[Q[q]]): a sweep output (fp64 or fp32) back to factor
+// order and fp64 (the boundary cast of cuda_cast.cu:24-34).
+template <class TO>
+__global__ __launch_bounds__(kBlock) void permute_out(int m, const unsigned* __restrict__ Q, const TO* __restrict__ src,
+                                                      double* __restrict__ dst) {
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < m; q += gridDim.x * kBlock)
+        dst[q] = static_cast<double>(src[Q ? Q[q] : static_cast<unsigned>(q)]);
 }
 
 }  // namespace apxchol_amd

@@ -48,6 +48,19 @@ def spmv_bytes(info: dict) -> float:
     return nnz * (4 + (4 if info["op_fp32"] else 8)) + 4 * (n + 1) + 16 * n
 
 
+def kernel_source_digest() -> str:
+    """sha256 over the HIP kernel sources and the shared device layouts: a
+    traffic profile applies to the kernels it was measured on only."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = ROOT / "apxchol_amd" / "csrc"
+    for f in sorted(list((csrc / "kernels").glob("*")) + [csrc / "device.hpp"]):
+        if f.suffix in (".hip", ".hpp"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()
+
+
 def percentile(xs, q):
     s = sorted(xs)
     if not s:
@@ -73,6 +86,8 @@ def run_serving(args, ax, mg, np) -> dict:
         mats.append((name, L, F.sddm))
         fids.append(srv.add(S))
         solvers.append(S)
+        print(f"[bench] serving: factor {name} (n={L.shape[0]}) resident after {time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
     t_setup = time.perf_counter() - t0
     print(f"[bench] serving: {len(mats)} factors resident ({t_setup:.1f}s)", file=sys.stderr, flush=True)
     rng = np.random.default_rng(6)
@@ -103,10 +118,21 @@ def run_serving(args, ax, mg, np) -> dict:
     print(f"[bench] serving: {len(plan)} requests submitted", file=sys.stderr, flush=True)
     worst = 0.0
     for i, (f, tol, _) in enumerate(plan):
-        x, r = srv.wait(ids[i], timeout=300)
+        # A request the server has not finished within 120 s ends the
+        # measurement with that reason (the serving section records it).
+        try:
+            x, r = srv.wait(ids[i], timeout=120)
+        except TimeoutError:
+            st = srv.stats()
+            srv.close()
+            raise RuntimeError(f"request {i} (factor {mats[f][0]}) not done after 120 s; "
+                               f"{st['completed']} of {st['submitted']} completed")
         L = mats[f][1]
         tr = float(np.linalg.norm(rhs[i] - L @ x) / np.linalg.norm(rhs[i]))
         worst = max(worst, tr / tol)
+        if (i + 1) % 64 == 0:
+            print(f"[bench] serving: {i + 1} of {len(plan)} results graded ({time.perf_counter() - t1:.1f}s)",
+                  file=sys.stderr, flush=True)
     wall = time.perf_counter() - t1
     st = srv.stats()
     srv.close()
@@ -173,7 +199,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sddm3d_128")
-    ap.add_argument("--iters-per-check", type=int, default=4)
+    ap.add_argument("--iters-per-check", type=int, default=2, help="as the library default (apx_options_default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--timing-solves", type=int, default=2)
@@ -203,6 +229,7 @@ def main() -> int:
     t_gen = time.perf_counter() - t0
     n = A.shape[0]
     print(f"[bench] workload {args.workload}: n={n} nnz={A.nnz} generated in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    print(f"[bench] host factorization of {args.workload} starts", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     F = ax.factorize(A, seed=0)
     t_fact = time.perf_counter() - t0
@@ -278,11 +305,12 @@ def main() -> int:
     achieved = per_kernel[dom]["gbps"]
     # HBM bytes per launch of that kernel from the PMC passes recorded with
     # the profiles (FETCH_SIZE / WRITE_SIZE, copy-calibrated), when present.
+    # Only a profile measured on these kernel sources counts (its digest).
     traffic = None
     tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
     if tf.exists():
         tj = json.loads(tf.read_text())
-        if tj.get("kernel") == dom:
+        if tj.get("kernel") == dom and tj.get("kernel_sources_sha256") == kernel_source_digest():
             traffic = tj["bytes_per_launch"]
     # Whole-sweep SpTRSV rate (level launches + tail runs, gaps included).
     sweep_gbps = {}

@@ -72,6 +72,14 @@ typedef struct {
     int64_t lumped_offdiag;
     double solve_vram_mb;
     int32_t status;
+    /* solve_result::timings (the checkpoint tree of src/solve.cpp:681-737 and
+     * preconditioner.h:238-286), host wall clock in milliseconds. The setup
+     * phases are the solver's (the same for every solve of one solver);
+     * factorize_ms is 0 when the solver was created with a given factor. */
+    double factorize_ms;     /* setup/factorize */
+    double sptrsv_setup_ms;  /* setup/sptrsv_setup: trsv preparation, device layouts and uploads */
+    double pcg_setup_ms;     /* setup/gpu_pcg_setup: the permuted operator (CSR + sliced ELL) */
+    double pcg_ms;           /* pcg/gpu_pcg_loop: this solve, its final synchronization included */
 } apx_solve_info;
 
 typedef struct {
@@ -226,6 +234,14 @@ apx_status apx_solver_reset_apply_count(apx_solver_t s);
  * and omp_sptrsv::forward_solve / transpose_solve (omp.h:1048-1058). d_rhs, d_out: m doubles. */
 apx_status apx_solver_sweep(apx_solver_t s, int32_t backward, const double* d_rhs, double* d_out, void* stream);
 
+/* The sweep pair y = L^-T L^-1 x on the stored L11 in factor order (m
+ * doubles each; d_x == d_y allowed), the two sweeps in the solver's vector
+ * precision -- fp32 vectors and accumulation when the solver was created with
+ * precision = 1, fp64 otherwise -- with fp64 at the boundary:
+ * cuda_sptrsv::solve_LLt_dev (include/apxchol/solver/sptrsv/cuda.h:480-487,
+ * 736-785), the cast kernels folded into the boundary permutations. */
+apx_status apx_solver_solve_LLt(apx_solver_t s, const double* d_x, double* d_y, void* stream);
+
 /* y = L x in the caller's order with the solver's single-vector SpMV (sliced
  * ELL or CSR lanes, fp64 accumulation) -- the device boundary of
  * pcg_cuda::spmv_pAp (include/apxchol/solver/pcg_cuda_kernels.h:75-93) without
@@ -255,6 +271,36 @@ apx_status apx_solver_export_factor(apx_solver_t s, uint64_t* outer, uint32_t* i
 /* apxchol::solve(L, b, opts) one-shot (solve.h:136-138, src/solve.cpp:681-737), host vectors. */
 apx_status apx_solve(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals, const double* b,
                      double* x, const apx_options* opts, apx_solve_info* info);
+
+/* ---- stand-alone PCG pieces on caller-owned device arrays ------------------
+ * The device boundary of the reference's GPU PCG
+ * (include/apxchol/solver/pcg_cuda_kernels.h:43-93, src/cuda_pcg_kernels.cu:18-152):
+ * every reduction writes one partial per workgroup of its producer into the
+ * caller's `d_part` (at least APX_PCG_MAX_BLOCKS doubles), over a grid that
+ * is a pure function of the size (apx_pcg_blocks), and apx_pcg_reduce_partials
+ * sums them in a fixed order -- bit-identical run to run. */
+#define APX_PCG_MAX_BLOCKS 2048
+/* pcg_blocks(units): min(ceil(units / 256), APX_PCG_MAX_BLOCKS), at least 1. */
+int32_t apx_pcg_blocks(int64_t units);
+/* spmv_lanes_for(avg): the power of two nearest avg_nnz_per_row in log2, in [1, 32]. */
+int32_t apx_pcg_spmv_lanes_for(double avg_nnz_per_row);
+/* spmv_pAp: y = A x over a full CSR (d_rowptr n+1 non-negative int32,
+ * d_colidx nnz int32, d_vals nnz fp64, or fp32 when vals_fp32 = 1, each
+ * product promoted to fp64), `lanes` threads per row (a power of two <= 64),
+ * and the partials of x.y into d_part[0 .. apx_pcg_blocks(n * lanes)). */
+apx_status apx_pcg_spmv_pAp(int32_t n, const int32_t* d_rowptr, const int32_t* d_colidx, const void* d_vals,
+                            int32_t vals_fp32, const double* d_x, double* d_y, double* d_part, int32_t lanes,
+                            void* stream);
+/* update_xr: x += alpha p; r -= alpha Ap; partials of r.r (the updated r)
+ * into d_part[0 .. apx_pcg_blocks(n)). */
+apx_status apx_pcg_update_xr(int64_t n, double* d_x, const double* d_p, double* d_r, const double* d_Ap, double alpha,
+                             double* d_part, void* stream);
+/* dot: partials of a.b into d_part[0 .. apx_pcg_blocks(n)). */
+apx_status apx_pcg_dot(int64_t n, const double* d_a, const double* d_b, double* d_part, void* stream);
+/* update_p: p = z + beta p. */
+apx_status apx_pcg_update_p(int64_t n, double* d_p, const double* d_z, double beta, void* stream);
+/* reduce_partials: d_out[0] = the fixed-order sum of d_part[0 .. blocks). */
+apx_status apx_pcg_reduce_partials(const double* d_part, int32_t blocks, double* d_out, void* stream);
 
 /* ---- MatrixMarket files and the reading of an input matrix -----------------
  * The reference CLI's input layer (src/main.cpp:14-98 with the

@@ -19,9 +19,10 @@
 //   operator build ........... src/solve.cpp:320-444 (row-major full symmetric, fp32 iff exact)
 //   PCG ...................... src/solve.cpp:93-269, 478-654
 //
-// Parity status of each piece is recorded in the test files that check it
-// (tests/test_oracle_ref.py against the compiled reference, tests/test_oracle_math.py
-// against closed forms).
+// Parity status of each piece is recorded in the tests that check it:
+// tests/test_cpu_parity.py (against the reference's own headers compiled into
+// oracle/_ref, its fixture files and known answers, and closed forms) and
+// tests/test_cpu_reference_cells.py (iteration counts the reference published).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>

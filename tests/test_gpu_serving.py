@@ -111,3 +111,33 @@ def test_serving_memory_budget(gpu):
     with pytest.raises(RuntimeError, match="memory budget"):
         srv.add(ax.Solver(A, factor=F))
     srv.close()
+
+
+def test_a_serving_solver_takes_no_other_call(gpu):
+    """A solver serves one server at a time and takes no other device call
+    while it does (the session owns its block buffers, graphs and capture
+    stream): a second add -- to the same or another server -- and every
+    direct call are refused before any launch; the server keeps serving it,
+    and once the server is gone the solver works again."""
+    import torch
+    A, F = factor("sddm3d_16")
+    n = A.shape[0]
+    s = ax.Solver(A, factor=F, precision="fp32")
+    srv = ax.Server(slots=8)
+    fid = srv.add(s)
+    other = ax.Server(slots=8)
+    for target in (other, srv):
+        with pytest.raises(ValueError, match="already serves"):
+            target.add(s)
+    b = torch.tensor(np.ones(n), device=gpu)
+    B = torch.ones((2, n), dtype=torch.float64, device=gpu)
+    for call in (lambda: s.solve(b), lambda: s.solve_batch(B), lambda: s.apply(b), lambda: s.apply_batch(B),
+                 lambda: s.spmv(b), lambda: s.spmv_batch(B), lambda: s.solve_LLt(b), lambda: s.autotune()):
+        with pytest.raises(ValueError, match="serving"):
+            call()
+    x, r = srv.wait(srv.submit(fid, np.ones(n), 1e-8), timeout=120)
+    assert r.status == 1 and np.linalg.norm(np.ones(n) - A @ x) <= 2e-8 * np.sqrt(n)
+    srv.close()
+    other.close()
+    assert s.solve(b).status == 1
+    s.close()
