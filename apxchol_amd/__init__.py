@@ -99,6 +99,7 @@ _SIGS = {
     "apx_pcg_blocks": ([I64], I32),
     "apx_pcg_spmv_lanes_for": ([F64], I32),
     "apx_pcg_spmv_pAp": ([I32, P, P, P, I32, P, P, P, I32, P], ctypes.c_int),
+    "apx_pcg_spmv_pAp_noalloc": ([I32, P, P, P, I32, P, P, P, I32, P], ctypes.c_int),
     "apx_pcg_update_xr": ([I64, P, P, P, P, F64, P, P], ctypes.c_int),
     "apx_pcg_dot": ([I64, P, P, P, P], ctypes.c_int),
     "apx_pcg_update_p": ([I64, P, P, F64, P], ctypes.c_int),

@@ -535,6 +535,23 @@ apx_status apx_pcg_spmv_pAp(int32_t n, const int32_t* d_rowptr, const int32_t* d
     });
 }
 
+apx_status apx_pcg_spmv_pAp_noalloc(int32_t n, const int32_t* d_rowptr, const int32_t* d_colidx, const void* d_vals,
+                                    int32_t vals_fp32, const double* d_x, double* d_y, double* d_pAp, int32_t lanes,
+                                    void* stream) {
+    return guarded([&] {
+        piece_n(n);
+        if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+            throw std::invalid_argument("lanes must be a power of two in [1, 64]");
+        need(d_pAp, "pAp");
+        if (n > 0) {
+            need(d_rowptr, "rowptr"); need(d_colidx, "colidx"); need(d_vals, "vals");
+            need(d_x, "x"); need(d_y, "y");
+        }
+        launch_piece_spmv_pAp_scratch(n, reinterpret_cast<const unsigned*>(d_rowptr), d_colidx, d_vals, vals_fp32 != 0,
+                                      lanes, d_x, d_y, d_pAp, static_cast<hipStream_t>(stream));
+    });
+}
+
 apx_status apx_pcg_update_xr(int64_t n, double* d_x, const double* d_p, double* d_r, const double* d_Ap, double alpha,
                              double* d_part, void* stream) {
     return guarded([&] {

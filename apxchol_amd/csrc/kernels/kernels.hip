@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -640,6 +641,65 @@ void launch_piece_update_p(int n, double* p, const double* z, double beta, hipSt
 void launch_piece_reduce(const double* part, int nb, double* out, hipStream_t s) {
     piece_reduce<<<1, kBlock, 0, s>>>(part, nb, out);
     check_launch("piece_reduce");
+}
+
+}  // namespace apxchol_amd
+
+// ---- allocation-free y = A x with x.y (library-owned partials) ---------------
+//
+// The partials of the SpMV's reduction live in `scratch`, a file-scope device
+// array: every GPU the library runs on holds its own copy, from module load to
+// the end of the process, so the entry point below allocates nothing. On the
+// host the name `scratch` is only a stand-in for the symbol (passing it to a
+// kernel would hand the kernel a host address): host code uses the address
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device on the first call there. Every call on one device shares that
+// device's array and leaves the last call's partials in it; calls that
+// overlap on one GPU (unordered streams) race. Nothing in it needs to start
+// at zero: each workgroup of the producer writes its own partial.
+
+namespace apxchol_amd {
+
+inline constexpr int kScratchDoubles = kMaxBlocks;   // 16 KiB: the largest producer grid's partials
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+inline constexpr int kScratchDevices = 64;
+
+double* scratch_partials() {
+    int dev = -1;
+    const hipError_t e0 = hipGetDevice(&dev);
+    if (e0 != hipSuccess) throw std::runtime_error(std::string("apxchol_amd: hipGetDevice failed: ") + hipGetErrorString(e0));
+    if (dev < 0 || dev >= kScratchDevices)
+        throw std::invalid_argument("apxchol_amd: the allocation-free entry point supports device ordinals below 64");
+    static std::mutex mu;
+    static double* addr[kScratchDevices] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("apxchol_amd: hipGetSymbolAddress(scratch) failed: ") +
+                                     hipGetErrorString(e));
+        addr[dev] = static_cast<double*>(p);
+    }
+    return addr[dev];
+}
+}  // namespace
+
+void launch_piece_spmv_pAp_scratch(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes,
+                                   const double* x, double* y, double* pAp, hipStream_t s) {
+    double* part = scratch_partials();
+    if (n <= 0) {   // no rows: x.y = 0
+        launch_piece_reduce(part, 0, pAp, s);
+        return;
+    }
+    const int nb = pcg_blocks(static_cast<long long>(n) * lanes);
+    if (nb > kScratchDoubles)
+        throw std::invalid_argument("apxchol_amd: " + std::to_string(nb) + " partials exceed the scratch array's " +
+                                    std::to_string(kScratchDoubles));
+    launch_piece_spmv_pAp(n, ptr, idx, val, val32, lanes, x, y, part, s);
+    launch_piece_reduce(part, nb, pAp, s);
 }
 
 }  // namespace apxchol_amd

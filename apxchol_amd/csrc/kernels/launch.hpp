@@ -177,5 +177,11 @@ void launch_piece_dot(int n, const double* a, const double* b, double* part, hip
 void launch_piece_update_p(int n, double* p, const double* z, double beta, hipStream_t s);
 // out[0] = the fixed-order sum of part[0..nb).
 void launch_piece_reduce(const double* part, int nb, double* out, hipStream_t s);
+// launch_piece_spmv_pAp then launch_piece_reduce with the partials in the
+// library's per-device scratch array (allocates nothing): y = A x and
+// pAp[0] = x.y, the bits of the two-step form. Calls that overlap on one
+// device race on the scratch.
+void launch_piece_spmv_pAp_scratch(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes,
+                                   const double* x, double* y, double* pAp, hipStream_t s);
 
 }  // namespace apxchol_amd

@@ -97,3 +97,26 @@ def reduce_partials(part, nblocks: int, out, stream=None) -> None:
     if not isinstance(out, torch.Tensor) or out.dtype != torch.float64 or out.device.type != "cuda" or out.numel() < 1:
         raise ValueError("out must be a float64 CUDA tensor with at least one element")
     _check(load_library().apx_pcg_reduce_partials(_ptr(part), int(nblocks), _ptr(out), _stream(stream)))
+
+
+def spmv_pAp_noalloc(rowptr, colidx, vals, x, y, pAp, lanes: int, stream=None) -> None:
+    """y = A x and pAp[0] = x.y with no partials buffer: the library's
+    per-device scratch array holds them (apx_pcg_spmv_pAp_noalloc). The same
+    bits as spmv_pAp followed by reduce_partials; calls that overlap on one
+    GPU race on the scratch array."""
+    import torch
+    n = x.numel()
+    _idx(rowptr, n + 1, "rowptr")
+    nnz = int(rowptr[-1].item()) if n > 0 else 0
+    _idx(colidx, nnz, "colidx")
+    if not isinstance(vals, torch.Tensor) or vals.dtype not in (torch.float64, torch.float32) or \
+            vals.device.type != "cuda" or not vals.is_contiguous() or vals.numel() != nnz:
+        raise ValueError(f"vals must be a contiguous float64 / float32 CUDA tensor of {nnz} values")
+    _dev(x, n, "x")
+    _dev(y, n, "y")
+    if not isinstance(pAp, torch.Tensor) or pAp.dtype != torch.float64 or pAp.device.type != "cuda" or \
+            pAp.numel() < 1:
+        raise ValueError("pAp must be a float64 CUDA tensor with at least one element")
+    _check(load_library().apx_pcg_spmv_pAp_noalloc(n, _ptr(rowptr), _ptr(colidx), _ptr(vals),
+                                                   int(vals.dtype == torch.float32), _ptr(x), _ptr(y), _ptr(pAp),
+                                                   int(lanes), _stream(stream)))

@@ -301,6 +301,19 @@ apx_status apx_pcg_dot(int64_t n, const double* d_a, const double* d_b, double* 
 apx_status apx_pcg_update_p(int64_t n, double* d_p, const double* d_z, double beta, void* stream);
 /* reduce_partials: d_out[0] = the fixed-order sum of d_part[0 .. blocks). */
 apx_status apx_pcg_reduce_partials(const double* d_part, int32_t blocks, double* d_out, void* stream);
+/* spmv_pAp_noalloc: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc). y = A x as apx_pcg_spmv_pAp, and
+ * d_pAp[0] = x.y, the bits of apx_pcg_spmv_pAp followed by
+ * apx_pcg_reduce_partials. The partials live in a library-owned device array
+ * of APX_PCG_MAX_BLOCKS doubles (16 KiB), one per GPU, held until the process
+ * ends; it covers every n (the producer grid never exceeds it), and a grid
+ * beyond it is refused with APX_ERR_INVALID. Allocates nothing. Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (streams
+ * with no order between them) race. n = 0 writes d_pAp[0] = 0. */
+apx_status apx_pcg_spmv_pAp_noalloc(int32_t n, const int32_t* d_rowptr, const int32_t* d_colidx, const void* d_vals,
+                                    int32_t vals_fp32, const double* d_x, double* d_y, double* d_pAp, int32_t lanes,
+                                    void* stream);
 
 /* ---- MatrixMarket files and the reading of an input matrix -----------------
  * The reference CLI's input layer (src/main.cpp:14-98 with the
