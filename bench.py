@@ -17,6 +17,13 @@ its plain fp64 single-vector rate, and on the 64^3 anisotropic grid
 (2-D grids 1e4..1e6, 3-D grids to 96^3, R-MAT scales 16..20) under Poisson
 arrivals and reports p50 / p99 latency and solves/s.
 
+The CPU baseline is the OpenMP form of the CPU restatement of the reference's
+solve path (level-set sweeps with thin levels on one thread and fat levels
+split over the team, fused PCG passes with thread-ordered partial sums),
+timed first, in a child process, on every physical core this process may use
+(capped by OMP_NUM_THREADS), bound and unbound; setup and the median of three
+solves are reported apart. The one-thread oracle's solve is the checker.
+
 The timed solves replay each PCG iteration as a captured HIP graph. After
 them, a timing pass of --timing-solves solves launches eagerly with HIP events
 around every kernel on the solve's stream; the roofline object covers the
@@ -144,6 +151,46 @@ def run_serving(args, ax, mg, np) -> dict:
             "worst_true_residual_over_tol": worst, "memory_used_bytes": st["memory_used"]}
 
 
+def run_cpu_baseline(args) -> dict | None:
+    """The OpenMP CPU path (oracle/cpu_baseline.py) in a child process on
+    every physical core this process may use (capped by OMP_NUM_THREADS):
+    once with the threads bound close to one core each (OMP_PROC_BIND=close,
+    OMP_PLACES=cores) and once unbound, setup and the median of three solves
+    timed apart in each; the faster placement is the baseline. A failure
+    there is reported and leaves the baseline out."""
+    import subprocess
+    sys.path.insert(0, str(ROOT))
+    from oracle.cpu_baseline import baseline_threads
+    threads, why = baseline_threads()
+    runs = {}
+    for label, extra in (("bound", {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}),
+                         ("unbound", {"OMP_PROC_BIND": "false", "OMP_PLACES": ""})):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), **extra)
+        if not extra["OMP_PLACES"]:
+            env.pop("OMP_PLACES")
+        print(f"[bench] cpu baseline ({label}): OpenMP solve of {args.workload} on {threads} threads ({why}) starts",
+              file=sys.stderr, flush=True)
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--workload", args.workload,
+                            "--threads", str(threads), "--repeat", "3"], cwd=str(ROOT), env=env,
+                           capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            print(f"[bench] cpu baseline ({label}) failed ({r.returncode}): {r.stderr[-400:]}", file=sys.stderr,
+                  flush=True)
+            continue
+        c = json.loads(r.stdout.strip().splitlines()[-1])
+        print(f"[bench] cpu baseline ({label}): {c['solve_s']:.3f}s/solve on {c['threads']} threads, "
+              f"{c['iterations']} iterations (setup {c['setup_factorize_s']:.1f}s + {c['setup_prep_s']:.1f}s)",
+              file=sys.stderr, flush=True)
+        runs[label] = c
+    if not runs:
+        return None
+    best = min(runs, key=lambda k: runs[k]["solve_s"])
+    out = dict(runs[best])
+    out["placement"] = best
+    out["solve_s_by_placement"] = {k: v["solve_s"] for k, v in runs.items()}
+    return out
+
+
 def run_batched(args, ax, mg, np, torch, dev, name, A, F, plain_rate, seed0) -> dict:
     """One blocked PCG of k right-hand sides (fp32 sweeps, graph-replayed
     iterations) after the autotuner has picked the SpMV form and block width;
@@ -216,6 +263,10 @@ def main() -> int:
         print(f"bench.py measures one GPU: the solve path is single-GPU (--gpus {args.gpus} is not supported)",
               file=sys.stderr)
         return 2
+
+    # The CPU baseline runs first, as a child process, before this process
+    # touches the GPU and while nothing else competes for the host cores.
+    cpu_run = None if args.no_cpu_baseline else run_cpu_baseline(args)
 
     import numpy as np
     import torch
@@ -342,19 +393,29 @@ def main() -> int:
     cpu = None
     x_cpu_err = None
     if not args.no_cpu_baseline:
+        # The checker: the one-thread oracle's fp64 solve of the same system.
         import oracle
-        fac = F.export()
-        t0 = time.perf_counter()
-        o = oracle.Oracle(A, fac, F.sddm)
-        t_or_setup = time.perf_counter() - t0
+        o = oracle.Oracle(A, F.export(), F.sddm)
         t0 = time.perf_counter()
         xo, ito, _ = o.solve(b, tol=1e-8)
-        t_cpu = time.perf_counter() - t0
+        t_one = time.perf_counter() - t0
         x_cpu_err = float(np.linalg.norm(mixed["x"] - xo) / np.linalg.norm(xo))
-        cpu = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"one full fp64 PCG solve of {args.workload} (n={n}) by the one-thread CPU oracle "
-                         f"({ito} iterations, {t_cpu:.1f}s; oracle setup {t_or_setup:.1f}s not counted)"}
-        print(f"[bench] cpu oracle: {t_cpu:.1f}s/solve, {ito} iterations", file=sys.stderr, flush=True)
+        print(f"[bench] cpu oracle (one thread, the checker): {t_one:.1f}s/solve, {ito} iterations",
+              file=sys.stderr, flush=True)
+        if cpu_run is not None:
+            c = cpu_run
+            cpu = {"value": 1.0 / c["solve_s"], "unit": "solves/s", "cores": c["threads"], "kind": "port",
+                   "sample": f"the OpenMP CPU restatement (level-set sweeps, fused passes) solving {args.workload} "
+                             f"(n={n}) to 1e-8 in fp64, median of {len(c['solve_times_s'])} solves "
+                             f"({c['iterations']} iterations, true residual {c['true_residual']:.2e}) on "
+                             f"{c['threads']} threads ({c['threads_from']}); median solve by thread placement "
+                             + ", ".join(f"{k} {v:.3f}s" for k, v in c["solve_s_by_placement"].items())
+                             + f" (bound: OMP_PROC_BIND=close, OMP_PLACES=cores), the faster one taken; "
+                             f"setup timed apart: host factorization "
+                             f"{c['setup_factorize_s']:.2f}s (one thread), SpTRSV preparation and operator "
+                             f"{c['setup_prep_s']:.2f}s; the same solve on one thread took {t_one:.2f}s",
+                   "solve_s": c["solve_s"], "setup_s": c["setup_factorize_s"] + c["setup_prep_s"],
+                   "one_thread_solve_s": t_one}
 
     value = args.steps / mixed["total"]
     out = {

@@ -1,8 +1,9 @@
 """ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py). A ctypes wrapper over oracle/liboracle.so, the
 one-thread CPU restatement of the reference's solve path (see oracle.cpp for
-the reference lines each function follows). The product library never loads
-it."""
+the reference lines each function follows), and the OpenMP form of the same
+path (the *_omp methods: the timed CPU baseline, not the checker). The
+product library never loads it."""
 from __future__ import annotations
 
 import ctypes
@@ -41,6 +42,10 @@ def lib():
         L.or_spmv.restype = ctypes.c_double
         L.or_spmv.argtypes = [P, P, P]
         L.or_solve.argtypes = [P, P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
+        L.or_forward_omp.argtypes = [P, P, P, ctypes.c_int]
+        L.or_back_omp.argtypes = [P, P, P, ctypes.c_int]
+        L.or_apply_omp.argtypes = [P, P, P, ctypes.c_int]
+        L.or_solve_omp.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
         _lib = L
     return _lib
 
@@ -127,4 +132,33 @@ class Oracle:
         x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
         lib().or_solve(self.h, b.ctypes.data, x.ctypes.data, None if x0a is None else x0a.ctypes.data, float(tol),
                        int(max_iter), int(stagnation_window), out.ctypes.data)
+        return x, int(out[0]), float(out[1])
+
+    # ---- the OpenMP CPU path (timed baseline; the one-thread methods above are the checker) ----
+
+    def forward_omp(self, x, threads: int):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.m)
+        lib().or_forward_omp(self.h, x.ctypes.data, y.ctypes.data, int(threads))
+        return y
+
+    def back_omp(self, x, threads: int):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.m)
+        lib().or_back_omp(self.h, x.ctypes.data, y.ctypes.data, int(threads))
+        return y
+
+    def apply_omp(self, r, threads: int):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros(self.n)
+        lib().or_apply_omp(self.h, r.ctypes.data, z.ctypes.data, int(threads))
+        return z
+
+    def solve_omp(self, b, threads: int, tol=1e-8, max_iter=200, stagnation_window=50):
+        """cpu_solver::solve on `threads` OpenMP threads (x0 = 0): x, iterations, residual."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n)
+        out = np.zeros(2)
+        lib().or_solve_omp(self.h, b.ctypes.data, x.ctypes.data, float(tol), int(max_iter), int(stagnation_window),
+                           int(threads), out.ctypes.data)
         return x, int(out[0]), float(out[1])

@@ -116,37 +116,41 @@ double dot_fat16(Get get, const u32* idx, u64 p, u64 end, const double* y) {
     return sum;
 }
 
-void sweep(const oracle& o, bool fwd, const double* x_in, double* y_out) {
-    const auto& levels = fwd ? o.fwd_levels : o.bck_levels;
+// One row (forward: CSR row v) or column (back: CSC column v) of a sweep,
+// omp.h:1343-1385 (solve_row): the thin-level 4-way sum, the fat-level plain
+// sum (fp32) or SIMD-order sum (fp16), then y_out[v] = (rhs - sum) / diag.
+void solve_row(const oracle& o, bool fwd, bool thin, u32 v, const double* x_in, double* y_out) {
     const auto& ptr = fwd ? o.csr_ptr : o.csc_ptr;
     const auto& idx = fwd ? o.csr_idx : o.csc_idx;
     const auto& v32 = fwd ? o.csr_v : o.csc_v;
     const auto& v16 = fwd ? o.csr_h : o.csc_h;
     auto get32 = [&](u64 p) { return static_cast<double>(v32[p]); };
     auto get16 = [&](u64 p) { return static_cast<double>(half_widen(v16[p])); };
-    for (const auto& level : levels) {
+    const u64 p0 = fwd ? ptr[v] : ptr[v] + 1;
+    const u64 p1 = fwd ? ptr[v + 1] - 1 : ptr[v + 1];
+    double sum;
+    if (o.fp16) {
+        sum = thin ? dot_thin(get16, idx.data(), p0, p1, y_out) : dot_fat16(get16, idx.data(), p0, p1, y_out);
+    } else if (thin) {
+        sum = dot_thin(get32, idx.data(), p0, p1, y_out);
+    } else {
+        sum = 0.0;
+        for (u64 p = p0; p < p1; ++p) sum += get32(p) * y_out[idx[p]];
+    }
+    double rhs = x_in[v];
+    if (!fwd && o.fp16) {
+        const double r = static_cast<double>(o.inv_scale16[v]);
+        rhs = x_in[v] * (r * r);
+    }
+    const double dg = o.fp16 ? static_cast<double>(o.diag16[v])
+                             : static_cast<double>(fwd ? v32[ptr[v + 1] - 1] : v32[ptr[v]]);
+    y_out[v] = (rhs - sum) / dg;
+}
+
+void sweep(const oracle& o, bool fwd, const double* x_in, double* y_out) {
+    for (const auto& level : fwd ? o.fwd_levels : o.bck_levels) {
         const bool thin = level.size() <= kThinLevel;
-        for (const u32 v : level) {
-            const u64 p0 = fwd ? ptr[v] : ptr[v] + 1;
-            const u64 p1 = fwd ? ptr[v + 1] - 1 : ptr[v + 1];
-            double sum;
-            if (o.fp16) {
-                sum = thin ? dot_thin(get16, idx.data(), p0, p1, y_out) : dot_fat16(get16, idx.data(), p0, p1, y_out);
-            } else if (thin) {
-                sum = dot_thin(get32, idx.data(), p0, p1, y_out);
-            } else {
-                sum = 0.0;
-                for (u64 p = p0; p < p1; ++p) sum += get32(p) * y_out[idx[p]];
-            }
-            double rhs = x_in[v];
-            if (!fwd && o.fp16) {
-                const double r = static_cast<double>(o.inv_scale16[v]);
-                rhs = x_in[v] * (r * r);
-            }
-            const double dg = o.fp16 ? static_cast<double>(o.diag16[v])
-                                     : static_cast<double>(fwd ? v32[ptr[v + 1] - 1] : v32[ptr[v]]);
-            y_out[v] = (rhs - sum) / dg;
-        }
+        for (const u32 v : level) solve_row(o, fwd, thin, v, x_in, y_out);
     }
 }
 
@@ -627,6 +631,285 @@ void or_solve(void* h, const double* b, double* x, const double* x0, double tol,
         rz = rz_new;
     }
     if (laplacian) center_x();
+}
+
+}  // extern "C"
+
+// ---- the OpenMP CPU path (timed CPU baseline, not the checker) ---------------
+//
+// The reference's multi-threaded CPU solve: the level-set sweeps of
+// omp.h:1094-1119 (one parallel region per sweep; a thin level -- at most 1024
+// rows -- runs on one thread, a fat level is split statically over the team,
+// the implicit barrier carrying the level dependency) with solve_row's per-row
+// arithmetic, so a sweep gives the one-thread bits; and the fused passes of
+// src/solve.cpp:122-269 and preconditioner.h:403-529 over static_chunk
+// partitions with per-thread partials summed in thread order
+// (preconditioner.h:44-127), so a solve is bit-identical run to run at a fixed
+// thread count. Each pass runs its team only above 2000 elements
+// (fused_omp_min, preconditioner.h:61).
+#include <omp.h>
+
+namespace {
+
+constexpr u64 kFusedOmpMin = 2000;
+constexpr int kPartStride = 8;
+
+struct team {
+    int threads = 1;
+    std::vector<double> part;
+    explicit team(int t) : threads(t < 1 ? 1 : t), part(static_cast<size_t>(threads) * kPartStride, 0.0) {}
+    void ids(int& tid, int& nt) const { tid = omp_get_thread_num(); nt = omp_get_num_threads(); }
+    double reduce(int nt, int slot = 0) const {
+        double s = 0.0;
+        for (int t = 0; t < nt; ++t) s += part[static_cast<size_t>(t) * kPartStride + slot];
+        return s;
+    }
+};
+
+void static_chunk(u64 n, int tid, int nt, u64& lo, u64& hi) {
+    const u64 base = n / static_cast<u64>(nt), rem = n % static_cast<u64>(nt);
+    lo = static_cast<u64>(tid) * base + std::min<u64>(static_cast<u64>(tid), rem);
+    hi = lo + base + (static_cast<u64>(tid) < rem ? 1 : 0);
+}
+
+void sweep_omp(const oracle& o, team& tm, bool fwd, const double* x_in, double* y_out) {
+    const auto& levels = fwd ? o.fwd_levels : o.bck_levels;
+    const size_t L = levels.size();
+#pragma omp parallel num_threads(tm.threads)
+    for (size_t l = 0; l < L; ++l) {
+        const auto& level = levels[l];
+        const long long sz = static_cast<long long>(level.size());
+        if (level.size() <= kThinLevel) {
+#pragma omp single
+            for (long long k = 0; k < sz; ++k) solve_row(o, fwd, true, level[k], x_in, y_out);
+        } else {
+#pragma omp for schedule(static)
+            for (long long k = 0; k < sz; ++k) solve_row(o, fwd, false, level[k], x_in, y_out);
+        }
+    }
+}
+
+double det_sum_omp(team& tm, const double* v, u64 n) {
+    int used = 1;
+#pragma omp parallel num_threads(tm.threads) if (n > kFusedOmpMin)
+    {
+        int tid, nt;
+        tm.ids(tid, nt);
+        if (tid == 0) used = nt;
+        u64 lo, hi;
+        static_chunk(n, tid, nt, lo, hi);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        u64 i = lo;
+        for (; i + 4 <= hi; i += 4) { s0 += v[i]; s1 += v[i + 1]; s2 += v[i + 2]; s3 += v[i + 3]; }
+        double s = (s0 + s1) + (s2 + s3);
+        for (; i < hi; ++i) s += v[i];
+        tm.part[static_cast<size_t>(tid) * kPartStride] = s;
+    }
+    return tm.reduce(used);
+}
+
+void apply_core_omp(oracle& o, team& tm, const double* b, double b_sum, bool center, double* z, double* rz_out) {
+    const u64 n = o.n;
+    const u32* P = o.perm.data();
+    const bool par = n > kFusedOmpMin;
+    auto gather_out = [&](const double* src, double shift) {
+        int used = 1;
+#pragma omp parallel num_threads(tm.threads) if (par)
+        {
+            int tid, nt;
+            tm.ids(tid, nt);
+            if (tid == 0) used = nt;
+            u64 lo, hi;
+            static_chunk(n, tid, nt, lo, hi);
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            u64 v = lo;
+            for (; v + 4 <= hi; v += 4) {
+                const double z0 = src[P[v + 0]] - shift;
+                const double z1 = src[P[v + 1]] - shift;
+                const double z2 = src[P[v + 2]] - shift;
+                const double z3 = src[P[v + 3]] - shift;
+                z[v + 0] = z0; z[v + 1] = z1; z[v + 2] = z2; z[v + 3] = z3;
+                s0 += b[v + 0] * z0; s1 += b[v + 1] * z1; s2 += b[v + 2] * z2; s3 += b[v + 3] * z3;
+            }
+            double s = (s0 + s1) + (s2 + s3);
+            for (; v < hi; ++v) {
+                const double zv = src[P[v]] - shift;
+                z[v] = zv;
+                s += b[v] * zv;
+            }
+            tm.part[static_cast<size_t>(tid) * kPartStride] = s;
+        }
+        if (rz_out) *rz_out = tm.reduce(used);
+    };
+    // Scatter, with the input centring on a centring Laplacian application
+    // (b - 0.0 is b exactly, so one loop serves the plain scatter too).
+    const double b_mean = (!o.sddm && center) ? b_sum / static_cast<double>(n) : 0.0;
+#pragma omp parallel for schedule(static) num_threads(tm.threads) if (par)
+    for (long long v = 0; v < static_cast<long long>(n); ++v) z[P[v]] = b[v] - b_mean;
+    sweep_omp(o, tm, true, z, o.s1.data());
+    sweep_omp(o, tm, false, o.s1.data(), o.s2.data());
+    if (o.sddm) {
+        gather_out(o.s2.data(), 0.0);
+    } else {
+        o.s2[o.m] = 0.0;
+        const double z_mean = center ? det_sum_omp(tm, o.s2.data(), n) / static_cast<double>(n) : 0.0;
+        gather_out(o.s2.data(), z_mean);
+    }
+}
+
+double spmv_omp(const oracle& o, team& tm, const double* x, double* y) {
+    const u64 n = o.n;
+    int used = 1;
+#pragma omp parallel num_threads(tm.threads) if (n > kFusedOmpMin)
+    {
+        int tid, nt;
+        tm.ids(tid, nt);
+        if (tid == 0) used = nt;
+        u64 lo, hi;
+        static_chunk(n, tid, nt, lo, hi);
+        auto val = [&](u64 q) { return o.op32 ? static_cast<double>(o.op_f[q]) : o.op_v[q]; };
+        double xy = 0.0;
+        for (u64 i = lo; i < hi; ++i) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            u64 k = o.op_ptr[i];
+            const u64 e = o.op_ptr[i + 1];
+            for (; k + 4 <= e; k += 4) {
+                s0 += val(k + 0) * x[o.op_idx[k + 0]];
+                s1 += val(k + 1) * x[o.op_idx[k + 1]];
+                s2 += val(k + 2) * x[o.op_idx[k + 2]];
+                s3 += val(k + 3) * x[o.op_idx[k + 3]];
+            }
+            double sum = (s0 + s1) + (s2 + s3);
+            for (; k < e; ++k) sum += val(k) * x[o.op_idx[k]];
+            y[i] = sum;
+            xy += x[i] * sum;
+        }
+        tm.part[static_cast<size_t>(tid) * kPartStride] = xy;
+    }
+    return tm.reduce(used);
+}
+
+// r = b - A x0 (ax0 null: r = b) or x += alpha p; r -= alpha Ap (update):
+// r.r and sum r of the new r, per-thread 4-way accumulators.
+void fused_r_omp(team& tm, u64 n, double* r, const double* b, const double* ax0, double* x, const double* p,
+                 const double* ap, double alpha, bool update, double& rr, double& rs) {
+    int used = 1;
+#pragma omp parallel num_threads(tm.threads) if (n > kFusedOmpMin)
+    {
+        int tid, nt;
+        tm.ids(tid, nt);
+        if (tid == 0) used = nt;
+        u64 lo, hi;
+        static_chunk(n, tid, nt, lo, hi);
+        double q[4] = {0.0, 0.0, 0.0, 0.0}, sm[4] = {0.0, 0.0, 0.0, 0.0};
+        auto step = [&](u64 i) {
+            double ri;
+            if (update) {
+                x[i] += alpha * p[i];
+                ri = r[i] - alpha * ap[i];
+            } else {
+                ri = ax0 ? b[i] - ax0[i] : b[i];
+            }
+            r[i] = ri;
+            return ri;
+        };
+        u64 i = lo;
+        for (; i + 4 <= hi; i += 4)
+            for (int k = 0; k < 4; ++k) {
+                const double ri = step(i + k);
+                q[k] += ri * ri;
+                sm[k] += ri;
+            }
+        double qq = (q[0] + q[1]) + (q[2] + q[3]), ss = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+        for (; i < hi; ++i) {
+            const double ri = step(i);
+            qq += ri * ri;
+            ss += ri;
+        }
+        tm.part[static_cast<size_t>(tid) * kPartStride + 0] = qq;
+        tm.part[static_cast<size_t>(tid) * kPartStride + 1] = ss;
+    }
+    rr = tm.reduce(used, 0);
+    rs = tm.reduce(used, 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+// The OpenMP CPU path's forward / back sweep and apply at `threads` threads.
+void or_forward_omp(void* h, const double* x, double* y, int threads) {
+    team tm(threads);
+    sweep_omp(*static_cast<oracle*>(h), tm, true, x, y);
+}
+void or_back_omp(void* h, const double* x, double* y, int threads) {
+    team tm(threads);
+    sweep_omp(*static_cast<oracle*>(h), tm, false, x, y);
+}
+void or_apply_omp(void* h, const double* r, double* z, int threads) {
+    auto* o = static_cast<oracle*>(h);
+    team tm(threads);
+    o->s1.assign(o->n, 0.0);
+    o->s2.assign(o->n, 0.0);
+    const bool center = !o->sddm && take_center(*o);
+    const double b_sum = center ? det_sum_omp(tm, r, o->n) : 0.0;
+    apply_core_omp(*o, tm, r, b_sum, center, z, nullptr);
+}
+
+// cpu_solver::solve_impl (solve.cpp:478-654) on `threads` threads, x0 = 0.
+// out: iterations, residual.
+void or_solve_omp(void* h, const double* b, double* x, double tol, int max_iter, int stag_window, int threads,
+                  double* out) {
+    auto* o = static_cast<oracle*>(h);
+    const u64 n = o->n;
+    team tm(threads);
+    std::vector<double> r(n), z(n), p(n), Ap(n);
+    o->s1.assign(n, 0.0);
+    o->s2.assign(n, 0.0);
+    out[0] = 0;
+    out[1] = 0.0;
+    double rr, rs;
+    fused_r_omp(tm, n, r.data(), b, nullptr, nullptr, nullptr, nullptr, 0.0, false, rr, rs);
+    const double bnorm = std::sqrt(rr);
+#pragma omp parallel for schedule(static) num_threads(tm.threads) if (n > kFusedOmpMin)
+    for (long long i = 0; i < static_cast<long long>(n); ++i) x[i] = 0.0;
+    if (bnorm == 0.0) return;
+    out[1] = 1.0;
+    o->apply_count = 0;
+    auto apply_fused = [&](double r_sum) {
+        const bool center = !o->sddm && take_center(*o);
+        double rz = 0.0;
+        apply_core_omp(*o, tm, r.data(), r_sum, center, z.data(), &rz);
+        return rz;
+    };
+    double rz = apply_fused(rs);
+#pragma omp parallel for schedule(static) num_threads(tm.threads) if (n > kFusedOmpMin)
+    for (long long i = 0; i < static_cast<long long>(n); ++i) p[i] = z[i];
+    double prev = 1.0;
+    for (int i = 0; i < max_iter; ++i) {
+        const double pAp = spmv_omp(*o, tm, p.data(), Ap.data());
+        if (pAp <= 0.0) break;
+        const double alpha = rz / pAp;
+        fused_r_omp(tm, n, r.data(), nullptr, nullptr, x, p.data(), Ap.data(), alpha, true, rr, rs);
+        const double rn = std::sqrt(rr) / bnorm;
+        out[0] = i + 1;
+        out[1] = rn;
+        if (rn < tol) break;
+        if (stag_window > 0 && (i + 1) % stag_window == 0) {
+            if (rn > prev * 0.5) break;
+            prev = rn;
+        }
+        const double rz_new = apply_fused(rs);
+        const double beta = rz_new / rz;
+#pragma omp parallel for schedule(static) num_threads(tm.threads) if (n > kFusedOmpMin)
+        for (long long j = 0; j < static_cast<long long>(n); ++j) p[j] = z[j] + beta * p[j];
+        rz = rz_new;
+    }
+    if (!o->sddm) {
+        const double mean = det_sum_omp(tm, x, n) / static_cast<double>(n);
+#pragma omp parallel for schedule(static) num_threads(tm.threads) if (n > kFusedOmpMin)
+        for (long long i = 0; i < static_cast<long long>(n); ++i) x[i] -= mean;
+    }
 }
 
 }  // extern "C"

@@ -434,3 +434,41 @@ def test_options_default_fills_exactly_the_python_struct():
     assert (o.precision, o.fp16_storage, o.schedule, o.drop_rel, o.keep_factor) == (0, 0, 2, -1.0, 0)
     assert (o.iters_per_check, o.graph, o.relabel, o.depth_levels) == (2, 1, 1, 1)
     assert (o.sweep_chunk, o.sweep_blocks, o.tail_rows, o.lane_batches, o.tail_batches, o.tail_cut) == (0,) * 6
+
+
+# ---- the OpenMP CPU path (bench.py's timed baseline) ---------------------------
+
+@pytest.mark.parametrize("name", ["sddm3d_6", "grid2d_12", "rmat_9"])
+def test_openmp_cpu_path_matches_the_one_thread_oracle(name):
+    """The OpenMP form of the CPU restatement: its sweeps give the one-thread
+    bits at any thread count (thin levels on one thread, fat levels split, the
+    per-row arithmetic of omp.h:1343-1385 either way); at one thread its whole
+    solve is the oracle's bit for bit; at four threads the reductions are
+    summed per thread (preconditioner.h:44-127), so the solve repeats its own
+    bits and stays within 1e-10 and one iteration of the one-thread solve."""
+    A, g = golden(name)
+    F = ax.factorize(A, seed=0)
+    o = oracle.Oracle(A, F.export(), F.sddm)
+    r = mg.request_rhs(A, F.sddm, 7)   # N(0,1), projected per component onto the range of a Laplacian
+    y = o.forward(r[:o.m])
+    assert np.array_equal(o.forward_omp(r[:o.m], 4), y)
+    assert np.array_equal(o.back_omp(y, 4), o.back(y))
+    x1, i1, _ = o.solve(r, tol=1e-10)
+    xa, ia, _ = o.solve_omp(r, 1, tol=1e-10)
+    assert ia == i1 and np.array_equal(xa, x1)
+    x4, i4, res4 = o.solve_omp(r, 4, tol=1e-10)
+    x4b, _, _ = o.solve_omp(r, 4, tol=1e-10)
+    assert np.array_equal(x4, x4b)
+    assert abs(i4 - i1) <= 1 and res4 < 1e-10
+    assert np.linalg.norm(x4 - x1) <= 1e-8 * np.linalg.norm(x1)
+
+
+def test_cpu_baseline_threads_follow_the_cores_and_the_omp_cap(monkeypatch):
+    from oracle.cpu_baseline import baseline_threads, physical_cores
+    phys, logical = physical_cores()
+    assert 1 <= phys <= logical
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    assert baseline_threads()[0] == 1
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    t, why = baseline_threads()
+    assert t == phys and "physical cores" in why
