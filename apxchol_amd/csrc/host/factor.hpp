@@ -38,6 +38,12 @@ struct factor_options {
     std::size_t omp_threshold = 2000;
     // SIZE_MAX = defer to the partitioner (block_greedy declares 500).
     std::size_t parallel_residual_threshold = SIZE_MAX;
+    // Host threads of the factorization (0: APXCHOL_AMD_FACTOR_THREADS, else
+    // the CPUs this process may use, capped by OMP_NUM_THREADS and 16) and the
+    // smallest round run in parallel (0: APXCHOL_AMD_FACTOR_PAR_MIN, else 2048). The factor does not depend on
+    // either.
+    int threads = 0;
+    std::size_t parallel_min = 0;
 };
 
 // Per-round statistics (solver/factorization.h:45-56).
@@ -73,8 +79,8 @@ struct operator_check {
 operator_check check_operator(const csc_view& A);
 
 // Factorize (src/factorization.cpp:125-175 with vec_pool storage and the
-// block_greedy partitioner), single-threaded: the factor is a pure function
-// of (A, seed).
+// block_greedy partitioner): the factor is a pure function of (A, seed), the
+// reference's one-thread factor, whatever opts.threads is.
 factorization factorize(const csc_view& A, const factor_options& opts = {});
 
 }  // namespace apxchol_amd

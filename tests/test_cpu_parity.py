@@ -51,8 +51,14 @@ def test_golden_fixtures_present():
 
 # ---- the factorization ------------------------------------------------------
 
+@pytest.mark.parametrize("threads", ["1", "8"])
 @pytest.mark.parametrize("name", CASES)
-def test_product_factor_bit_identical_to_reference(name):
+def test_product_factor_bit_identical_to_reference(name, threads, monkeypatch):
+    """The reference's one-thread factor (fixtures written from oracle/_ref),
+    bit for bit -- also when the rounds run on 8 host threads (every round
+    and selection pass forced onto the team)."""
+    monkeypatch.setenv("APXCHOL_AMD_FACTOR_THREADS", threads)
+    monkeypatch.setenv("APXCHOL_AMD_FACTOR_PAR_MIN", "1")
     A, g = golden(name)
     F = ax.factorize(A, seed=0)
     outer, inner, vals, perm, ris = F.export()
@@ -62,6 +68,30 @@ def test_product_factor_bit_identical_to_reference(name):
     assert np.array_equal(bits(vals), bits(g["vals"]))
     assert np.array_equal(perm, g["perm"])
     assert np.array_equal(ris, g["round_is"])
+
+
+@pytest.mark.parametrize("gen", ["sddm3d_40", "rmat_14", "aniso3d_30"])
+def test_threaded_factor_equals_one_thread_factor(gen, monkeypatch):
+    """Larger graphs than the fixtures (main-loop rounds of thousands of
+    vertices, Baumann-Kyng rounds over hubs): the factor on 1, 3 and 8 host
+    threads, with the default and with a forced-parallel round threshold, is
+    the same bits."""
+    if gen == "sddm3d_40":
+        A = mg.add_excess(mg.grid_laplacian((40, 40, 40), "lognormal"), 1e-3)
+    elif gen == "rmat_14":
+        A = mg.rmat_laplacian(14)
+    else:
+        A = mg.grid_laplacian((30, 30, 30), "lognormal", sigma=0.5, axis_weights=(1.0, 10.0, 100.0))
+    ref = None
+    for threads, par_min in (("1", "0"), ("3", "1"), ("8", "0"), ("8", "1")):
+        monkeypatch.setenv("APXCHOL_AMD_FACTOR_THREADS", threads)
+        monkeypatch.setenv("APXCHOL_AMD_FACTOR_PAR_MIN", par_min)
+        e = ax.factorize(A, seed=3).export()
+        if ref is None:
+            ref = e
+            continue
+        for a, b in zip(ref, e):
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), (threads, par_min)
 
 
 @pytest.mark.parametrize("name", CASES)
