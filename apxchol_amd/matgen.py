@@ -68,22 +68,42 @@ def add_excess(L: sp.csc_matrix, frac: float = 1e-3, seed: int = 2) -> sp.csc_ma
     return (L + sp.diags(rng.uniform(0.0, frac, L.shape[0]) * deg)).tocsc()
 
 
+def _rmat_endpoints(scale: int, m: int, a: float, b: float, c: float, seed: int, chunk: int = 1 << 20):
+    """The m R-MAT samples: bit `bit` of sample i comes from draw bit * m + i of
+    one PCG64 stream seeded with `seed` (u gets 1 when the draw is >= a + b, v
+    when it lies in [a, a + b) or >= a + b + c). Chunks of samples are drawn
+    from the stream advanced to their position, on a few threads; the result
+    is that of drawing the whole stream in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    ab, abc = a + b, a + b + c
+    u = np.empty(m, dtype=np.int64)
+    v = np.empty(m, dtype=np.int64)
+
+    def work(s0: int) -> None:
+        s1 = min(m, s0 + chunk)
+        uu = np.zeros(s1 - s0, dtype=np.int64)
+        vv = np.zeros(s1 - s0, dtype=np.int64)
+        for bit in range(scale):
+            bg = np.random.PCG64(seed)
+            bg.advance(bit * m + s0)
+            r = np.random.Generator(bg).random(s1 - s0)
+            uu |= (r >= ab).astype(np.int64) << bit
+            vv |= (((r >= a) & (r < ab)) | (r >= abc)).astype(np.int64) << bit
+        u[s0:s1] = uu
+        v[s0:s1] = vv
+
+    with ThreadPoolExecutor(max(1, min(8, (m + chunk - 1) // chunk))) as ex:
+        list(ex.map(work, range(0, m, chunk)))
+    return u, v
+
+
 def rmat_laplacian(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
                    seed: int = 3, wseed: int = 4, wlo: float = 1.0, whi: float = 10.0) -> sp.csc_matrix:
     """R-MAT graph Laplacian: 2^scale vertices, edge_factor * 2^scale samples,
     symmetrized, duplicates merged, self-loops dropped, weights U[wlo, whi)."""
     n = 1 << scale
     m = edge_factor * n
-    rng = np.random.default_rng(seed)
-    u = np.zeros(m, dtype=np.int64)
-    v = np.zeros(m, dtype=np.int64)
-    ab, abc = a + b, a + b + c
-    for bit in range(scale):
-        r = rng.random(m)
-        ubit = (r >= ab).astype(np.int64)
-        vbit = (((r >= a) & (r < ab)) | (r >= abc)).astype(np.int64)
-        u |= ubit << bit
-        v |= vbit << bit
+    u, v = _rmat_endpoints(scale, m, a, b, c, seed)
     keep = u != v
     u, v = u[keep], v[keep]
     lo, hi = np.minimum(u, v), np.maximum(u, v)

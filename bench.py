@@ -13,7 +13,9 @@ compared with the CPU oracle's fp64 solve of the same system.
 Beside the headline, blocked k-right-hand-side solves (k = 32, fp32 sweeps,
 autotuned SpMV form and block width) are timed on the same system, against
 its plain fp64 single-vector rate, and on the 64^3 anisotropic grid
-(weights 1/10/100 x exp(N(0,0.5))); the serving mix runs 8 resident factors
+(weights 1/10/100 x exp(N(0,0.5))); the R-MAT scale-22 Laplacian (edge
+factor 16, weights U[1,10), per-component mean-projected N(0,1) right-hand
+side) is timed in mixed and fp64 single solves and one blocked k = 32 solve; the serving mix runs 8 resident factors
 (2-D grids 1e4..1e6, 3-D grids to 96^3, R-MAT scales 16..20) under Poisson
 arrivals and reports p50 / p99 latency and solves/s.
 
@@ -240,6 +242,45 @@ def run_batched(args, ax, mg, np, torch, dev, name, A, F, plain_rate, seed0) -> 
                          "block_width": tune["block_width"]}}
 
 
+def run_second_workload(args, ax, mg, np, torch, dev) -> dict:
+    """The R-MAT configuration (default scale 22, mean-projected per component
+    N(0,1) right-hand side, tol 1e-8): single solves in mixed precision and
+    in fp64 (graph replay, one warm-up then --second-steps timed solves), and
+    one blocked k-right-hand-side solve against the fp64 single-solve rate."""
+    name = args.second_workload
+    t0 = time.perf_counter()
+    print(f"[bench] {name}: generation and host factorization start", file=sys.stderr, flush=True)
+    A, b = mg.workload(name)
+    F = ax.factorize(A, seed=0)
+    t_fact = time.perf_counter() - t0
+    print(f"[bench] {name}: n={A.shape[0]} nnz={A.nnz} factor ready after {t_fact:.1f}s", file=sys.stderr, flush=True)
+    bd = torch.tensor(b, device=dev)
+    xd = torch.empty_like(bd)
+    out = {"n": int(A.shape[0]), "nnz": int(A.nnz), "factor_nnz": F.nnz, "generation_and_factorization_s": t_fact}
+    for prec, label in (("fp32", "mixed"), ("fp64", "fp64")):
+        s = ax.Solver(A, factor=F, precision=prec)
+        s.solve(bd, out=xd)
+        torch.cuda.synchronize()
+        lat, iters = [], []
+        for _ in range(args.second_steps):
+            t1 = time.perf_counter()
+            r = s.solve(bd, out=xd)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t1)
+            iters.append(r.iterations)
+        x = xd.cpu().numpy()
+        tr = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+        out[label] = {"solves_per_s": len(lat) / sum(lat), "p50_ms": 1e3 * percentile(lat, 50),
+                      "p99_ms": 1e3 * percentile(lat, 99), "iterations": iters[0], "true_residual": tr}
+        s.close()
+        print(f"[bench] {name} {label}: {out[label]['solves_per_s']:.2f} solves/s iters={iters[0]} true_res={tr:.2e}",
+              file=sys.stderr, flush=True)
+    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    if args.batch_k > 0:
+        out["batched_mixed"] = run_batched(args, ax, mg, np, torch, dev, name, A, F, out["fp64"]["solves_per_s"], 100)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +294,9 @@ def main() -> int:
     ap.add_argument("--batch-k", type=int, default=32, help="right-hand sides of the blocked measurements (0: skip)")
     ap.add_argument("--batch-workload", default="aniso3d_64",
                     help="second blocked workload (default: the 64^3 anisotropic grid; empty: none)")
+    ap.add_argument("--second-workload", default="rmat_22",
+                    help="R-MAT configuration timed beside the headline (single mixed / fp64, blocked k; empty: none)")
+    ap.add_argument("--second-steps", type=int, default=3)
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serving-qps", type=float, default=25.0)
     ap.add_argument("--serving-slots", type=int, default=32)
@@ -379,6 +423,10 @@ def main() -> int:
             batched[args.batch_workload] = run_batched(args, ax, mg, np, torch, dev, args.batch_workload, None, None,
                                                        None, 0)
 
+    second = None
+    if args.second_workload:
+        second = run_second_workload(args, ax, mg, np, torch, dev)
+
     # The serving mix is reported beside the headline: a failure there is
     # recorded in its section (with the server's reason) and does not void
     # the measured single-solve line.
@@ -450,6 +498,7 @@ def main() -> int:
                 "true_residual": plain["true_res"], "setup_s": plain["setup"]},
             "mixed_over_fp64": None if not plain else value / (args.steps / plain["total"]),
             "batched_mixed": batched,
+            "second_workload": None if second is None else {args.second_workload: second},
             "serving": serving,
             "kernels_mixed": per_kernel,
             "sptrsv_sweep_gbps_mixed": sweep_gbps,
