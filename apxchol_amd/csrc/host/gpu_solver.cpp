@@ -830,7 +830,7 @@ void gpu_solver::setup(const csc_view& L) {
     d.st.alloc(sizeof(pcg_state));
     d.st_apply.alloc(sizeof(pcg_state));
     d.st_apply.zero();
-    d.ctrl.alloc(4 * sizeof(int));
+    d.ctrl.alloc(8 * sizeof(int));
     d.ctrl.zero();
     d.err.alloc(sizeof(int));
     d.err.zero();
@@ -852,8 +852,9 @@ void gpu_solver::setup(const csc_view& L) {
         s.batch_pos = (backward ? d.b_pos : d.f_pos).as<std::uint32_t>();
         s.batch_mask = (backward ? d.b_mask : d.f_mask).as<std::uint64_t>();
         s.nb = static_cast<int>(bt.mask.size());
-        s.chunk = opts_.sweep_chunk > 0 ? opts_.sweep_chunk : env_int("APXCHOL_AMD_SWEEP_CHUNK", 8);
-        s.ctrl = d.ctrl.as<int>() + (backward ? 2 : 0);
+        s.chunk = opts_.sweep_chunk > 0 ? opts_.sweep_chunk : env_int("APXCHOL_AMD_SWEEP_CHUNK", 2);
+        s.window = env_int("APXCHOL_AMD_SWEEP_WINDOW", 4096);
+        s.ctrl = d.ctrl.as<int>() + (backward ? 4 : 0);
     };
     fill_desc(d.fwd, false, fb);
     fill_desc(d.bck, true, bb);
@@ -1003,6 +1004,10 @@ void gpu_solver::setup(const csc_view& L) {
         sc.lane_rec_dev = lrec.as<lane_rec>();
         sc.lane_ent_dev = lent.as<entry32>();
         sc.lane_batches = lt.batches;
+        // Big launched levels run several batches per wave (fewer waves to
+        // dispatch; the same bits).
+        sc.lane_bpw2 = env_int("APXCHOL_AMD_LANE_BPW2", 0);
+        sc.lane_bpw4 = env_int("APXCHOL_AMD_LANE_BPW4", 0);
         // Algorithmic bytes per level (see solver_stats).
         if (sc.kind != 0) {
             const auto& ptr = backward ? T_.b_ptr : T_.f_ptr;

@@ -34,7 +34,7 @@ struct solve_options {
                                   // (0: APXCHOL_AMD_TAIL_CUT or 1; < 0: never)
     double drop_rel = -1.0;       // < 0: APXCHOL_FACTOR_DROP / the 1e-4 default
     int iters_per_check = 2;      // iterations enqueued between completion tests
-    int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 8)
+    int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 2)
     int sweep_blocks = 0;         // dataflow grid (0: APXCHOL_AMD_SWEEP_BLOCKS or the resident grid)
     bool graph = true;            // replay each PCG iteration as one captured HIP graph
     bool relabel = true;          // sort every level's rows by original vertex (APXCHOL_AMD_RELABEL=0: off)

@@ -71,8 +71,19 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         if (lb >= 0) {
             if (static_cast<long long>(lb) + (b1 - b0) > sc.lane_batches)
                 throw std::runtime_error("apxchol_amd: level outside the lane layout");
-            launch_timed(hook, 0, sweep_lanes<TI, TO, F16>, dim3(g), dim3(kBlock), 0, s, d, lb, b1 - b0,
-                         sc.lane_rec_dev, sc.lane_ent_dev, in, o, st);
+            // Big levels: several batches per wave (fewer waves to dispatch).
+            const int nlb = b1 - b0;
+            const int bpw = (sc.lane_bpw4 > 0 && nlb >= sc.lane_bpw4) ? 4 : (sc.lane_bpw2 > 0 && nlb >= sc.lane_bpw2) ? 2 : 1;
+            const int gb = (nlb + kWavesPerBlock * bpw - 1) / (kWavesPerBlock * bpw);
+            if (bpw == 4)
+                launch_timed(hook, 0, sweep_lanes<TI, TO, F16, kBlock, 4>, dim3(gb), dim3(kBlock), 0, s, d, lb, nlb,
+                             sc.lane_rec_dev, sc.lane_ent_dev, in, o, st);
+            else if (bpw == 2)
+                launch_timed(hook, 0, sweep_lanes<TI, TO, F16, kBlock, 2>, dim3(gb), dim3(kBlock), 0, s, d, lb, nlb,
+                             sc.lane_rec_dev, sc.lane_ent_dev, in, o, st);
+            else
+                launch_timed(hook, 0, sweep_lanes<TI, TO, F16>, dim3(gb), dim3(kBlock), 0, s, d, lb, nlb,
+                             sc.lane_rec_dev, sc.lane_ent_dev, in, o, st);
             return;
         }
         if (g > kLevelMaxBlocks) g = kLevelMaxBlocks;

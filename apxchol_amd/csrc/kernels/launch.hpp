@@ -44,6 +44,8 @@ struct sweep_sched {
     const lane_rec* lane_rec_dev = nullptr;  // static lane layout of the launched levels (device)
     const entry32* lane_ent_dev = nullptr;
     long long lane_batches = 0;
+    int lane_bpw2 = 0;                     // lane levels of >= this many batches: 2 per wave (0: never)
+    int lane_bpw4 = 0;                     // ... >= this many: 4 per wave (0: never)
 };
 
 // Kernel timing: for kernel `kind` (0 sweep_levels, 1 sweep_tail,

@@ -42,6 +42,16 @@ __device__ __forceinline__ bool sweep_failed(const int* err) {
     return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// Exponential backoff between polls: `nap` units of s_sleep 1 (64 clocks
+// each), doubled after every empty poll up to `cap`, so waves that wait long
+// put less and less load on the memory system.
+__device__ __forceinline__ void backoff(unsigned& nap, unsigned cap) {
+    for (unsigned i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
+    nap = nap < cap ? 2 * nap : cap;
+}
+inline constexpr unsigned kPollNapCap = 8;      // rows waiting on their dependencies
+inline constexpr unsigned kClaimNapCap = 64;    // tickets waiting on the claim window
+
 __device__ __forceinline__ int lanes_for(unsigned len) {
     unsigned g = (len + kEpl - 1) / kEpl;
     if (g <= 1) return 1;
@@ -200,12 +210,12 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
                 pend |= valid ? (1u << k) : 0u;
             }
             if constexpr (WAIT) {
-                unsigned spins = 0;
+                unsigned spins = 0, nap = 1;
                 for (;;) {
                     if (pend) poll<TO>(out, jj, xv, pend);
                     if (__ballot(pend != 0) == 0) break;
                     if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) { ok = false; break; }
-                    __builtin_amdgcn_s_sleep(1);
+                    backoff(nap, kPollNapCap);
                 }
             } else {
                 gather<TO>(out, jj, xv, pend);
@@ -251,7 +261,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         return true;
     }
     bool done = !active;
-    unsigned spins = 0;
+    unsigned spins = 0, nap = 1;
     for (;;) {
         if (pend) poll<TO>(out, jj, xv, pend);
         const unsigned long long ready = __ballot(pend == 0);
@@ -271,7 +281,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
             if (!done && sub == 0) publish<TO, WAIT>(out + row, static_cast<AT>(0));
             break;
         }
-        if (!go) __builtin_amdgcn_s_sleep(1);
+        if (!go) backoff(nap, kPollNapCap);
     }
     return ok;
 }
@@ -286,23 +296,56 @@ __global__ __launch_bounds__(kBlock) void sweep_dataflow(sweep_desc d, const TI*
     // A wave claims d.chunk consecutive batches per ticket and runs them in
     // order, so it only ever waits on batches it already finished or on
     // smaller tickets held by running waves.
+    //
+    // Claim window: a ticket is started only once it lies fewer than
+    // d.window batches past the count of finished batches (ctrl[2], added
+    // per finished ticket). Every batch below the smallest unfinished ticket
+    // is finished, so that ticket is always inside the window and the sweep
+    // cannot stall on it; the waves holding tickets far ahead of the frontier
+    // sleep on the one counter (with a growing nap) instead of polling the
+    // output slots of rows whose dependencies are many levels away. Without
+    // the window every resident wave polls from the moment it claims: at the
+    // full resident grid that is ~57,000 batches ahead of the frontier on a
+    // 128^3 forward sweep (7168 waves x 8 batches per ticket), all polling
+    // with agent-scope loads that compete with the frontier's own loads and
+    // publishing stores. The sweep slowed with the polling population
+    // (128^3 solve: 448 -> 224 workgroups 158 -> 117 ms, 8 -> 2 batches per
+    // ticket 158 -> 81 ms), and at the full grid waiting waves exhausted the
+    // spin limit.
     for (;;) {
         int b0 = 0;
         if (lane == 0) b0 = atomicAdd(&d.ctrl[0], d.chunk);
         b0 = __shfl(b0, 0, kWave);
         if (b0 >= d.nb) break;
+        if (d.window > 0) {
+            unsigned spins = 0, nap = 1;
+            for (;;) {
+                int fin = 0;
+                if (lane == 0) fin = __hip_atomic_load(&d.ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                fin = __shfl(fin, 0, kWave);
+                if (b0 - fin < d.window) break;
+                if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) {
+                    if (lane == 0) atomicOr(err, 1);
+                    ok = false;
+                    break;
+                }
+                backoff(nap, kClaimNapCap);
+            }
+        }
         const int b1 = min(b0 + d.chunk, d.nb);
         for (int b = b0; b < b1; ++b) {
             const bool bok = sweep_batch<TI, TO, F16, true>(d, b, rhs, out, shift, err);
             if (!bok && lane == 0) atomicOr(err, 1);
             ok = ok && bok;
         }
+        if (lane == 0) __hip_atomic_fetch_add(&d.ctrl[2], b1 - b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (lane == 0) {
         const int prev = atomicAdd(&d.ctrl[1], 1);
         if (prev == total_waves - 1) {
             __hip_atomic_store(&d.ctrl[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&d.ctrl[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&d.ctrl[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -317,46 +360,61 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
     for (int b = b0 + w; b < b1; b += waves) sweep_batch<TI, TO, F16, false>(d, b, rhs, out, shift, err);
 }
 
-// One launched level from the static lane layout: wave w runs lane batch
-// lb0 + w. The layout loads are issued before the kernel reads the solve
-// state, so the completion test does not add a dependent load; the per-row
-// arithmetic is sweep_batch's (same slots, fma chain and butterfly), so the
-// bits equal the other schedules'.
-template <class TI, class TO, bool F16, int BS = kBlock>
+// One launched level from the static lane layout: wave w runs lane batches
+// lb0 + w * BPW .. + BPW - 1 (BPW batches per wave: all their layout loads,
+// then all their gathers, are in flight together, and a big level needs BPW
+// times fewer waves). The layout loads are issued before the kernel reads the
+// solve state, so the completion test does not add a dependent load; the
+// per-row arithmetic is sweep_batch's (same slots, fma chain and butterfly),
+// so the bits equal the other schedules'.
+template <class TI, class TO, bool F16, int BS = kBlock, int BPW = 1>
 __global__ __launch_bounds__(BS) void sweep_lanes(sweep_desc d, int lb0, int nlb, const lane_rec* __restrict__ rec,
                                                   const entry32* __restrict__ ent, const TI* __restrict__ rhs,
                                                   TO* out, const pcg_state* st) {
     using AT = TO;
     const int w = static_cast<int>(blockIdx.x) * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
-    if (w >= nlb) return;
+    const int first = w * BPW;
+    if (first >= nlb) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const size_t b = static_cast<size_t>(lb0 + w);
-    const lane_rec rc = rec[b * kWave + lane];
-    entry32 e[kEpl];
+    lane_rec rc[BPW];
+    entry32 e[BPW][kEpl];
 #pragma unroll
-    for (int k = 0; k < kEpl; ++k) e[k] = ent[(b * kEpl + k) * kWave + lane];
+    for (int i = 0; i < BPW; ++i) {
+        // A wave's batches past the level's end reload its first batch and
+        // store nothing.
+        const size_t b = static_cast<size_t>(lb0 + (first + i < nlb ? first + i : first));
+        rc[i] = rec[b * kWave + lane];
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) e[i][k] = ent[(b * kEpl + k) * kWave + lane];
+    }
     asm volatile("" ::: "memory");
     if (st && st->done) return;
     const double shift = (st && !d.backward) ? st->shift : 0.0;
-    const int row = rc.row;
-    const int G = 1 << (rc.info & 7u);
-    const int sub = static_cast<int>((rc.info >> 3) & 63u);
-    const int nv = static_cast<int>((rc.info >> 9) & 7u);
-    AT xv[kEpl];
+    AT xv[BPW][kEpl];
 #pragma unroll
-    for (int k = 0; k < kEpl; ++k) xv[k] = k < nv ? out[e[k].idx] : static_cast<AT>(0);
-    const bool head = row >= 0 && sub == 0;
-    AT r = 0;
-    AT dg = 1;
-    if (head) {
-        r = row_rhs<TI, AT>(d, rhs, row, shift);
-        dg = static_cast<AT>(d.diag[row]);
+    for (int i = 0; i < BPW; ++i) {
+        const int nv = static_cast<int>((rc[i].info >> 9) & 7u);
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) xv[i][k] = k < nv ? out[e[i][k].idx] : static_cast<AT>(0);
     }
-    AT part = 0;
 #pragma unroll
-    for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(e[k].val), xv[k], part);
-    part = group_tree_sum(part, G);
-    if (head) out[row] = (r - part) / dg;
+    for (int i = 0; i < BPW; ++i) {
+        const int row = rc[i].row;
+        const int G = 1 << (rc[i].info & 7u);
+        const int sub = static_cast<int>((rc[i].info >> 3) & 63u);
+        const bool head = row >= 0 && sub == 0 && first + i < nlb;
+        AT r = 0;
+        AT dg = 1;
+        if (head) {
+            r = row_rhs<TI, AT>(d, rhs, row, shift);
+            dg = static_cast<AT>(d.diag[row]);
+        }
+        AT part = 0;
+#pragma unroll
+        for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(e[i][k].val), xv[i][k], part);
+        part = group_tree_sum(part, G);
+        if (head) out[row] = (r - part) / dg;
+    }
 }
 
 // A run of small levels [l0, l1) -- sweep positions [p0, p1) -- walked by ONE

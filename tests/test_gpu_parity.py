@@ -389,3 +389,35 @@ def test_adjacency_graph_through_laplacian_converges(gpu):
     r = s.solve(torch.tensor(b, device=gpu))
     assert r.status == 1 and np.linalg.norm(b - L @ r.x.cpu().numpy()) <= 2e-8 * np.linalg.norm(b)
     s.close()
+
+
+@pytest.mark.parametrize("name", ["sddm3d_16", "rmat_16"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lane_levels_several_batches_per_wave_same_bits(gpu, name, precision, monkeypatch):
+    """Launched lane levels run 1, 2 or 4 batches per wave (thresholds taken
+    from APXCHOL_AMD_LANE_BPW2 / _BPW4 when a solver is built; 1 forces the
+    wider form on every launched level): both sweeps and an application give
+    the same bits as one batch per wave."""
+    import torch
+    A, F, fac = case(name)
+    print(f"[case] lane batches per wave {name} {precision}", flush=True)
+    outs = []
+    for bpw2, bpw4 in (("0", "0"), ("1", "0"), ("0", "1"), ("2", "5")):
+        monkeypatch.setenv("APXCHOL_AMD_LANE_BPW2", bpw2)
+        monkeypatch.setenv("APXCHOL_AMD_LANE_BPW4", bpw4)
+        s = ax.Solver(A, factor=F, precision=precision, tail_rows=1, tail_batches=1)
+        assert sum(s.info["lane_levels"]) > 0
+        m = s.info["m"]
+        x = torch.tensor(np.random.default_rng(3).standard_normal(m), device=gpu)
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        s.sweep(x, y, backward=False)
+        s.sweep(y, z, backward=True)
+        r = torch.tensor(np.random.default_rng(4).standard_normal(A.shape[0]), device=gpu)
+        s.reset_apply_count()
+        a = s.apply(r)
+        torch.cuda.synchronize()
+        outs.append((z.cpu().numpy().view(np.int64).copy(), a.cpu().numpy().view(np.int64).copy()))
+        s.close()
+    for k in range(1, len(outs)):
+        assert np.array_equal(outs[0][0], outs[k][0]) and np.array_equal(outs[0][1], outs[k][1]), k
