@@ -69,6 +69,10 @@ _SIGS = {
     "apx_solver_solve_batch": ([P, I32, P, P, F64, I32, P, P], ctypes.c_int),
     "apx_solver_apply_batch": ([P, I32, P, P, P], ctypes.c_int),
     "apx_server_create": ([ctypes.c_uint64, I32, I32, P], ctypes.c_int),
+    "apx_group_create": ([P, I32, P], ctypes.c_int),
+    "apx_group_destroy": ([P], None),
+    "apx_group_info": ([P, P, P], ctypes.c_int),
+    "apx_group_solve": ([P, P, P, F64, I32, P, P], ctypes.c_int),
     "apx_solver_autotune": ([P, I32, P, P], ctypes.c_int),
     "apx_server_destroy": ([P], None),
     "apx_server_add": ([P, P, P], ctypes.c_int),
@@ -602,6 +606,56 @@ class Server:
     def close(self):
         if getattr(self, "_h", None):
             load_library().apx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
+class Group:
+    """Many small factors in one launch: the PCG solves of several Solvers
+    (one right-hand side each) run together, every kernel kind of an
+    iteration one launch over all members (apx_group_solve). Each member's
+    result is the bits of its own Solver.solve(b, tol, maxiter). Members share
+    the vector precision and factor storage; the group keeps them alive."""
+
+    def __init__(self, solvers):
+        solvers = list(solvers)
+        if not solvers:
+            raise ValueError("a group needs at least one solver")
+        arr = (P * len(solvers))(*[s._h for s in solvers])
+        h = P()
+        _check(load_library().apx_group_create(arr, len(solvers), ctypes.byref(h)))
+        self._h = h
+        self._solvers = solvers
+
+    @property
+    def launches_per_iteration(self) -> int:
+        m, k = I32(), I32()
+        _check(load_library().apx_group_info(self._h, ctypes.byref(m), ctypes.byref(k)))
+        return int(k.value)
+
+    def solve(self, bs, tol: float | None = None, maxiter: int | None = None, outs=None, stream=None):
+        """bs: one float64 CUDA tensor per member (its n values, caller order);
+        returns (xs, results)."""
+        import torch
+        if len(bs) != len(self._solvers):
+            raise ValueError("one right-hand side per member")
+        bs = [s._dev(b, s._n, "b") for s, b in zip(self._solvers, bs)]
+        xs = [torch.empty_like(b) for b in bs] if outs is None else \
+            [s._dev(x, s._n, "out") for s, x in zip(self._solvers, outs)]
+        k = len(bs)
+        pb = (P * k)(*[b.data_ptr() for b in bs])
+        px = (P * k)(*[x.data_ptr() for x in xs])
+        infos = (SolveInfo * k)()
+        _check(load_library().apx_group_solve(self._h, pb, px, -1.0 if tol is None else float(tol),
+                                              -1 if maxiter is None else int(maxiter), infos, _stream(stream)))
+        return xs, [_result(x, info) for x, info in zip(xs, infos)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().apx_group_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -23,7 +23,7 @@ LIB = ROOT / "libapxchol_amd.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
-HOST_SOURCES = ["host/factorize.cpp", "host/trsv_prep.cpp", "host/gpu_solver.cpp", "host/server.cpp", "host/mtx.cpp",
+HOST_SOURCES = ["host/factorize.cpp", "host/trsv_prep.cpp", "host/gpu_solver.cpp", "host/server.cpp", "host/group.cpp", "host/mtx.cpp",
                 "capi.cpp"]
 KERNEL_SOURCES = ["kernels/kernels.hip"]
 

@@ -14,6 +14,7 @@
 
 #include "host/factor.hpp"
 #include "host/gpu_solver.hpp"
+#include "host/group.hpp"
 #include "host/mtx.hpp"
 #include "host/server.hpp"
 #include "host/trsv_prep.hpp"
@@ -29,6 +30,9 @@ struct apx_solver_s {
 };
 struct apx_server_s {
     std::unique_ptr<solve_server> s;
+};
+struct apx_group_s {
+    std::unique_ptr<solver_group> g;
 };
 struct apx_matrix_s {
     csc_matrix A;
@@ -281,6 +285,48 @@ apx_status apx_solver_solve(apx_solver_t s, const double* d_b, double* d_x, cons
         need(s, "solver");
         if (s->s->rows() > 0) { need(d_b, "b"); need(d_x, "x"); }
         fill_info(s->s->solve_device(d_b, d_x, tol, max_iter, d_x0, static_cast<hipStream_t>(stream)), info);
+    });
+}
+
+apx_status apx_group_create(const apx_solver_t* solvers, int32_t count, apx_group_t* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = nullptr;
+        if (count <= 0) throw std::invalid_argument("a group needs at least one solver");
+        need(solvers, "solvers");
+        std::vector<std::shared_ptr<const gpu_solver>> members;
+        for (int32_t i = 0; i < count; ++i) {
+            need(solvers[i], "solver");
+            members.push_back(solvers[i]->s);
+        }
+        auto g = std::make_unique<apx_group_s>();
+        g->g = std::make_unique<solver_group>(std::move(members));
+        *out = g.release();
+    });
+}
+
+void apx_group_destroy(apx_group_t g) { delete g; }
+
+apx_status apx_group_info(apx_group_t g, int32_t* members, int32_t* launches_per_iteration) {
+    return guarded([&] {
+        need(g, "group");
+        if (members) *members = g->g->size();
+        if (launches_per_iteration) *launches_per_iteration = g->g->launches_per_iteration();
+    });
+}
+
+apx_status apx_group_solve(apx_group_t g, const double* const* d_b, double* const* d_x, double tol, int32_t max_iter,
+                           apx_solve_info* infos, void* stream) {
+    return guarded([&] {
+        need(g, "group");
+        need(d_b, "b");
+        need(d_x, "x");
+        const int N = g->g->size();
+        std::vector<const double*> b(d_b, d_b + N);
+        std::vector<double*> x(d_x, d_x + N);
+        const auto r = g->g->solve(b, x, tol, max_iter, static_cast<hipStream_t>(stream));
+        if (infos)
+            for (int i = 0; i < N; ++i) fill_info(r[i], infos + i);
     });
 }
 

@@ -43,9 +43,7 @@ struct sweep_desc {
     const std::uint64_t* batch_mask = nullptr; // nb
     int nb = 0;
     int chunk = 1;                          // batches claimed per ticket (dataflow schedule)
-    int window = 0;                         // dataflow: a ticket starts only while it lies fewer than this many
-                                            // batches past the finished-batch count (0: no limit)
-    int* ctrl = nullptr;                    // 3 ints: ticket, finished waves, finished batches (zero between sweeps)
+    int* ctrl = nullptr;                    // 2 ints: ticket, finished waves (zero between sweeps)
 };
 
 // Sliced-ELL form of the PCG operator (the single-vector SpMV). Slice s holds

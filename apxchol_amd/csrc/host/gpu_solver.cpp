@@ -830,7 +830,7 @@ void gpu_solver::setup(const csc_view& L) {
     d.st.alloc(sizeof(pcg_state));
     d.st_apply.alloc(sizeof(pcg_state));
     d.st_apply.zero();
-    d.ctrl.alloc(8 * sizeof(int));
+    d.ctrl.alloc(4 * sizeof(int));
     d.ctrl.zero();
     d.err.alloc(sizeof(int));
     d.err.zero();
@@ -852,9 +852,8 @@ void gpu_solver::setup(const csc_view& L) {
         s.batch_pos = (backward ? d.b_pos : d.f_pos).as<std::uint32_t>();
         s.batch_mask = (backward ? d.b_mask : d.f_mask).as<std::uint64_t>();
         s.nb = static_cast<int>(bt.mask.size());
-        s.chunk = opts_.sweep_chunk > 0 ? opts_.sweep_chunk : env_int("APXCHOL_AMD_SWEEP_CHUNK", 2);
-        s.window = env_int("APXCHOL_AMD_SWEEP_WINDOW", 4096);
-        s.ctrl = d.ctrl.as<int>() + (backward ? 4 : 0);
+        s.chunk = opts_.sweep_chunk > 0 ? opts_.sweep_chunk : env_int("APXCHOL_AMD_SWEEP_CHUNK", 8);
+        s.ctrl = d.ctrl.as<int>() + (backward ? 2 : 0);
     };
     fill_desc(d.fwd, false, fb);
     fill_desc(d.bck, true, bb);
@@ -863,7 +862,11 @@ void gpu_solver::setup(const csc_view& L) {
     // per CU) its polling traffic grew superlinearly and two 128^3 sweeps
     // exceeded the spin limit (reported, not hung), so it is kept well below.
     d.grid = opts_.sweep_blocks > 0 ? opts_.sweep_blocks : env_int("APXCHOL_AMD_SWEEP_BLOCKS", 0);
-    if (d.grid <= 0) d.grid = std::max(1, sweep_dataflow_grid(d.prec, d.fp16) / 4);
+    const int validated_grid = std::max(1, sweep_dataflow_grid(d.prec, d.fp16) / 4);
+    if (d.grid <= 0) d.grid = validated_grid;
+    if (opts_.schedule == sweep_schedule::dataflow && d.grid > validated_grid)
+        throw std::invalid_argument("gpu_solver: the dataflow schedule runs at most " + std::to_string(validated_grid) +
+                                    " workgroups (2 per CU); larger grids exhausted the spin limit on 128^3 sweeps");
     // Hybrid segments: a level with at least tail_rows rows or more than
     // level_tail_batches batches is launched on its own; every maximal run of the
     // other (small) levels becomes one tail segment.
@@ -1858,6 +1861,95 @@ void gpu_solver::solve_LLt_device(const double* d_x, double* d_y, hipStream_t s)
     launch_sweep(d.fwd, d.prec, d.fp16, d.rp.get(), d.y.get(), nullptr, d.err.as<int>(), d.f_sched, s);
     launch_sweep(d.bck, d.prec, d.fp16, d.y.get(), d.z.get(), nullptr, d.err.as<int>(), d.b_sched, s);
     launch_permute_out(d.prec, m, Q, d.z.get(), d_y, s);
+}
+
+group_member_view gpu_solver::group_view() const {
+    not_serving("group");
+    const auto& d = *dev_;
+    group_member_view v;
+    auto& g = v.mem;
+    g.n = static_cast<int>(n_);
+    g.m = static_cast<int>(T_.m);
+    g.r = d.r.as<double>();
+    g.x = d.x.as<double>();
+    g.p = d.p.as<double>();
+    g.ap = d.ap.as<double>();
+    g.y = d.y.get();
+    g.z = d.z.get();
+    g.part = d.part.as<double>();
+    g.use_sell = d.use_sell ? 1 : 0;
+    g.val32 = d.op32 ? 1 : 0;
+    g.lanes = d.lanes;
+    g.sell = d.sell;
+    g.optr = d.op_ptr.as<unsigned>();
+    g.oidx = d.op_idx.as<int>();
+    g.oval = d.op_val.get();
+    v.perm = d.perm.as<unsigned>();
+    v.sddm = F_.sddm;
+    v.prec = d.prec;
+    v.fp16 = d.fp16;
+    v.stagnation_window = opts_.stagnation_window;
+    v.center_k = center_k_;
+    v.lumped_offdiag = F_.lumped_offdiag;
+    const int m = static_cast<int>(T_.m);
+    const std::size_t vsz = d.prec == vec_prec::fp64 ? sizeof(double) : sizeof(float);
+    for (int dir = 0; dir < 2; ++dir) {
+        const sweep_sched& sc = dir ? d.b_sched : d.f_sched;
+        const sweep_desc& desc = dir ? d.bck : d.fwd;
+        if (desc.nb == 0) continue;
+        if (sc.kind != 2) throw std::invalid_argument("gpu_solver::group: a group member's sweeps must be the hybrid schedule");
+        for (int k = 0; k < sc.nseg; ++k) {
+            if (sc.seg_tail[k]) {
+                group_unit u;
+                u.kind = kGroupTail;
+                u.d = desc;
+                u.b0 = sc.level_first[sc.seg_l0[k]];
+                u.b1 = sc.level_first[sc.seg_l1[k]];
+                u.p0 = sc.seg_p0[k];
+                u.td = sc.tail;
+                u.base = sc.seg_base[k];
+                u.nlev = sc.seg_phases[k];
+                u.zslot = sc.seg_p1[k] - sc.seg_p0[k];
+                u.row0 = dir ? m - 1 - u.p0 : u.p0;
+                u.tv = static_cast<unsigned char*>(sc.tail_val) + static_cast<std::size_t>(u.base) * kWave * 2 * vsz;
+                u.rhs = dir ? d.y.get() : d.r.get();
+                u.out = dir ? d.z.get() : d.y.get();
+                u.err = d.err.as<int>();
+                u.grid = u.b1 - u.b0;
+                u.lds = sweep_tail_lds(d.prec, u.zslot, u.b1 - u.b0);
+                if (u.b1 <= u.b0 || u.base < 0 || u.base + (u.b1 - u.b0) > sc.tail_batches)
+                    throw std::runtime_error("gpu_solver::group: tail run outside the tail layout");
+                v.steps[dir].push_back(u);
+                continue;
+            }
+            for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) {
+                group_unit u;
+                u.d = desc;
+                u.b0 = sc.level_first[l];
+                u.b1 = sc.level_first[l + 1];
+                if (u.b1 <= u.b0) continue;
+                u.rhs = dir ? d.y.get() : d.r.get();
+                u.out = dir ? d.z.get() : d.y.get();
+                u.err = d.err.as<int>();
+                const int lb = sc.level_lb ? sc.level_lb[l] : -1;
+                if (lb >= 0) {
+                    u.kind = kGroupLane;
+                    u.lb0 = lb;
+                    u.nlb = u.b1 - u.b0;
+                    u.rec = sc.lane_rec_dev;
+                    u.ent = sc.lane_ent_dev;
+                    if (static_cast<long long>(lb) + u.nlb > sc.lane_batches)
+                        throw std::runtime_error("gpu_solver::group: level outside the lane layout");
+                    u.grid = (u.nlb + kLevelWavesPerBlock - 1) / kLevelWavesPerBlock;
+                } else {
+                    u.kind = kGroupLevels;
+                    u.grid = std::min((u.b1 - u.b0 + kLevelWavesPerBlock - 1) / kLevelWavesPerBlock, kLevelLaunchMaxBlocks);
+                }
+                v.steps[dir].push_back(u);
+            }
+        }
+    }
+    return v;
 }
 
 void gpu_solver::spmv_device(const double* d_x, double* d_y, hipStream_t s) const {

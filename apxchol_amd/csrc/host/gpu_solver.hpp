@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../device.hpp"
+#include "../kernels/launch.hpp"
 #include "factor.hpp"
 #include "trsv_prep.hpp"
 
@@ -34,7 +35,7 @@ struct solve_options {
                                   // (0: APXCHOL_AMD_TAIL_CUT or 1; < 0: never)
     double drop_rel = -1.0;       // < 0: APXCHOL_FACTOR_DROP / the 1e-4 default
     int iters_per_check = 2;      // iterations enqueued between completion tests
-    int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 2)
+    int sweep_chunk = 0;          // batches per dataflow ticket (0: APXCHOL_AMD_SWEEP_CHUNK or 8)
     int sweep_blocks = 0;         // dataflow grid (0: APXCHOL_AMD_SWEEP_BLOCKS or the resident grid)
     bool graph = true;            // replay each PCG iteration as one captured HIP graph
     bool relabel = true;          // sort every level's rows by original vertex (APXCHOL_AMD_RELABEL=0: off)
@@ -89,7 +90,21 @@ struct solver_stats {
 };
 
 struct block_sched;
-struct launch_hook;
+
+// What a grouped solve (solver_group) needs of one member solver: its device
+// arrays, its solve-state defaults and the steps of its two sweeps (launched
+// levels and tail runs, in order; host copies of the device descriptors).
+struct group_member_view {
+    group_member mem;                  // st is left null: the group owns the states
+    const unsigned* perm = nullptr;    // original vertex -> permuted position (device)
+    bool sddm = true;
+    vec_prec prec = vec_prec::fp64;
+    bool fp16 = false;
+    std::vector<group_unit> steps[2];  // forward, back (st left null)
+    int stagnation_window = 50;
+    int center_k = 10;
+    std::int64_t lumped_offdiag = 0;
+};
 
 class gpu_solver {
 public:
@@ -193,6 +208,9 @@ public:
     void release_serving() const { serving_.store(false); }
     bool serving() const { return serving_.load(); }
     const solve_timings& setup_timings() const { return setup_times_; }
+    // The member view of a grouped solve (see group_member_view). Throws if the
+    // solver is serving or its sweeps are not the hybrid schedule.
+    group_member_view group_view() const;
     const factorization& factor() const { return F_; }
     const trsv_factor& trsv() const { return T_; }
     const solver_stats& stats() const { return stats_; }

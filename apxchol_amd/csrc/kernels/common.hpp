@@ -117,12 +117,18 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* sh) {
 // needs no fence: an agent-scope fence in every workgroup writes back and
 // invalidates its XCD's L2 each time, which halved the bandwidth of the
 // streaming kernels that reduce.
+// blk: the producer's workgroup index (its slot in part); a grouped launch
+// passes the workgroup's index inside its segment's own grid.
 template <int N>
-__device__ __forceinline__ void block_partials(double (&v)[N], double* part, double* sh) {
+__device__ __forceinline__ void block_partials(double (&v)[N], double* part, double* sh, int blk) {
     block_sum<N>(v, sh);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < N; ++k) part[blockIdx.x * N + k] = v[k];
+        for (int k = 0; k < N; ++k) part[blk * N + k] = v[k];
+}
+template <int N>
+__device__ __forceinline__ void block_partials(double (&v)[N], double* part, double* sh) {
+    block_partials<N>(v, part, sh, static_cast<int>(blockIdx.x));
 }
 
 // Stage 2 over the partials of nb workgroups, in block order: strided

@@ -14,8 +14,11 @@
 #include "sweep.hip"
 #include "block.hip"
 #include "pieces.hip"
+#include "group.hip"
 
 namespace apxchol_amd {
+
+static_assert(kWavesPerBlock == kLevelWavesPerBlock, "level launches: host and device waves per workgroup");
 
 int vec_block_cap() {
     static const int cap = [] {
@@ -56,7 +59,7 @@ int dataflow_grid_for() {
 }
 
 // Workgroups of one level launch: one wave per batch, up to this many.
-inline constexpr int kLevelMaxBlocks = 16384;
+inline constexpr int kLevelMaxBlocks = kLevelLaunchMaxBlocks;
 
 template <class TI, class TO, bool F16>
 void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* st, int* err, const sweep_sched& sc,
@@ -139,6 +142,10 @@ int sweep_dataflow_grid(vec_prec p, bool fp16) {
 
 int sweep_tail_max_rows(vec_prec p) {
     return kTailLdsBytes / static_cast<int>(p == vec_prec::fp64 ? sizeof(double) : sizeof(float)) - 1;
+}
+
+std::size_t sweep_tail_lds(vec_prec p, int rows, int batches) {
+    return tail_lds_bytes(rows, batches, p == vec_prec::fp64 ? sizeof(double) : sizeof(float));
 }
 
 bool sweep_tail_fits(vec_prec p, int rows, int batches, int levels) {
@@ -711,6 +718,133 @@ void launch_piece_spmv_pAp_scratch(int n, const unsigned* ptr, const int* idx, c
                                     std::to_string(kScratchDoubles));
     launch_piece_spmv_pAp(n, ptr, idx, val, val32, lanes, x, y, part, s);
     launch_piece_reduce(part, nb, pAp, s);
+}
+
+}  // namespace apxchol_amd
+
+// ---- grouped launches ----------------------------------------------------------
+
+namespace apxchol_amd {
+
+int group_vec_blocks(int n) { return vec_blocks(n); }
+
+int group_spmv_blocks(const group_member& g) {
+    if (!g.use_sell) return vec_blocks(static_cast<long long>(g.n) * g.lanes);
+    // launch_spmv_sell's grid.
+    const long long items = static_cast<long long>(g.sell.nslices) + g.sell.nlong;
+    long long b = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (b > g.sell.max_blocks) b = g.sell.max_blocks;
+    return static_cast<int>(std::max<long long>(kXcds, b / kXcds * kXcds));
+}
+
+int group_fin_op(const char* name) {
+    const std::string k(name);
+    if (k == "pAp") return kFinPAp;
+    if (k == "xr") return kFinXR;
+    if (k == "dot") return kFinDot;
+    if (k == "dot_first") return kFinDotFirst;
+    throw std::invalid_argument("apxchol_amd: unknown reduction stage " + k);
+}
+
+void launch_group_spmv(const group_member* mem, const int* starts, int count, int blocks, hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    group_spmv<<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    check_launch("group_spmv");
+}
+
+void launch_group_finish(int op, const group_member* mem, const int* starts, int count, hipStream_t s) {
+    if (count <= 0) return;
+    group_finish<<<count, kBlock, 0, s>>>(op, mem, starts);
+    check_launch("group_finish");
+}
+
+void launch_group_update_xr(vec_prec p, const group_member* mem, const int* starts, int count, int blocks,
+                            hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    if (p == vec_prec::fp64) group_update_xr<double><<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    else group_update_xr<float><<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    check_launch("group_update_xr");
+}
+
+void launch_group_fill_outputs(vec_prec p, const group_member* mem, const int* starts, int count, int blocks,
+                               hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    if (p == vec_prec::fp64) group_fill_outputs<double><<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    else group_fill_outputs<float><<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    check_launch("group_fill_outputs");
+}
+
+void launch_group_dot_center(vec_prec p, const group_member* mem, const int* starts, int count, int blocks,
+                             hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    if (p == vec_prec::fp64) group_dot_center<double><<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    else group_dot_center<float><<<blocks, kBlock, 0, s>>>(mem, starts, count);
+    check_launch("group_dot_center");
+}
+
+void launch_group_update_p(vec_prec p, const group_member* mem, const int* starts, int count, int blocks, bool first,
+                           hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    if (p == vec_prec::fp64) group_update_p<double><<<blocks, kBlock, 0, s>>>(mem, starts, count, first ? 1 : 0);
+    else group_update_p<float><<<blocks, kBlock, 0, s>>>(mem, starts, count, first ? 1 : 0);
+    check_launch("group_update_p");
+}
+
+void launch_group_sweep_level(vec_prec p, bool fp16, bool backward, const group_unit* units, const int* starts,
+                              int count, int blocks, hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, s, units, starts, count); };
+    if (p == vec_prec::fp64) {
+        if (fp16) go(group_sweep_level<double, double, true>);
+        else go(group_sweep_level<double, double, false>);
+    } else if (!backward) {
+        if (fp16) go(group_sweep_level<double, float, true>);
+        else go(group_sweep_level<double, float, false>);
+    } else {
+        if (fp16) go(group_sweep_level<float, float, true>);
+        else go(group_sweep_level<float, float, false>);
+    }
+    check_launch("group_sweep_level");
+}
+
+void launch_group_sweep_couple(vec_prec p, bool fp16, bool backward, const group_unit* units, const int* starts,
+                               int count, int blocks, hipStream_t s) {
+    if (count <= 0 || blocks <= 0) return;
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kWave), 0, s, units, starts, count); };
+    if (p == vec_prec::fp64) {
+        if (fp16) go(group_sweep_couple<double, double, true>);
+        else go(group_sweep_couple<double, double, false>);
+    } else if (!backward) {
+        if (fp16) go(group_sweep_couple<double, float, true>);
+        else go(group_sweep_couple<double, float, false>);
+    } else {
+        if (fp16) go(group_sweep_couple<float, float, true>);
+        else go(group_sweep_couple<float, float, false>);
+    }
+    check_launch("group_sweep_couple");
+}
+
+void launch_group_sweep_tail(vec_prec p, const group_unit* units, int count, std::size_t lds, hipStream_t s) {
+    if (count <= 0) return;
+    if (lds > static_cast<std::size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: group tail run exceeds LDS");
+    if (p == vec_prec::fp64) {
+        static const bool attr = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&group_sweep_tail<double>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
+            return true;
+        }();
+        (void)attr;
+        group_sweep_tail<double><<<count, kTailBlock, static_cast<unsigned>(lds), s>>>(units);
+    } else {
+        static const bool attr = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&group_sweep_tail<float>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
+            return true;
+        }();
+        (void)attr;
+        group_sweep_tail<float><<<count, kTailBlock, static_cast<unsigned>(lds), s>>>(units);
+    }
+    check_launch("group_sweep_tail");
 }
 
 }  // namespace apxchol_amd

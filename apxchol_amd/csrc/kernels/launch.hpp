@@ -15,6 +15,8 @@ int sweep_dataflow_grid(vec_prec p, bool fp16);
 int sweep_tail_max_rows(vec_prec p);
 // Whether a tail run of `rows` rows, `batches` batches and `levels` levels fits the tail workgroup.
 bool sweep_tail_fits(vec_prec p, int rows, int batches, int levels);
+// LDS bytes of the walk of a tail run of `rows` rows and `batches` batches.
+std::size_t sweep_tail_lds(vec_prec p, int rows, int batches);
 
 // How one sweep direction is scheduled.
 //   dataflow: one persistent launch of `grid` workgroups (polling);
@@ -162,6 +164,76 @@ void launch_gather_cols_k(int n, int kp, const unsigned* perm, int ncol, const i
                           const double* x, const pcg_state* st, bool use_mean, hipStream_t s);
 void launch_gather_k(vec_prec p, int n, int k, int kp, const unsigned* perm, const void* x, double* X,
                      const pcg_state* st, bool use_mean, hipStream_t s);
+
+// ---- grouped launches: several factors' PCG iterations, one launch per kernel kind ----
+//
+// A member is one factor's solver state (its own vectors, partials, solve
+// state and SpMV form); a unit is one step of one member's sweep (a launched
+// level, or a tail run = its coupling launch + its one-workgroup walk). A
+// grouped launch concatenates the members' (units') own grids; starts[i] is
+// the first workgroup of entry i, starts[count] the launch's grid (device
+// arrays).
+struct group_member {
+    int n = 0, m = 0;
+    double* r = nullptr;
+    double* x = nullptr;
+    double* p = nullptr;
+    double* ap = nullptr;
+    void* y = nullptr;                      // sweep outputs (the solver's vector precision)
+    void* z = nullptr;
+    pcg_state* st = nullptr;
+    double* part = nullptr;
+    int use_sell = 0, val32 = 0, lanes = 1;
+    sell_desc sell;
+    const unsigned* optr = nullptr;         // CSR operator (use_sell == 0)
+    const int* oidx = nullptr;
+    const void* oval = nullptr;
+};
+inline constexpr int kGroupLane = 0, kGroupLevels = 1, kGroupTail = 2;
+// Workgroups of one level launch from the batch tables: one wave per batch, at most this many;
+// waves per workgroup of the level launches.
+inline constexpr int kLevelLaunchMaxBlocks = 16384;
+inline constexpr int kLevelWavesPerBlock = 4;
+struct group_unit {
+    int kind = kGroupLane;
+    sweep_desc d;
+    int lb0 = 0, nlb = 0;                   // lane level: lane batches [lb0, lb0 + nlb)
+    const lane_rec* rec = nullptr;
+    const entry32* ent = nullptr;
+    int b0 = 0, b1 = 0;                     // batch-table level / tail run: batches [b0, b1)
+    int p0 = 0;                             // tail run: its first sweep position
+    tail_desc td;                           // tail run: the direction's tail layout
+    int base = 0, nlev = 0, zslot = 0, row0 = 0;
+    void* tv = nullptr;                     // tail run: its coupling values
+    const void* rhs = nullptr;
+    void* out = nullptr;
+    const pcg_state* st = nullptr;
+    int* err = nullptr;
+    int grid = 0;                           // workgroups of the unit's own launch (couple: batches)
+    std::size_t lds = 0;                    // tail run: LDS bytes of its walk
+};
+// The own grid of a member's single-vector SpMV and of its vector passes.
+int group_spmv_blocks(const group_member& g);
+int group_vec_blocks(int n);
+void launch_group_spmv(const group_member* mem, const int* starts, int count, int blocks, hipStream_t s);
+void launch_group_finish(int op, const group_member* mem, const int* starts, int count, hipStream_t s);
+void launch_group_update_xr(vec_prec p, const group_member* mem, const int* starts, int count, int blocks,
+                            hipStream_t s);
+void launch_group_fill_outputs(vec_prec p, const group_member* mem, const int* starts, int count, int blocks,
+                               hipStream_t s);
+void launch_group_dot_center(vec_prec p, const group_member* mem, const int* starts, int count, int blocks,
+                             hipStream_t s);
+void launch_group_update_p(vec_prec p, const group_member* mem, const int* starts, int count, int blocks, bool first,
+                           hipStream_t s);
+// One step of the members' sweeps in one direction: the launched levels, the
+// coupling launches of the tail runs, then the tail runs (count units each).
+void launch_group_sweep_level(vec_prec p, bool fp16, bool backward, const group_unit* units, const int* starts,
+                              int count, int blocks, hipStream_t s);
+void launch_group_sweep_couple(vec_prec p, bool fp16, bool backward, const group_unit* units, const int* starts,
+                               int count, int blocks, hipStream_t s);
+void launch_group_sweep_tail(vec_prec p, const group_unit* units, int count, std::size_t lds, hipStream_t s);
+// The reduction-stage op codes of pcg_finish (kFinPAp ...), for the group host.
+int group_fin_op(const char* name);
 
 // ---- stand-alone PCG pieces (pcg_cuda_kernels.h:43-93): caller-owned partials,
 // one double per producer workgroup, pcg_blocks(units) of them ----

@@ -18,16 +18,19 @@ namespace apxchol_amd {
 // entries + xor butterfly). With want_dot, the partials of x.y go to part;
 // pcg_finish (kFinPAp) sets st->pAp and ends the solve on a non-positive pAp
 // (solve.cpp:596, breakdown).
+// vb / vg: the workgroup's index and the grid (a grouped launch passes the
+// workgroup's index inside its segment's own grid); the same in every kernel
+// body below.
 template <class VT, int G>
-__global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __restrict__ ptr,
-                                                   const int* __restrict__ idx, const VT* __restrict__ val,
-                                                   const double* __restrict__ x, double* __restrict__ y,
-                                                   const pcg_state* st, double* part, int want_dot) {
+__device__ __forceinline__ void spmv_pAp_body(int vb, int vg, int n, const unsigned* __restrict__ ptr,
+                                              const int* __restrict__ idx, const VT* __restrict__ val,
+                                              const double* __restrict__ x, double* __restrict__ y,
+                                              const pcg_state* st, double* part, int want_dot) {
     if (st && st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
-    const int tid = blockIdx.x * kBlock + threadIdx.x;
+    const int tid = vb * kBlock + threadIdx.x;
     const int sub = tid & (G - 1);
-    const int stride = gridDim.x * kBlock / G;
+    const int stride = vg * kBlock / G;
     double acc = 0.0;
     for (int row = tid / G; row < n; row += stride) {
         const unsigned b = ptr[row], e = ptr[row + 1];
@@ -42,7 +45,15 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    block_partials<1>(v, part, sh);
+    block_partials<1>(v, part, sh, vb);
+}
+template <class VT, int G>
+__global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __restrict__ ptr,
+                                                   const int* __restrict__ idx, const VT* __restrict__ val,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const pcg_state* st, double* part, int want_dot) {
+    spmv_pAp_body<VT, G>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), n, ptr, idx, val, x, y, st, part,
+                         want_dot);
 }
 
 // y = A x over the sliced-ELL form (sell_desc): one lane per row, a slice's
@@ -52,8 +63,9 @@ __global__ __launch_bounds__(kBlock) void spmv_pAp(int n, const unsigned* __rest
 // row's sum is its entries in CSR order (padding adds fma(0, x[row], s) = s).
 // With want_dot, the partials of x.y go to part as in spmv_pAp.
 template <class VT>
-__global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* __restrict__ x, double* __restrict__ y,
-                                                    const pcg_state* st, double* part, int want_dot) {
+__device__ __forceinline__ void spmv_sell_body(int vb, int vg, const sell_desc& a, const double* __restrict__ x,
+                                               double* __restrict__ y, const pcg_state* st, double* part,
+                                               int want_dot) {
     if (st && st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     const int lane = threadIdx.x & (kWave - 1);
@@ -63,13 +75,12 @@ __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* _
     // vertex order each XCD's L2 then serves one band of rows and the x
     // entries they gather. (The grid is a multiple of kXcds; the split only
     // places work, every item is done once whatever the placement.)
-    const int band = static_cast<int>(blockIdx.x) % kXcds;
-    const int per = static_cast<int>(gridDim.x) / kXcds;
+    const int band = vb % kXcds;
+    const int per = vg / kXcds;
     const int items = a.nslices + a.nlong;
     const int i0 = static_cast<int>(static_cast<long long>(items) * band / kXcds);
     const int i1 = static_cast<int>(static_cast<long long>(items) * (band + 1) / kXcds);
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) / kXcds * kWavesPerBlock +
-                                                    static_cast<int>(threadIdx.x) / kWave);
+    const int wave = __builtin_amdgcn_readfirstlane(vb / kXcds * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave);
     const int* __restrict__ sidx = a.idx;
     const VT* __restrict__ sval = static_cast<const VT*>(a.val);
     double acc = 0.0;
@@ -116,7 +127,12 @@ __global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* _
     }
     if (!want_dot) return;
     double v[1] = {acc};
-    block_partials<1>(v, part, sh);
+    block_partials<1>(v, part, sh, vb);
+}
+template <class VT>
+__global__ __launch_bounds__(kBlock) void spmv_sell(sell_desc a, const double* __restrict__ x, double* __restrict__ y,
+                                                    const pcg_state* st, double* part, int want_dot) {
+    spmv_sell_body<VT>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), a, x, y, st, part, want_dot);
 }
 
 // r[perm[v]] = b[v]; x[perm[v]] = x0[v] (or 0).
@@ -167,10 +183,10 @@ __global__ __launch_bounds__(kBlock) void init_residual(int n, double* __restric
 // (solve.cpp:169-208), plus the sentinel fill of the next application's sweep
 // outputs. The loop's bookkeeping follows in pcg_finish (kFinXR).
 template <class TO>
-__global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __restrict__ x, const double* __restrict__ p,
-                                                    double* __restrict__ r, const double* __restrict__ ap,
-                                                    TO* __restrict__ y, TO* __restrict__ z, const pcg_state* st,
-                                                    double* part) {
+__device__ __forceinline__ void update_xr_body(int vb, int vg, int n, int m, double* __restrict__ x,
+                                               const double* __restrict__ p, double* __restrict__ r,
+                                               const double* __restrict__ ap, TO* __restrict__ y, TO* __restrict__ z,
+                                               const pcg_state* st, double* part) {
     if (st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     const double alpha = st->rz / st->pAp;
@@ -178,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __rest
     B* yb = reinterpret_cast<B*>(y);
     B* zb = reinterpret_cast<B*>(z);
     double q = 0.0, s = 0.0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int i = vb * kBlock + threadIdx.x; i < n; i += vg * kBlock) {
         x[i] = fma(alpha, p[i], x[i]);
         const double ri = fma(-alpha, ap[i], r[i]);
         r[i] = ri;
@@ -188,52 +204,74 @@ __global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __rest
         else z[i] = static_cast<TO>(0);
     }
     double v[2] = {q, s};
-    block_partials<2>(v, part, sh);
+    block_partials<2>(v, part, sh, vb);
+}
+template <class TO>
+__global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __restrict__ x, const double* __restrict__ p,
+                                                    double* __restrict__ r, const double* __restrict__ ap,
+                                                    TO* __restrict__ y, TO* __restrict__ z, const pcg_state* st,
+                                                    double* part) {
+    update_xr_body<TO>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), n, m, x, p, r, ap, y, z, st, part);
 }
 
 // Sentinel fill for the first application of a solve (update_xr does it for
 // every later one).
 template <class TO>
-__global__ __launch_bounds__(kBlock) void fill_outputs(int n, int m, TO* __restrict__ y, TO* __restrict__ z,
-                                                       const pcg_state* st) {
+__device__ __forceinline__ void fill_outputs_body(int vb, int vg, int n, int m, TO* __restrict__ y,
+                                                  TO* __restrict__ z, const pcg_state* st) {
     if (st && st->done) return;
     using B = typename flag_bits<TO>::type;
     B* yb = reinterpret_cast<B*>(y);
     B* zb = reinterpret_cast<B*>(z);
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int i = vb * kBlock + threadIdx.x; i < n; i += vg * kBlock) {
         if (i < m) { yb[i] = flag_bits<TO>::pending; zb[i] = flag_bits<TO>::pending; }
         else z[i] = static_cast<TO>(0);
     }
+}
+template <class TO>
+__global__ __launch_bounds__(kBlock) void fill_outputs(int n, int m, TO* __restrict__ y, TO* __restrict__ z,
+                                                       const pcg_state* st) {
+    fill_outputs_body<TO>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), n, m, y, z, st);
 }
 
 // Partials of r.out and sum(out) of the application's output; kFinDot then
 // sets the output centring shift mu = mean(out) on centring applications
 // (preconditioner.h:516-525), r.z = r.out - mu sum(r), beta = r.z / previous r.z.
 template <class TO>
-__global__ __launch_bounds__(kBlock) void dot_center(int n, const double* __restrict__ r, const TO* __restrict__ out,
-                                                     const pcg_state* st, double* part) {
+__device__ __forceinline__ void dot_center_body(int vb, int vg, int n, const double* __restrict__ r,
+                                                const TO* __restrict__ out, const pcg_state* st, double* part) {
     if (st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     double d = 0.0, s = 0.0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int i = vb * kBlock + threadIdx.x; i < n; i += vg * kBlock) {
         const double o = static_cast<double>(out[i]);
         d = fma(r[i], o, d);
         s += o;
     }
     double v[2] = {d, s};
-    block_partials<2>(v, part, sh);
+    block_partials<2>(v, part, sh, vb);
+}
+template <class TO>
+__global__ __launch_bounds__(kBlock) void dot_center(int n, const double* __restrict__ r, const TO* __restrict__ out,
+                                                     const pcg_state* st, double* part) {
+    dot_center_body<TO>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), n, r, out, st, part);
 }
 
 // p = (out - mu) + beta p  (first application: p = out - mu).
 template <class TO>
-__global__ __launch_bounds__(kBlock) void update_p(int n, double* __restrict__ p, const TO* __restrict__ out,
-                                                   const pcg_state* st, int first) {
+__device__ __forceinline__ void update_p_body(int vb, int vg, int n, double* __restrict__ p,
+                                              const TO* __restrict__ out, const pcg_state* st, int first) {
     if (st->done) return;
     const double mu = st->mu, beta = st->beta;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int i = vb * kBlock + threadIdx.x; i < n; i += vg * kBlock) {
         const double zi = static_cast<double>(out[i]) - mu;
         p[i] = first ? zi : fma(beta, p[i], zi);
     }
+}
+template <class TO>
+__global__ __launch_bounds__(kBlock) void update_p(int n, double* __restrict__ p, const TO* __restrict__ out,
+                                                   const pcg_state* st, int first) {
+    update_p_body<TO>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), n, p, out, st, first);
 }
 
 // Final min-norm centring (Laplacian, solve.cpp:262-269) folded into the
@@ -314,8 +352,8 @@ __global__ __launch_bounds__(kBlock) void apply_gather(int n, const unsigned* __
 // their producers do not run then.
 enum : int { kFinPAp, kFinBnorm, kFinInit, kFinInitX0, kFinXR, kFinDot, kFinDotFirst, kFinOutSum, kFinShift };
 
-__global__ __launch_bounds__(kBlock) void pcg_finish(int op, int n, const double* __restrict__ part, int nb,
-                                                     pcg_state* st) {
+__device__ __forceinline__ void pcg_finish_body(int op, int n, const double* __restrict__ part, int nb,
+                                                pcg_state* st) {
     if ((op == kFinPAp || op == kFinXR || op == kFinDot || op == kFinDotFirst) && st->done) return;
     __shared__ double sh[kWavesPerBlock * 2];
     double v[2] = {0.0, 0.0};
@@ -394,6 +432,10 @@ __global__ __launch_bounds__(kBlock) void pcg_finish(int op, int n, const double
         default:
             break;
     }
+}
+__global__ __launch_bounds__(kBlock) void pcg_finish(int op, int n, const double* __restrict__ part, int nb,
+                                                     pcg_state* st) {
+    pcg_finish_body(op, n, part, nb, st);
 }
 
 // Between factor order and storage order: dst[Q[q]] = src[q] (in) or

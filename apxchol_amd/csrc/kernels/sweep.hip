@@ -42,16 +42,6 @@ __device__ __forceinline__ bool sweep_failed(const int* err) {
     return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// Exponential backoff between polls: `nap` units of s_sleep 1 (64 clocks
-// each), doubled after every empty poll up to `cap`, so waves that wait long
-// put less and less load on the memory system.
-__device__ __forceinline__ void backoff(unsigned& nap, unsigned cap) {
-    for (unsigned i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
-    nap = nap < cap ? 2 * nap : cap;
-}
-inline constexpr unsigned kPollNapCap = 8;      // rows waiting on their dependencies
-inline constexpr unsigned kClaimNapCap = 64;    // tickets waiting on the claim window
-
 __device__ __forceinline__ int lanes_for(unsigned len) {
     unsigned g = (len + kEpl - 1) / kEpl;
     if (g <= 1) return 1;
@@ -210,12 +200,12 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
                 pend |= valid ? (1u << k) : 0u;
             }
             if constexpr (WAIT) {
-                unsigned spins = 0, nap = 1;
+                unsigned spins = 0;
                 for (;;) {
                     if (pend) poll<TO>(out, jj, xv, pend);
                     if (__ballot(pend != 0) == 0) break;
                     if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) { ok = false; break; }
-                    backoff(nap, kPollNapCap);
+                    __builtin_amdgcn_s_sleep(1);
                 }
             } else {
                 gather<TO>(out, jj, xv, pend);
@@ -261,7 +251,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         return true;
     }
     bool done = !active;
-    unsigned spins = 0, nap = 1;
+    unsigned spins = 0;
     for (;;) {
         if (pend) poll<TO>(out, jj, xv, pend);
         const unsigned long long ready = __ballot(pend == 0);
@@ -281,7 +271,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
             if (!done && sub == 0) publish<TO, WAIT>(out + row, static_cast<AT>(0));
             break;
         }
-        if (!go) backoff(nap, kPollNapCap);
+        if (!go) __builtin_amdgcn_s_sleep(1);
     }
     return ok;
 }
@@ -296,68 +286,42 @@ __global__ __launch_bounds__(kBlock) void sweep_dataflow(sweep_desc d, const TI*
     // A wave claims d.chunk consecutive batches per ticket and runs them in
     // order, so it only ever waits on batches it already finished or on
     // smaller tickets held by running waves.
-    //
-    // Claim window: a ticket is started only once it lies fewer than
-    // d.window batches past the count of finished batches (ctrl[2], added
-    // per finished ticket). Every batch below the smallest unfinished ticket
-    // is finished, so that ticket is always inside the window and the sweep
-    // cannot stall on it; the waves holding tickets far ahead of the frontier
-    // sleep on the one counter (with a growing nap) instead of polling the
-    // output slots of rows whose dependencies are many levels away. Without
-    // the window every resident wave polls from the moment it claims: at the
-    // full resident grid that is ~57,000 batches ahead of the frontier on a
-    // 128^3 forward sweep (7168 waves x 8 batches per ticket), all polling
-    // with agent-scope loads that compete with the frontier's own loads and
-    // publishing stores. The sweep slowed with the polling population
-    // (128^3 solve: 448 -> 224 workgroups 158 -> 117 ms, 8 -> 2 batches per
-    // ticket 158 -> 81 ms), and at the full grid waiting waves exhausted the
-    // spin limit.
     for (;;) {
         int b0 = 0;
         if (lane == 0) b0 = atomicAdd(&d.ctrl[0], d.chunk);
         b0 = __shfl(b0, 0, kWave);
         if (b0 >= d.nb) break;
-        if (d.window > 0) {
-            unsigned spins = 0, nap = 1;
-            for (;;) {
-                int fin = 0;
-                if (lane == 0) fin = __hip_atomic_load(&d.ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                fin = __shfl(fin, 0, kWave);
-                if (b0 - fin < d.window) break;
-                if (++spins > kSpinLimit || ((spins & 1023u) == 0 && sweep_failed(err))) {
-                    if (lane == 0) atomicOr(err, 1);
-                    ok = false;
-                    break;
-                }
-                backoff(nap, kClaimNapCap);
-            }
-        }
         const int b1 = min(b0 + d.chunk, d.nb);
         for (int b = b0; b < b1; ++b) {
             const bool bok = sweep_batch<TI, TO, F16, true>(d, b, rhs, out, shift, err);
             if (!bok && lane == 0) atomicOr(err, 1);
             ok = ok && bok;
         }
-        if (lane == 0) __hip_atomic_fetch_add(&d.ctrl[2], b1 - b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (lane == 0) {
         const int prev = atomicAdd(&d.ctrl[1], 1);
         if (prev == total_waves - 1) {
             __hip_atomic_store(&d.ctrl[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&d.ctrl[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&d.ctrl[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
 
 template <class TI, class TO, bool F16>
-__global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int b1, const TI* __restrict__ rhs,
-                                                       TO* out, const pcg_state* st, int* err) {
+__device__ __forceinline__ void sweep_levels_body(int vb, int vg, const sweep_desc& d, int b0, int b1,
+                                                  const TI* __restrict__ rhs, TO* out, const pcg_state* st,
+                                                  int* err) {
     if (st && st->done) return;
     const double shift = (st && !d.backward) ? st->shift : 0.0;
-    const int waves = static_cast<int>(gridDim.x) * kWavesPerBlock;
-    const int w = static_cast<int>(blockIdx.x) * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
+    const int waves = vg * kWavesPerBlock;
+    const int w = vb * kWavesPerBlock + static_cast<int>(threadIdx.x) / kWave;
     for (int b = b0 + w; b < b1; b += waves) sweep_batch<TI, TO, F16, false>(d, b, rhs, out, shift, err);
+}
+template <class TI, class TO, bool F16>
+__global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int b1, const TI* __restrict__ rhs,
+                                                       TO* out, const pcg_state* st, int* err) {
+    sweep_levels_body<TI, TO, F16>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), d, b0, b1, rhs, out,
+                                   st, err);
 }
 
 // One launched level from the static lane layout: wave w runs lane batches
@@ -368,11 +332,11 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
 // per-row arithmetic is sweep_batch's (same slots, fma chain and butterfly),
 // so the bits equal the other schedules'.
 template <class TI, class TO, bool F16, int BS = kBlock, int BPW = 1>
-__global__ __launch_bounds__(BS) void sweep_lanes(sweep_desc d, int lb0, int nlb, const lane_rec* __restrict__ rec,
-                                                  const entry32* __restrict__ ent, const TI* __restrict__ rhs,
-                                                  TO* out, const pcg_state* st) {
+__device__ __forceinline__ void sweep_lanes_body(int vb, const sweep_desc& d, int lb0, int nlb,
+                                                 const lane_rec* __restrict__ rec, const entry32* __restrict__ ent,
+                                                 const TI* __restrict__ rhs, TO* out, const pcg_state* st) {
     using AT = TO;
-    const int w = static_cast<int>(blockIdx.x) * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
+    const int w = vb * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
     const int first = w * BPW;
     if (first >= nlb) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -415,6 +379,12 @@ __global__ __launch_bounds__(BS) void sweep_lanes(sweep_desc d, int lb0, int nlb
         part = group_tree_sum(part, G);
         if (head) out[row] = (r - part) / dg;
     }
+}
+template <class TI, class TO, bool F16, int BS = kBlock, int BPW = 1>
+__global__ __launch_bounds__(BS) void sweep_lanes(sweep_desc d, int lb0, int nlb, const lane_rec* __restrict__ rec,
+                                                  const entry32* __restrict__ ent, const TI* __restrict__ rhs,
+                                                  TO* out, const pcg_state* st) {
+    sweep_lanes_body<TI, TO, F16, BS, BPW>(static_cast<int>(blockIdx.x), d, lb0, nlb, rec, ent, rhs, out, st);
 }
 
 // A run of small levels [l0, l1) -- sweep positions [p0, p1) -- walked by ONE
@@ -484,14 +454,13 @@ __device__ __forceinline__ lane_row map_lane(const sweep_desc& d, int b) {
 // the run's first tail batch; column c's values start nb * 64 further on per
 // column (nb = blast - bfirst).
 template <class TI, class TO, bool F16, int BS = kBlock>
-__global__ __launch_bounds__(BS) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
-                                                   const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
-                                                   const pcg_state* st, int kp, const batch_state* bs) {
+__device__ __forceinline__ void sweep_couple_body(int vb, int c, const sweep_desc& d, int bfirst, int blast, int p0,
+                                                  const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
+                                                  const pcg_state* st, int kp, const batch_state* bs) {
     if (bs ? bs->active == 0 : (st && st->done)) return;
-    const int c = static_cast<int>(blockIdx.y);
     const double shift = (st && !d.backward) ? st[c].shift : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
-    const int b = bfirst + static_cast<int>(blockIdx.x) * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
+    const int b = bfirst + vb * (BS / kWave) + static_cast<int>(threadIdx.x) / kWave;
     if (b >= blast) return;
     const lane_row x = map_lane(d, b);
     const auto value = [&](int j) { return out[static_cast<size_t>(j) * kp + c]; };
@@ -524,6 +493,13 @@ __global__ __launch_bounds__(BS) void sweep_couple(sweep_desc d, int bfirst, int
     o.part0 = part;
     o.r = r;
     tv[(static_cast<size_t>(c) * (blast - bfirst) + (b - bfirst)) * kWave + lane] = o;
+}
+template <class TI, class TO, bool F16, int BS = kBlock>
+__global__ __launch_bounds__(BS) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
+                                                   const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
+                                                   const pcg_state* st, int kp, const batch_state* bs) {
+    sweep_couple_body<TI, TO, F16, BS>(static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), d, bfirst, blast,
+                                       p0, rhs, out, tv, st, kp, bs);
 }
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -632,13 +608,12 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
 // one vector); tv: the run's coupling values, nb * 64 per column; bs, when
 // given, stops the launch once no column is active (else st->done does).
 template <class TO>
-__global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, int nb, int nlev, int zslot,
-                                                         int row0, int backward, TO* out, const tail_val<TO>* tv,
-                                                         const pcg_state* st, int kp, const batch_state* bs) {
+__device__ __forceinline__ void sweep_tail_body(int col, unsigned char* tail_lds, const tail_desc& td, int base,
+                                                int nb, int nlev, int zslot, int row0, int backward, TO* out,
+                                                const tail_val<TO>* tv, const pcg_state* st, int kp,
+                                                const batch_state* bs) {
     if (bs ? bs->active == 0 : (st && st->done)) return;
-    const int col = static_cast<int>(blockIdx.x);
     tv += static_cast<size_t>(col) * nb * kWave;
-    extern __shared__ unsigned char tail_lds[];
     TO* lds = reinterpret_cast<TO*>(tail_lds);
     // Past the outputs and the zero slot: the run's walk, as (next batch of
     // the wave, its phase) pairs, and every wave's first batch and phase.
@@ -719,6 +694,14 @@ __global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base,
     }
     for (int q = tid; q < zslot; q += kTailBlock)
         out[static_cast<size_t>(backward ? row0 - q : row0 + q) * kp + col] = lds[q];
+}
+template <class TO>
+__global__ __launch_bounds__(kTailBlock) void sweep_tail(tail_desc td, int base, int nb, int nlev, int zslot,
+                                                         int row0, int backward, TO* out, const tail_val<TO>* tv,
+                                                         const pcg_state* st, int kp, const batch_state* bs) {
+    extern __shared__ unsigned char tail_lds[];
+    sweep_tail_body<TO>(static_cast<int>(blockIdx.x), tail_lds, td, base, nb, nlev, zslot, row0, backward, out, tv,
+                        st, kp, bs);
 }
 
 // Fill a vector with the pending sentinel (the state every sweep output must

@@ -221,6 +221,27 @@ apx_status apx_server_wait(apx_server_t srv, int64_t request_id, double* x, apx_
                            int32_t* ready);
 apx_status apx_server_get_stats(apx_server_t srv, apx_server_stats* st);
 
+/* ---- many small factors in one launch (no counterpart in the reference) ---
+ * A group runs the PCG solves of several resident solvers together: every
+ * kernel kind of an iteration (SpMV, vector passes, reductions, each step of
+ * the two sweeps) is ONE launch over all members, each member's share being
+ * the launch its own solve would make. Meant for many small factors, none of
+ * which fills the GPU alone. Members: 1..64 solvers of one vector precision
+ * and factor storage on the hybrid schedule (the default); the group keeps
+ * them alive and uses their device arrays while it solves (a member runs no
+ * other solve meanwhile; a serving solver is refused). */
+typedef struct apx_group_s* apx_group_t;
+apx_status apx_group_create(const apx_solver_t* solvers, int32_t count, apx_group_t* out);
+void apx_group_destroy(apx_group_t g);
+/* Members and the launches of one grouped PCG iteration (all members). */
+apx_status apx_group_info(apx_group_t g, int32_t* members, int32_t* launches_per_iteration);
+/* One right-hand side per member: d_b[i], d_x[i] are member i's n doubles
+ * (device, its caller order); tol / max_iter as in apx_solver_solve. Every
+ * member's x and info are the bits of its own apx_solver_solve(b, x, NULL,
+ * tol, max_iter). infos: count entries, or NULL. */
+apx_status apx_group_solve(apx_group_t g, const double* const* d_b, double* const* d_x, double tol, int32_t max_iter,
+                           apx_solve_info* infos, void* stream);
+
 /* cpu_solver::apply(r) = apx_cholesky::_solve_impl (solve.h:103, preconditioner.h:344-353):
  * z = M^-1 r with the center-k schedule (every K-th application centres). */
 apx_status apx_solver_apply(apx_solver_t s, const double* d_r, double* d_z, void* stream);

@@ -1,5 +1,7 @@
 """The dataflow sweep's claim protocol (sweep.hip, sweep_dataflow), simulated
-on the CPU over a real factor's forward dependency structure.
+on the CPU over a real factor's forward dependency structure -- as the
+kernel runs it (no claim window) and with a claim window, a design candidate
+for bounding the number of polling waves.
 
 Batches are up to 64 consecutive rows of one topological-depth level, in
 sweep order, as the device batch table holds them (a batch never straddles a
@@ -13,11 +15,11 @@ publishes the batch when all of them were published in an earlier step.
 
 Checked: every configuration finishes (no wave waits forever on a ticket
 nobody can finish), every batch is published exactly once and after all of
-its dependencies, and the window bounds the tickets whose waves poll row
-slots to ceil(W / chunk) + 1 -- the protocol's guarantee that the polling
-population does not grow with the grid. A windowed run (512 waves, a window
-of 8 tickets) finishes within a few steps of the unwindowed one with less
-than half its polls."""
+its dependencies; without a window the polling population grows with the
+number of resident waves (every claimed ticket polls), and a window bounds
+the tickets whose waves poll row slots to ceil(W / chunk) + 1. A windowed
+run (512 waves, a window of 8 tickets) finishes within a few steps of the
+unwindowed one with less than half its polls."""
 import numpy as np
 import pytest
 
@@ -120,3 +122,10 @@ def test_window_cuts_polls_not_progress(dag):
     steps1, polls1, _ = simulate(dag, 512, 2, 16)
     assert steps1 <= steps0 + 8
     assert polls1 * 2 < polls0
+
+
+def test_without_a_window_every_claimed_ticket_polls(dag):
+    """The kernel's protocol: the peak number of polling waves follows the
+    resident grid (all of them poll from the moment they claim)."""
+    peaks = [simulate(dag, w, 2, 0)[2] for w in (8, 64, 512)]
+    assert peaks[0] <= 8 and peaks[1] == 64 and peaks[2] >= 80
