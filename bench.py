@@ -15,7 +15,9 @@ autotuned SpMV form and block width) are timed on the same system, against
 its plain fp64 single-vector rate, and on the 64^3 anisotropic grid
 (weights 1/10/100 x exp(N(0,0.5))); the R-MAT scale-22 Laplacian (edge
 factor 16, weights U[1,10), per-component mean-projected N(0,1) right-hand
-side) is timed in mixed and fp64 single solves and one blocked k = 32 solve; the serving mix runs 8 resident factors
+side) is timed in mixed and fp64 single solves and one blocked k = 32 solve;
+the serving mix's factors of at most 300k rows are solved one after another
+and as one grouped solve (many small factors in one launch per kernel kind); the serving mix runs 8 resident factors
 (2-D grids 1e4..1e6, 3-D grids to 96^3, R-MAT scales 16..20) under Poisson
 arrivals and reports p50 / p99 latency and solves/s.
 
@@ -147,10 +149,30 @@ def run_serving(args, ax, mg, np) -> dict:
     srv.close()
     print(f"[bench] serving: {st['solves_per_s']:.1f} solves/s p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms",
           file=sys.stderr, flush=True)
-    return {"factors": [m[0] for m in mats], "requests": len(plan), "qps_offered": args.serving_qps,
-            "slots": args.serving_slots, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
-            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "wall_s": wall, "setup_s": t_setup,
-            "worst_true_residual_over_tol": worst, "memory_used_bytes": st["memory_used"]}
+    out = {"factors": [m[0] for m in mats], "requests": len(plan), "qps_offered": args.serving_qps,
+           "slots": args.serving_slots, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
+           "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "wall_s": wall, "setup_s": t_setup,
+           "worst_true_residual_over_tol": worst, "memory_used_bytes": st["memory_used"]}
+    # Capacity: the same requests submitted at once to a fresh server over the
+    # same resident solvers (the first server released them), so the rate is
+    # what the device sustains rather than the offered load.
+    cap = ax.Server(slots=args.serving_slots, iters_per_check=2)
+    cfids = [cap.add(S) for S in solvers]
+    t2 = time.perf_counter()
+    cids = [cap.submit(cfids[f], rhs[i], tol) for i, (f, tol, _) in enumerate(plan)]
+    cworst = 0.0
+    for i, (f, tol, _) in enumerate(plan):
+        x, r = cap.wait(cids[i], timeout=120)
+        L = mats[f][1]
+        cworst = max(cworst, float(np.linalg.norm(rhs[i] - L @ x) / np.linalg.norm(rhs[i])) / tol)
+    cst = cap.stats()
+    cap.close()
+    out["capacity"] = {"requests": len(plan), "submitted_at_once": True, "solves_per_s": cst["solves_per_s"],
+                       "p50_ms": cst["p50_ms"], "p99_ms": cst["p99_ms"], "wall_s": time.perf_counter() - t2,
+                       "worst_true_residual_over_tol": cworst}
+    print(f"[bench] serving capacity (all {len(plan)} requests at once): {cst['solves_per_s']:.1f} solves/s "
+          f"p50 {cst['p50_ms']:.1f} ms p99 {cst['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+    return out
 
 
 def run_cpu_baseline(args) -> dict | None:
@@ -281,6 +303,61 @@ def run_second_workload(args, ax, mg, np, torch, dev) -> dict:
     return out
 
 
+def run_group(args, ax, mg, np, torch, dev) -> dict:
+    """Many small factors in one launch: the serving mix's factors of at most
+    300k rows (mixed precision), one N(0,1) right-hand side each (projected
+    per component on a Laplacian), solved one after another with their own
+    solves and as one grouped solve (every kernel kind of an iteration one
+    launch over all members); both timed after a warm-up, graph replay."""
+    names, solvers, bs, mats = [], [], [], []
+    for nm in mg.SERVING_FACTORS:
+        if nm in ("grid2d_1000", "grid3d_96", "rmat_20"):   # the mix's factors above 300k rows
+            continue
+        A, _ = mg.workload(nm)
+        i = len(names)
+        names.append(nm)
+        F = ax.factorize(A, seed=0)
+        solvers.append(ax.Solver(A, factor=F, precision="fp32"))
+        b = mg.request_rhs(A, F.sddm, 500 + i)
+        bs.append(torch.tensor(b, device=dev))
+        mats.append((A, b))
+    g = ax.Group(solvers)
+    reps = args.group_reps
+
+    def own_round():
+        return [s.solve(b) for s, b in zip(solvers, bs)]
+
+    own_round()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        own = own_round()
+    torch.cuda.synchronize()
+    t_own = (time.perf_counter() - t0) / reps
+    g.solve(bs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        xs, rs = g.solve(bs)
+    torch.cuda.synchronize()
+    t_grp = (time.perf_counter() - t0) / reps
+    worst = max(float(np.linalg.norm(b - A @ x.cpu().numpy()) / np.linalg.norm(b)) for (A, b), x in zip(mats, xs))
+    same = all(torch.equal(x.view(torch.int64), o.x.view(torch.int64)) for x, o in zip(xs, own))
+    launches = sum(7 + sum(s.info["level_launches"]) + 2 * sum(s.info["tail_runs"]) for s in solvers)
+    out = {"factors": names, "n": [int(A.shape[0]) for A, _ in mats], "one_after_another_ms": 1e3 * t_own,
+           "group_ms": 1e3 * t_grp, "solves_per_s_one_after_another": len(names) / t_own,
+           "solves_per_s_group": len(names) / t_grp, "speedup": t_own / t_grp,
+           "iterations": [r.iterations for r in rs], "worst_true_residual": worst, "same_bits_as_own_solves": same,
+           "launches_per_iteration_group": g.launches_per_iteration,
+           "launches_per_iteration_own_sum": launches}
+    g.close()
+    for s in solvers:
+        s.close()
+    print(f"[bench] group of {len(names)} small factors: {1e3 * t_grp:.1f} ms vs {1e3 * t_own:.1f} ms one after "
+          f"another ({t_own / t_grp:.2f}x), bits equal: {same}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +374,8 @@ def main() -> int:
     ap.add_argument("--second-workload", default="rmat_22",
                     help="R-MAT configuration timed beside the headline (single mixed / fp64, blocked k; empty: none)")
     ap.add_argument("--second-steps", type=int, default=3)
+    ap.add_argument("--group-reps", type=int, default=5,
+                    help="grouped solves of the serving mix's small factors against their own solves (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serving-qps", type=float, default=25.0)
     ap.add_argument("--serving-slots", type=int, default=32)
@@ -426,6 +505,7 @@ def main() -> int:
     second = None
     if args.second_workload:
         second = run_second_workload(args, ax, mg, np, torch, dev)
+    group = run_group(args, ax, mg, np, torch, dev) if args.group_reps > 0 else None
 
     # The serving mix is reported beside the headline: a failure there is
     # recorded in its section (with the server's reason) and does not void
@@ -499,6 +579,7 @@ def main() -> int:
             "mixed_over_fp64": None if not plain else value / (args.steps / plain["total"]),
             "batched_mixed": batched,
             "second_workload": None if second is None else {args.second_workload: second},
+            "group_small_factors": group,
             "serving": serving,
             "kernels_mixed": per_kernel,
             "sptrsv_sweep_gbps_mixed": sweep_gbps,
