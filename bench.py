@@ -16,8 +16,9 @@ its plain fp64 single-vector rate, and on the 64^3 anisotropic grid
 (weights 1/10/100 x exp(N(0,0.5))); the R-MAT scale-22 Laplacian (edge
 factor 16, weights U[1,10), per-component mean-projected N(0,1) right-hand
 side) is timed in mixed and fp64 single solves and one blocked k = 32 solve;
-the serving mix's factors of at most 300k rows are solved one after another
-and as one grouped solve (many small factors in one launch per kernel kind); the serving mix runs 8 resident factors
+the serving mix's factors of at most 300k rows, and 16 small 2-D grids, are
+solved one after another and as one grouped solve (many small factors in one
+launch per kernel kind); the serving mix runs 8 resident factors
 (2-D grids 1e4..1e6, 3-D grids to 96^3, R-MAT scales 16..20) under Poisson
 arrivals and reports p50 / p99 latency and solves/s.
 
@@ -303,17 +304,24 @@ def run_second_workload(args, ax, mg, np, torch, dev) -> dict:
     return out
 
 
-def run_group(args, ax, mg, np, torch, dev) -> dict:
-    """Many small factors in one launch: the serving mix's factors of at most
-    300k rows (mixed precision), one N(0,1) right-hand side each (projected
-    per component on a Laplacian), solved one after another with their own
-    solves and as one grouped solve (every kernel kind of an iteration one
-    launch over all members); both timed after a warm-up, graph replay."""
+def run_group(args, ax, mg, np, torch, dev, which: str) -> dict:
+    """Many small factors in one launch, mixed precision, one N(0,1)
+    right-hand side each (projected per component on a Laplacian), solved
+    one after another with their own solves and as one grouped solve (every
+    kernel kind of an iteration one launch over all members); both timed
+    after a warm-up, graph replay. which = "serving": the serving mix's
+    factors of at most 300k rows (different sizes and kinds); "grids16": 16
+    2-D 100 x 100 grids with exp(N(0,1)) weights of seeds 1..16."""
+    if which == "serving":
+        # the mix's factors above 300k rows (grid2d_1000, grid3d_96, rmat_20) stay out
+        gen = [(nm, lambda nm=nm: mg.workload(nm)[0]) for nm in mg.SERVING_FACTORS
+               if nm not in ("grid2d_1000", "grid3d_96", "rmat_20")]
+    else:
+        gen = [(f"grid2d_100_w{k}", lambda k=k: mg.grid_laplacian((100, 100), "lognormal", 1.0, seed=k))
+               for k in range(1, 17)]
     names, solvers, bs, mats = [], [], [], []
-    for nm in mg.SERVING_FACTORS:
-        if nm in ("grid2d_1000", "grid3d_96", "rmat_20"):   # the mix's factors above 300k rows
-            continue
-        A, _ = mg.workload(nm)
+    for nm, make in gen:
+        A = make()
         i = len(names)
         names.append(nm)
         F = ax.factorize(A, seed=0)
@@ -353,8 +361,8 @@ def run_group(args, ax, mg, np, torch, dev) -> dict:
     g.close()
     for s in solvers:
         s.close()
-    print(f"[bench] group of {len(names)} small factors: {1e3 * t_grp:.1f} ms vs {1e3 * t_own:.1f} ms one after "
-          f"another ({t_own / t_grp:.2f}x), bits equal: {same}", file=sys.stderr, flush=True)
+    print(f"[bench] group ({which}) of {len(names)} small factors: {1e3 * t_grp:.1f} ms vs {1e3 * t_own:.1f} ms "
+          f"one after another ({t_own / t_grp:.2f}x), bits equal: {same}", file=sys.stderr, flush=True)
     return out
 
 
@@ -505,7 +513,8 @@ def main() -> int:
     second = None
     if args.second_workload:
         second = run_second_workload(args, ax, mg, np, torch, dev)
-    group = run_group(args, ax, mg, np, torch, dev) if args.group_reps > 0 else None
+    group = None if args.group_reps <= 0 else {w: run_group(args, ax, mg, np, torch, dev, w)
+                                               for w in ("serving", "grids16")}
 
     # The serving mix is reported beside the headline: a failure there is
     # recorded in its section (with the server's reason) and does not void
