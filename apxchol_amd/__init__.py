@@ -282,7 +282,9 @@ def _current_device():
 
 
 class Factor:
-    """Host approximate-Cholesky factorization (apxchol::factorize at one thread)."""
+    """Host approximate-Cholesky factorization: apxchol::factorize's one-thread
+    factor (src/factorization.cpp:125-175), bit for bit, computed on
+    ``threads`` host threads."""
 
     def __init__(self, A, *, seed: int = 42):
         lib = load_library()
@@ -295,6 +297,7 @@ class Factor:
         _check(lib.apx_factor_info(h, info.ctypes.data))
         self.n, self.nnz, self.nrounds, self.sddm, self.lumped = (int(info[0]), int(info[1]), int(info[2]),
                                                                    bool(info[3]), int(info[4]))
+        self.threads = int(info[5])  # host threads the factorization ran on
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -172,7 +172,7 @@ apx_status apx_factor_info(apx_factor_t f, int64_t* out) {
         out[2] = static_cast<int64_t>(f->F.rounds.size());
         out[3] = f->F.sddm ? 1 : 0;
         out[4] = f->F.lumped_offdiag;
-        out[5] = 0;
+        out[5] = f->F.threads;
         out[6] = 0;
     });
 }

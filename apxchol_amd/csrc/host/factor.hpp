@@ -65,6 +65,7 @@ struct factorization {
     bool sddm = false;
     std::int64_t lumped_offdiag = 0;
     std::vector<round_stats> rounds;
+    int threads = 1;  // host threads the factorization ran on (the factor does not depend on it)
     off_t64 nnz() const { return outer.empty() ? 0 : outer.back(); }
 };
 

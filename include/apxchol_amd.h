@@ -104,12 +104,14 @@ const char* apx_last_error(void);
 
 /* ---- host factorization -------------------------------------------------
  * apxchol::factorize(L, storage, opts, cp) (include/apxchol/solver/factorization.h:156-159,
- * src/factorization.cpp:125-175) with vec_pool storage and block_greedy, at one
- * thread. A: n x n symmetric Laplacian / SDDM M-matrix in CSC with BOTH
+ * src/factorization.cpp:125-175) with vec_pool storage and block_greedy: the
+ * reference's one-thread factor, bit for bit, whatever number of host threads
+ * runs it (APXCHOL_AMD_FACTOR_THREADS, else the CPUs of this process's
+ * affinity capped by OMP_NUM_THREADS and 16). A: n x n symmetric Laplacian / SDDM M-matrix in CSC with BOTH
  * triangles stored (what Eigen::SparseMatrix<double> holds). */
 apx_status apx_factorize(int64_t n, const int64_t* colptr, const int32_t* rowidx, const double* vals,
                          uint32_t seed, apx_factor_t* out);
-/* out[0..6] = n, nnz, rounds, sddm, lumped_offdiag, 0, 0 */
+/* out[0..6] = n, nnz, rounds, sddm, lumped_offdiag, host threads used, 0 */
 apx_status apx_factor_info(apx_factor_t f, int64_t* out);
 /* factorization::L (CSC, diagonal first; outer n+1, inner/vals nnz), perm (n),
  * per-round IS sizes (rounds) -- factorization.h:22-56. */

@@ -90,7 +90,7 @@ def main() -> int:
     print(json.dumps({"workload": args.workload, "n": int(A.shape[0]), "threads": threads, "threads_from": why,
                       "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
                       "omp_places": os.environ.get("OMP_PLACES", ""),
-                      "setup_factorize_s": t_fact, "setup_prep_s": t_prep, "solve_s": t_solve,
+                      "setup_factorize_s": t_fact, "factor_threads": F.threads, "setup_prep_s": t_prep, "solve_s": t_solve,
                       "solve_times_s": times, "iterations": iters[0], "true_residual": true_res}))
     return 0
 

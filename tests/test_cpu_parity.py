@@ -61,6 +61,7 @@ def test_product_factor_bit_identical_to_reference(name, threads, monkeypatch):
     monkeypatch.setenv("APXCHOL_AMD_FACTOR_PAR_MIN", "1")
     A, g = golden(name)
     F = ax.factorize(A, seed=0)
+    assert F.threads == int(threads)
     outer, inner, vals, perm, ris = F.export()
     assert F.sddm == bool(g["sddm"])
     assert np.array_equal(outer, g["outer"].astype(np.uint64))
