@@ -410,6 +410,13 @@ apx_status apx_server_create(uint64_t memory_budget, int32_t slots, int32_t iter
 
 void apx_server_destroy(apx_server_t srv) { delete srv; }
 
+apx_status apx_server_set_single_lane(apx_server_t srv, int32_t on) {
+    return guarded([&] {
+        need(srv, "server");
+        srv->s->set_single_lane(on != 0);
+    });
+}
+
 apx_status apx_server_add(apx_server_t srv, apx_solver_t solver, int32_t* factor_id) {
     return guarded([&] {
         need(srv, "server");
@@ -454,6 +461,7 @@ apx_status apx_server_get_stats(apx_server_t srv, apx_server_stats* st) {
         st->solves_per_s = s.solves_per_s;
         st->memory_used = s.memory_used;
         st->memory_budget = s.memory_budget;
+        st->single_lane_completed = s.single_lane_completed;
     });
 }
 

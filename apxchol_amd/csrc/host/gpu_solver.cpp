@@ -522,6 +522,9 @@ struct gpu_solver::device_state {
     hipStream_t cap = nullptr;
     hipGraph_t iter_graph = nullptr;
     hipGraphExec_t iter_exec = nullptr;
+    // The single-vector solve in progress (single_begin .. single_collect).
+    bool single_graph = false;
+    int single_enqueued = 0, single_last = 1, single_max_iter = 0;
     // Blocked solves: the block buffers (allocated on first use for the
     // widest block seen) and the sweep tables per block width.
     struct block_tables {
@@ -1161,12 +1164,26 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     const auto t_solve = std::chrono::steady_clock::now();
     if (tol < 0.0) tol = opts_.tol;
     if (max_iter < 0) max_iter = opts_.max_iter;
-    auto& d = *dev_;
-    const int n = static_cast<int>(n_);
     solve_result res;
     res.lumped_offdiag = F_.lumped_offdiag;
     res.timings = setup_times_;
-    if (n == 0) return res;
+    if (n_ == 0) return res;
+    single_begin(d_b, d_x0, tol, max_iter, s);
+    const int J = std::max(1, opts_.iters_per_check);
+    session_slot state;
+    do single_step(J, s);
+    while (!single_poll(state, s));
+    single_collect(d_x, res, s);
+    res.timings.pcg_ms = ms_since(t_solve);
+    std::size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && tot >= fr) res.solve_vram_mb = static_cast<double>(tot - fr) / (1024.0 * 1024.0);
+    return res;
+}
+
+void gpu_solver::single_begin(const double* d_b, const double* d_x0, double tol, int max_iter, hipStream_t s) const {
+    auto& d = *dev_;
+    const int n = static_cast<int>(n_);
+    if (n == 0) throw std::invalid_argument("gpu_solver: an empty system has no solve steps");
     pcg_state& h = *d.host_st;
     std::memset(&h, 0, sizeof h);
     h.tol = tol;
@@ -1188,14 +1205,12 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
     } else {
         launch_init_residual(n, d.r.as<double>(), nullptr, st, part, s);
     }
-    timer& tm = *timer_;
     enqueue_precondition(true, 0, s);
-    const int J = std::max(1, opts_.iters_per_check);
     // Graph replay of the iteration (every argument of its kernels is a
     // pointer into this solver's device state or a size, so one capture
     // serves every solve); eager launches while timing.
-    const bool graph = opts_.graph && !tm.on;
-    if (graph && !d.iter_exec) {
+    d.single_graph = opts_.graph && !timer_->on;
+    if (d.single_graph && !d.iter_exec) {
         std::lock_guard<std::mutex> capture_lock(g_capture_mu);
         if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -1210,31 +1225,55 @@ solve_result gpu_solver::solve_device(const double* d_b, double* d_x, double tol
         hip_check(hipStreamEndCapture(d.cap, &d.iter_graph), "hipStreamEndCapture");
         hip_check(hipGraphInstantiate(&d.iter_exec, d.iter_graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
-    int enqueued = 0;
-    for (;;) {
-        for (int j = 0; j < J; ++j) {
-            if (graph) hip_check(hipGraphLaunch(d.iter_exec, s), "hipGraphLaunch");
-            else enqueue_iteration(enqueued + j + 1, s);
-        }
-        enqueued += J;
-        hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
-        hip_check(hipStreamSynchronize(s), "solve");
-        if (h.done || enqueued > max_iter + J) break;
+    d.single_enqueued = 0;
+    d.single_last = 1;
+    d.single_max_iter = max_iter;
+}
+
+void gpu_solver::single_step(int iterations, hipStream_t s) const {
+    auto& d = *dev_;
+    for (int j = 0; j < iterations; ++j) {
+        if (d.single_graph) hip_check(hipGraphLaunch(d.iter_exec, s), "hipGraphLaunch");
+        else enqueue_iteration(d.single_enqueued + j + 1, s);
     }
+    d.single_enqueued += iterations;
+    d.single_last = std::max(1, iterations);
+}
+
+bool gpu_solver::single_poll(session_slot& out, hipStream_t s) const {
+    auto& d = *dev_;
+    pcg_state& h = *d.host_st;
+    hip_check(hipMemcpyAsync(&h, d.st.get(), sizeof h, hipMemcpyDeviceToHost, s), "state read");
+    hip_check(hipStreamSynchronize(s), "solve");
+    out.done = h.done != 0;
+    out.status = h.status;
+    out.iterations = h.iter;
+    out.residual = h.residual;
+    // The enqueued iterations run out once max_iter is passed (the device
+    // stops at max_iter; this bounds the host loop whatever the state says).
+    return out.done || d.single_enqueued > d.single_max_iter + d.single_last;
+}
+
+void gpu_solver::single_collect(double* d_x, solve_result& res, hipStream_t s) const {
+    auto& d = *dev_;
+    const int n = static_cast<int>(n_);
+    pcg_state* st = d.st.as<pcg_state>();
+    double* part = d.part.as<double>();
+    pcg_state& h = *d.host_st;
     if (!F_.sddm) launch_sum_vec(n, d.x.as<double>(), st, part, s);
     launch_gather_shift(n, d.perm.as<unsigned>(), d.x.as<double>(), d_x, st, !F_.sddm, s);
     hip_check(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s), "state read");
     hip_check(hipStreamSynchronize(s), "solve");
-    if (h.status == 5) hip_check(hipMemsetAsync(d_x, 0, sizeof(double) * n, s), "zero x");
-    tm.collect(h.iter - 1);
+    if (h.status == 5) {
+        hip_check(hipMemsetAsync(d_x, 0, sizeof(double) * n, s), "zero x");
+        hip_check(hipStreamSynchronize(s), "solve");
+    }
+    timer_->collect(h.iter - 1);
     check_sweep_error();
+    res.lumped_offdiag = F_.lumped_offdiag;
     res.iterations = h.iter;
     res.residual = h.residual;
     res.status = h.status;
-    res.timings.pcg_ms = ms_since(t_solve);
-    std::size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) == hipSuccess && tot >= fr) res.solve_vram_mb = static_cast<double>(tot - fr) / (1024.0 * 1024.0);
-    return res;
 }
 
 // ---- blocked solves -----------------------------------------------------

@@ -156,6 +156,16 @@ public:
     void session_step(int iterations, hipStream_t s) const;
     void session_read(std::vector<session_slot>& out, hipStream_t s) const;
     void session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s) const;
+    // A single-vector solve in steps (a serving loop's single lane runs one
+    // beside its block): solve_device is single_begin, single_step until
+    // single_poll says done, then single_collect, so a request run this way
+    // gets the bits of solve_device with the same tol and max_iter. It uses
+    // the solver's single-vector buffers and state, which a session leaves
+    // alone; its steps may run on another stream than the session's.
+    void single_begin(const double* d_b, const double* d_x0, double tol, int max_iter, hipStream_t s) const;
+    void single_step(int iterations, hipStream_t s) const;
+    bool single_poll(session_slot& out, hipStream_t s) const;
+    void single_collect(double* d_x, solve_result& res, hipStream_t s) const;
 
     // Autotuner: measures on this device, for this factor and operator, the
     // SpMV lanes per row (powers of two; the single SpMV and the blocked

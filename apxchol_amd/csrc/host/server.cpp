@@ -37,8 +37,12 @@ solve_server::~solve_server() {
         if (l->worker.joinable()) l->worker.join();
     for (auto& l : lanes_) {
         if (l->stream) (void)hipStreamSynchronize(l->stream);
+        if (l->single_stream) (void)hipStreamSynchronize(l->single_stream);
         if (l->d_b) (void)hipFree(l->d_b);
         if (l->d_x) (void)hipFree(l->d_x);
+        if (l->d_sb) (void)hipFree(l->d_sb);
+        if (l->d_sx) (void)hipFree(l->d_sx);
+        if (l->single_stream) (void)hipStreamDestroy(l->single_stream);
         if (l->h_stage) (void)hipHostFree(l->h_stage);
         if (l->stream) (void)hipStreamDestroy(l->stream);
         if (l->solver) l->solver->release_serving();
@@ -60,9 +64,11 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
     int kp = 1;
     while (kp < want) kp <<= 1;
     const std::int64_t n = solver->rows();
-    // Solver arrays + the block (r, x, p, Ap, y, z) + the b / x staging.
+    // Solver arrays + the block (r, x, p, Ap, y, z) + the b / x staging of
+    // the block and of the single lane.
     const std::uint64_t need = solver->stats().device_bytes +
-                               static_cast<std::uint64_t>(8) * static_cast<std::uint64_t>(n) * kp * sizeof(double);
+                               static_cast<std::uint64_t>(8) * static_cast<std::uint64_t>(n) * kp * sizeof(double) +
+                               static_cast<std::uint64_t>(2) * static_cast<std::uint64_t>(n) * sizeof(double);
     auto ln = std::make_unique<lane>();
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -80,6 +86,10 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
     const std::size_t vb = static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * kp * sizeof(double);
     check(hipMalloc(&ln->d_b, vb), "hipMalloc");
     check(hipMalloc(&ln->d_x, vb), "hipMalloc");
+    check(hipStreamCreateWithFlags(&ln->single_stream, hipStreamNonBlocking), "hipStreamCreate");
+    const std::size_t v1 = static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * sizeof(double);
+    check(hipMalloc(&ln->d_sb, v1), "hipMalloc");
+    check(hipMalloc(&ln->d_sx, v1), "hipMalloc");
     // Pinned staging of one vector: every host <-> device copy of the lane is
     // a DMA from / to it, never a copy from pageable memory (which the
     // runtime stages internally and may serialize behind other streams).
@@ -142,6 +152,14 @@ bool solve_server::wait(std::int64_t id, double* x, solve_result* res, double ti
     return true;
 }
 
+void solve_server::set_single_lane(bool on) {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        opts_.single_lane = on;
+    }
+    work_cv_.notify_all();
+}
+
 server_stats solve_server::stats() const {
     std::lock_guard<std::mutex> lk(mu_);
     server_stats s;
@@ -150,6 +168,7 @@ server_stats solve_server::stats() const {
     s.factors = static_cast<std::int64_t>(lanes_.size());
     s.memory_used = used_;
     s.memory_budget = opts_.memory_budget;
+    s.single_lane_completed = single_completed_;
     if (!latencies_ms_.empty()) {
         std::vector<double> v = latencies_ms_;
         std::sort(v.begin(), v.end());
@@ -168,24 +187,66 @@ server_stats solve_server::stats() const {
     return s;
 }
 
+// Marks requests done (their x and result filled) and wakes the waiters.
+void solve_server::finish(lane& ln, const std::vector<std::shared_ptr<request>>& done) {
+    std::lock_guard<std::mutex> lk(mu_);
+    const auto now = std::chrono::steady_clock::now();
+    for (const auto& rq : done) {
+        rq->t_done = now;
+        rq->done = true;
+        latencies_ms_.push_back(std::chrono::duration<double, std::milli>(now - rq->t_submit).count());
+        last_done_ = now;
+        for (auto& c : ln.slot)
+            if (c == rq) c = nullptr;
+        if (ln.single == rq) {
+            ln.single = nullptr;
+            ++single_completed_;
+        }
+    }
+    done_cv_.notify_all();
+}
+
 void solve_server::run(lane& ln) {
     (void)hipSetDevice(device_);
     const std::size_t n = static_cast<std::size_t>(ln.n);
     double* db = static_cast<double*>(ln.d_b);
     double* dx = static_cast<double*>(ln.d_x);
+    double* stage = static_cast<double*>(ln.h_stage);
+    const int J = opts_.iters_per_check;
     std::vector<gpu_solver::session_slot> states;
+    // Host <-> device copies of one vector through the pinned staging vector
+    // (reused, so each copy completes before the next).
+    auto upload = [&](const std::vector<double>& b, double* dst, hipStream_t s) {
+        std::copy(b.begin(), b.end(), stage);
+        check(hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyHostToDevice, s), "upload b");
+        check(hipStreamSynchronize(s), "upload b");
+    };
+    auto download = [&](const double* src, std::vector<double>& x, hipStream_t s) {
+        check(hipMemcpyAsync(stage, src, n * sizeof(double), hipMemcpyDeviceToHost, s), "download x");
+        check(hipStreamSynchronize(s), "download x");
+        x.assign(stage, stage + n);
+    };
     for (;;) {
         std::vector<std::pair<int, std::shared_ptr<request>>> admit;
+        std::shared_ptr<request> single_new, single;
         bool busy = false;
         {
             std::unique_lock<std::mutex> lk(mu_);
             work_cv_.wait(lk, [&] {
-                if (stop_ || !ln.queue.empty()) return true;
+                if (stop_ || !ln.queue.empty() || ln.single) return true;
                 for (auto& r : ln.slot)
                     if (r) return true;
                 return false;
             });
             if (stop_) return;
+            // The single lane takes the oldest waiting request when it is free;
+            // the rest join free columns of the block.
+            if (opts_.single_lane && !ln.single && !ln.queue.empty() && ln.n > 0) {
+                ln.single = ln.queue.front();
+                ln.queue.pop_front();
+                single_new = ln.single;
+            }
+            single = ln.single;
             for (int c = 0; c < ln.kp && !ln.queue.empty(); ++c)
                 if (!ln.slot[c]) {
                     ln.slot[c] = ln.queue.front();
@@ -196,62 +257,66 @@ void solve_server::run(lane& ln) {
                 if (r) busy = true;
         }
         try {
+            if (single_new) {
+                upload(single_new->b, static_cast<double*>(ln.d_sb), ln.single_stream);
+                ln.solver->single_begin(static_cast<const double*>(ln.d_sb), nullptr, single_new->tol,
+                                        single_new->max_iter, ln.single_stream);
+            }
             if (!admit.empty()) {
                 std::vector<gpu_solver::session_column> cols;
-                double* stage = static_cast<double*>(ln.h_stage);
                 for (auto& [c, rq] : admit) {
                     double* dst = db + static_cast<std::size_t>(c) * n;
-                    std::copy(rq->b.begin(), rq->b.end(), stage);
-                    check(hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyHostToDevice, ln.stream), "upload b");
-                    check(hipStreamSynchronize(ln.stream), "upload b");   // the staging vector is reused
+                    upload(rq->b, dst, ln.stream);
                     cols.push_back({c, dst, rq->tol, rq->max_iter});
                 }
                 ln.solver->session_admit(cols, ln.stream);
             }
-            if (!busy) continue;
-            ln.solver->session_step(opts_.iters_per_check, ln.stream);
-            ln.solver->session_read(states, ln.stream);
-            std::vector<int> fin;
-            std::vector<double*> dst;
-            {
-                std::lock_guard<std::mutex> lk(mu_);
-                for (int c = 0; c < ln.kp; ++c)
-                    if (ln.slot[c] && states[c].done) {
-                        fin.push_back(c);
-                        dst.push_back(dx + static_cast<std::size_t>(c) * n);
-                    }
-            }
-            if (fin.empty()) continue;
-            ln.solver->session_collect(fin, dst, ln.stream);
+            // One iteration group of each, on their own streams, so the device
+            // runs them side by side; then each is read back.
+            if (single) ln.solver->single_step(J, ln.single_stream);
+            if (busy) ln.solver->session_step(J, ln.stream);
             std::vector<std::shared_ptr<request>> finished;
-            for (std::size_t j = 0; j < fin.size(); ++j) {
-                auto rq = ln.slot[fin[j]];
-                double* stage = static_cast<double*>(ln.h_stage);
-                check(hipMemcpyAsync(stage, dst[j], n * sizeof(double), hipMemcpyDeviceToHost, ln.stream),
-                      "download x");
-                check(hipStreamSynchronize(ln.stream), "download x");
-                rq->x.assign(stage, stage + n);
-                const auto& st = states[fin[j]];
-                rq->res.iterations = st.iterations;
-                rq->res.residual = st.residual;
-                rq->res.status = st.status;
-                rq->res.lumped_offdiag = ln.solver->factor().lumped_offdiag;
-                if (st.status == 5) std::fill(rq->x.begin(), rq->x.end(), 0.0);
-                finished.push_back(rq);
-            }
-            {
-                std::lock_guard<std::mutex> lk(mu_);
-                const auto now = std::chrono::steady_clock::now();
-                for (std::size_t j = 0; j < fin.size(); ++j) {
-                    auto& rq = finished[j];
-                    rq->t_done = now;
-                    rq->done = true;
-                    latencies_ms_.push_back(std::chrono::duration<double, std::milli>(now - rq->t_submit).count());
-                    last_done_ = now;
-                    ln.slot[fin[j]] = nullptr;
+            if (single) {
+                gpu_solver::session_slot st1;
+                if (ln.solver->single_poll(st1, ln.single_stream)) {
+                    solve_result res;
+                    ln.solver->single_collect(static_cast<double*>(ln.d_sx), res, ln.single_stream);
+                    download(static_cast<const double*>(ln.d_sx), single->x, ln.single_stream);
+                    single->res.iterations = res.iterations;
+                    single->res.residual = res.residual;
+                    single->res.status = res.status;
+                    single->res.lumped_offdiag = res.lumped_offdiag;
+                    finished.push_back(single);
                 }
             }
-            done_cv_.notify_all();
+            if (busy) {
+                ln.solver->session_read(states, ln.stream);
+                std::vector<int> fin;
+                std::vector<double*> dst;
+                {
+                    std::lock_guard<std::mutex> lk(mu_);
+                    for (int c = 0; c < ln.kp; ++c)
+                        if (ln.slot[c] && states[c].done) {
+                            fin.push_back(c);
+                            dst.push_back(dx + static_cast<std::size_t>(c) * n);
+                        }
+                }
+                if (!fin.empty()) {
+                    ln.solver->session_collect(fin, dst, ln.stream);
+                    for (std::size_t j = 0; j < fin.size(); ++j) {
+                        auto rq = ln.slot[fin[j]];
+                        download(dst[j], rq->x, ln.stream);
+                        const auto& st = states[fin[j]];
+                        rq->res.iterations = st.iterations;
+                        rq->res.residual = st.residual;
+                        rq->res.status = st.status;
+                        rq->res.lumped_offdiag = ln.solver->factor().lumped_offdiag;
+                        if (st.status == 5) std::fill(rq->x.begin(), rq->x.end(), 0.0);
+                        finished.push_back(rq);
+                    }
+                }
+            }
+            if (!finished.empty()) finish(ln, finished);
         } catch (const std::exception& e) {
             // A device failure ends this lane: its requests fail (status -1)
             // and later submissions to it are refused with the reason.
@@ -265,6 +330,7 @@ void solve_server::run(lane& ln) {
                 rq = nullptr;
             };
             for (auto& r : ln.slot) fail(r);
+            fail(ln.single);
             for (auto& r : ln.queue) fail(r);
             ln.queue.clear();
             done_cv_.notify_all();

@@ -8,8 +8,17 @@
 // iteration groups (continuous batching): a request that arrives while others
 // are iterating joins them at the next group boundary, and a finished column
 // is released and refilled without stopping the others. A column's
-// arithmetic does not depend on its neighbours or its slot, so a request's
-// result is the same bits alone or in a full block.
+// arithmetic does not depend on its neighbours or its slot, so a request that
+// runs in the block gets the same bits alone or in a full block.
+//
+// Single lane (on by default): beside its block, each factor runs one request
+// at a time as a single-vector solve on a second stream. The oldest waiting
+// request takes it whenever it is free, so at light load a request costs a
+// single-vector iteration instead of a whole block's; its result is the bits
+// of the solver's own solve with that tol and max_iter. The block and the
+// single lane iterate side by side, in the same iteration groups. With the
+// single lane on, a request's bits are therefore those of its own solve or of
+// its block column, whichever took it; both meet its tolerance.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,6 +42,7 @@ struct server_options {
     int slots = 32;                    // columns per factor block (power of two <= 64 after rounding;
                                        // 0: each factor's autotuned block width)
     int iters_per_check = 2;           // PCG iterations between admissions / completions
+    bool single_lane = true;           // one request per factor at a time as a single-vector solve
 };
 
 struct server_stats {
@@ -40,6 +50,7 @@ struct server_stats {
     double p50_ms = 0.0, p99_ms = 0.0, mean_ms = 0.0;
     double solves_per_s = 0.0;          // completed / (last completion - first submission)
     std::uint64_t memory_used = 0, memory_budget = 0;
+    std::int64_t single_lane_completed = 0;
 };
 
 class solve_server {
@@ -59,6 +70,8 @@ public:
     // and its result, and forgets the request. False on timeout.
     bool wait(std::int64_t id, double* x, solve_result* res, double timeout_s);
     server_stats stats() const;
+    // Turns the single lanes on or off (requests already on one finish there).
+    void set_single_lane(bool on);
 
 private:
     struct request {
@@ -77,14 +90,19 @@ private:
         std::thread worker;
         std::deque<std::shared_ptr<request>> queue;
         std::vector<std::shared_ptr<request>> slot;   // request in each column
+        std::shared_ptr<request> single;              // request on the single lane
+        hipStream_t single_stream = nullptr;
         void* d_b = nullptr;                          // slots x n staging (device)
         void* d_x = nullptr;
+        void* d_sb = nullptr;                         // single lane's b and x (device, n doubles each)
+        void* d_sx = nullptr;
         void* h_stage = nullptr;                      // n doubles, pinned host memory
         int kp = 0;
         std::int64_t n = 0;
         std::string error;                            // set when a device failure ended the lane
     };
     void run(lane& ln);
+    void finish(lane& ln, const std::vector<std::shared_ptr<request>>& done);
 
     server_options opts_;
     mutable std::mutex mu_;
@@ -96,6 +114,7 @@ private:
     std::int64_t next_id_ = 1;
     std::uint64_t used_ = 0;
     std::int64_t submitted_ = 0;
+    std::int64_t single_completed_ = 0;
     std::vector<double> latencies_ms_;
     std::chrono::steady_clock::time_point first_submit_, last_done_;
     bool any_submit_ = false;

@@ -84,19 +84,16 @@ def percentile(xs, q):
 def run_serving(args, ax, mg, np) -> dict:
     """The serving mix: 8 resident factors, Poisson arrivals at a fixed rate,
     one N(0,1) right-hand side per request with tol drawn from {1e-6, 1e-8,
-    1e-10}; mixed precision, continuous batching. Every result is graded by
-    its true residual."""
+    1e-10}; mixed precision, continuous batching with the single lanes on
+    (and, for comparison, off). Every result is graded by its true residual."""
     import threading
     t0 = time.perf_counter()
     mats, solvers = [], []
-    srv = ax.Server(slots=args.serving_slots, iters_per_check=2)
-    fids = []
     for name in mg.SERVING_FACTORS:
         L, _ = mg.workload(name)
         F = ax.factorize(L, seed=0)
         S = ax.Solver(L, factor=F, precision="fp32")
         mats.append((name, L, F.sddm))
-        fids.append(srv.add(S))
         solvers.append(S)
         print(f"[bench] serving: factor {name} (n={L.shape[0]}) resident after {time.perf_counter() - t0:.1f}s",
               file=sys.stderr, flush=True)
@@ -109,55 +106,68 @@ def run_serving(args, ax, mg, np) -> dict:
         f = int(rng.integers(len(mats)))
         plan.append((f, float(tols[int(rng.integers(3))]), float(rng.exponential(1.0 / args.serving_qps))))
     rhs = [mg.request_rhs(mats[f][1], mats[f][2], 1000 + i) for i, (f, _, _) in enumerate(plan)]
-    ids = [None] * len(plan)
 
-    errors = []
+    def offered(single_lane: bool) -> dict:
+        """The Poisson arrivals into a server over the resident solvers (a
+        solver serves one server at a time, so each pass has its own)."""
+        label = "serving" if single_lane else "serving, blocks only"
+        srv = ax.Server(slots=args.serving_slots, iters_per_check=2, single_lane=single_lane)
+        fids = [srv.add(S) for S in solvers]
+        ids = [None] * len(plan)
+        errors = []
 
-    def produce():
-        try:
-            for i, (f, tol, gap) in enumerate(plan):
-                time.sleep(gap)
-                ids[i] = srv.submit(fids[f], rhs[i], tol)
-        except Exception as e:   # a refused submission ends the run with its reason
-            errors.append(e)
+        def produce():
+            try:
+                for i, (f, tol, gap) in enumerate(plan):
+                    time.sleep(gap)
+                    ids[i] = srv.submit(fids[f], rhs[i], tol)
+            except Exception as e:   # a refused submission ends the run with its reason
+                errors.append(e)
 
-    th = threading.Thread(target=produce)
-    t1 = time.perf_counter()
-    th.start()
-    th.join()
-    if errors:
-        raise errors[0]
-    print(f"[bench] serving: {len(plan)} requests submitted", file=sys.stderr, flush=True)
-    worst = 0.0
-    for i, (f, tol, _) in enumerate(plan):
-        # A request the server has not finished within 120 s ends the
-        # measurement with that reason (the serving section records it).
-        try:
-            x, r = srv.wait(ids[i], timeout=120)
-        except TimeoutError:
-            st = srv.stats()
+        th = threading.Thread(target=produce)
+        t1 = time.perf_counter()
+        th.start()
+        th.join()
+        if errors:
             srv.close()
-            raise RuntimeError(f"request {i} (factor {mats[f][0]}) not done after 120 s; "
-                               f"{st['completed']} of {st['submitted']} completed")
-        L = mats[f][1]
-        tr = float(np.linalg.norm(rhs[i] - L @ x) / np.linalg.norm(rhs[i]))
-        worst = max(worst, tr / tol)
-        if (i + 1) % 64 == 0:
-            print(f"[bench] serving: {i + 1} of {len(plan)} results graded ({time.perf_counter() - t1:.1f}s)",
-                  file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t1
-    st = srv.stats()
-    srv.close()
-    print(f"[bench] serving: {st['solves_per_s']:.1f} solves/s p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms",
-          file=sys.stderr, flush=True)
+            raise errors[0]
+        print(f"[bench] {label}: {len(plan)} requests submitted", file=sys.stderr, flush=True)
+        worst = 0.0
+        for i, (f, tol, _) in enumerate(plan):
+            # A request the server has not finished within 120 s ends the
+            # measurement with that reason (the serving section records it).
+            try:
+                x, r = srv.wait(ids[i], timeout=120)
+            except TimeoutError:
+                st = srv.stats()
+                srv.close()
+                raise RuntimeError(f"request {i} (factor {mats[f][0]}) not done after 120 s; "
+                                   f"{st['completed']} of {st['submitted']} completed")
+            L = mats[f][1]
+            tr = float(np.linalg.norm(rhs[i] - L @ x) / np.linalg.norm(rhs[i]))
+            worst = max(worst, tr / tol)
+            if (i + 1) % 64 == 0:
+                print(f"[bench] {label}: {i + 1} of {len(plan)} results graded ({time.perf_counter() - t1:.1f}s)",
+                      file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t1
+        st = srv.stats()
+        srv.close()
+        print(f"[bench] {label}: {st['solves_per_s']:.1f} solves/s p50 {st['p50_ms']:.1f} ms "
+              f"p99 {st['p99_ms']:.1f} ms ({st['single_lane_completed']} on single lanes)", file=sys.stderr,
+              flush=True)
+        return {"solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                "mean_ms": st["mean_ms"], "wall_s": wall, "worst_true_residual_over_tol": worst,
+                "memory_used_bytes": st["memory_used"], "single_lane_completed": st["single_lane_completed"]}
+
+    main = offered(True)
     out = {"factors": [m[0] for m in mats], "requests": len(plan), "qps_offered": args.serving_qps,
-           "slots": args.serving_slots, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
-           "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "wall_s": wall, "setup_s": t_setup,
-           "worst_true_residual_over_tol": worst, "memory_used_bytes": st["memory_used"]}
+           "slots": args.serving_slots, "single_lane": True, **main, "setup_s": t_setup}
+    if args.serving_blocks_only:
+        out["blocks_only"] = offered(False)
     # Capacity: the same requests submitted at once to a fresh server over the
     # same resident solvers (the first server released them), so the rate is
     # what the device sustains rather than the offered load.
-    cap = ax.Server(slots=args.serving_slots, iters_per_check=2)
+    cap = ax.Server(slots=args.serving_slots, iters_per_check=2, single_lane=True)
     cfids = [cap.add(S) for S in solvers]
     t2 = time.perf_counter()
     cids = [cap.submit(cfids[f], rhs[i], tol) for i, (f, tol, _) in enumerate(plan)]
@@ -170,7 +180,8 @@ def run_serving(args, ax, mg, np) -> dict:
     cap.close()
     out["capacity"] = {"requests": len(plan), "submitted_at_once": True, "solves_per_s": cst["solves_per_s"],
                        "p50_ms": cst["p50_ms"], "p99_ms": cst["p99_ms"], "wall_s": time.perf_counter() - t2,
-                       "worst_true_residual_over_tol": cworst}
+                       "worst_true_residual_over_tol": cworst,
+                       "single_lane_completed": cst["single_lane_completed"]}
     print(f"[bench] serving capacity (all {len(plan)} requests at once): {cst['solves_per_s']:.1f} solves/s "
           f"p50 {cst['p50_ms']:.1f} ms p99 {cst['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     return out
@@ -387,6 +398,8 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serving-qps", type=float, default=25.0)
     ap.add_argument("--serving-slots", type=int, default=32)
+    ap.add_argument("--serving-blocks-only", type=int, default=1,
+                    help="also run the arrivals with the single lanes off (blocks only), for comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibration of the HBM byte counters)")
     args = ap.parse_args()

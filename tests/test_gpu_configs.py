@@ -102,7 +102,8 @@ def test_serving_eight_factors(gpu):
     """The serving mix's 8 resident factors: 48 requests with Poisson
     arrivals and tolerances from {1e-6, 1e-8, 1e-10}; each result meets its
     tolerance, and one request per factor is replayed alone in a fresh
-    server (same block width) with the same bits."""
+    server (same block width) with the same bits (the first request of a
+    factor finds its single lane free, in the mix and alone)."""
     print("[case] serving with the 8 configured factors", flush=True)
     mats = {}
     for name in mg.SERVING_FACTORS:
