@@ -76,6 +76,7 @@ _SIGS = {
     "apx_solver_autotune": ([P, I32, P, P], ctypes.c_int),
     "apx_server_destroy": ([P], None),
     "apx_server_set_single_lane": ([P, I32], ctypes.c_int),
+    "apx_server_set_narrow_block": ([P, I32], ctypes.c_int),
     "apx_server_add": ([P, P, P], ctypes.c_int),
     "apx_server_submit": ([P, I32, P, F64, I32, P], ctypes.c_int),
     "apx_server_wait": ([P, I64, P, P, F64, P], ctypes.c_int),
@@ -561,21 +562,24 @@ class ServerStats(ctypes.Structure):
 
 class Server:
     """Serving loop over resident factors (continuous batching): every added
-    Solver keeps one blocked PCG of `slots` columns running on its own stream;
-    submitted right-hand sides join free columns between iteration groups and
-    leave when they finish. With ``single_lane`` (the default) each factor also
+    Solver keeps one blocked PCG of `slots` columns running on its own stream
+    (and a narrow one of ``narrow_slots`` columns, filled first); submitted
+    right-hand sides join free columns between iteration groups and leave
+    when they finish. With ``single_lane`` (the default) each factor also
     runs one request at a time as a single-vector solve beside its block (the
     oldest waiting request takes it when it is free; its result is the bits of
     Solver.solve with the same tol and maxiter). A solver added here is used
     by the server only."""
 
-    def __init__(self, memory_budget: int = 0, slots: int = 32, iters_per_check: int = 2, single_lane: bool = True):
+    def __init__(self, memory_budget: int = 0, slots: int = 32, iters_per_check: int = 2, single_lane: bool = True,
+                 narrow_slots: int = 4):
         lib = load_library()
         h = P()
         _check(lib.apx_server_create(int(memory_budget), int(slots), int(iters_per_check), ctypes.byref(h)))
         self._h = h
         self._n = {}
         _check(lib.apx_server_set_single_lane(h, int(bool(single_lane))))
+        _check(lib.apx_server_set_narrow_block(h, int(narrow_slots)))
 
     def add(self, solver: "Solver") -> int:
         fid = I32()

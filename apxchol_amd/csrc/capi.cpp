@@ -410,6 +410,13 @@ apx_status apx_server_create(uint64_t memory_budget, int32_t slots, int32_t iter
 
 void apx_server_destroy(apx_server_t srv) { delete srv; }
 
+apx_status apx_server_set_narrow_block(apx_server_t srv, int32_t slots) {
+    return guarded([&] {
+        need(srv, "server");
+        srv->s->set_small_slots(slots);
+    });
+}
+
 apx_status apx_server_set_single_lane(apx_server_t srv, int32_t on) {
     return guarded([&] {
         need(srv, "server");

@@ -559,7 +559,7 @@ struct gpu_solver::device_state {
             if (host_bs) (void)hipHostFree(host_bs);
             if (host_adm) (void)hipHostFree(host_adm);
         }
-    } blk;
+    } blk, blk2;   // blk: batches and a session's main block; blk2: a session's second block
     ~device_state() {
         if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
         if (iter_graph) (void)hipGraphDestroy(iter_graph);
@@ -568,6 +568,15 @@ struct gpu_solver::device_state {
         if (host_err) (void)hipHostFree(host_err);
     }
 };
+
+namespace {
+// Block buffers `which` (0 or 1) of a solver's device state.
+template <class D>
+auto& block_in(D& d, int which) {
+    if (which != 0 && which != 1) throw std::invalid_argument("gpu_solver: no such block");
+    return which == 0 ? d.blk : d.blk2;
+}
+}  // namespace
 
 // HIP-event timing of kernel classes. Events are pooled and reused; each
 // record carries the PCG step it belongs to, so launches after convergence
@@ -1206,28 +1215,32 @@ void gpu_solver::single_begin(const double* d_b, const double* d_x0, double tol,
         launch_init_residual(n, d.r.as<double>(), nullptr, st, part, s);
     }
     enqueue_precondition(true, 0, s);
-    // Graph replay of the iteration (every argument of its kernels is a
-    // pointer into this solver's device state or a size, so one capture
-    // serves every solve); eager launches while timing.
+    // Graph replay of the iteration; eager launches while timing.
     d.single_graph = opts_.graph && !timer_->on;
-    if (d.single_graph && !d.iter_exec) {
-        std::lock_guard<std::mutex> capture_lock(g_capture_mu);
-        if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
-        hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        try {
-            enqueue_iteration(0, d.cap);
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(d.cap, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
-        }
-        hip_check(hipStreamEndCapture(d.cap, &d.iter_graph), "hipStreamEndCapture");
-        hip_check(hipGraphInstantiate(&d.iter_exec, d.iter_graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    }
+    if (d.single_graph) single_prepare();
     d.single_enqueued = 0;
     d.single_last = 1;
     d.single_max_iter = max_iter;
+}
+
+void gpu_solver::single_prepare() const {
+    auto& d = *dev_;
+    if (!opts_.graph || d.iter_exec || n_ == 0) return;
+    // Every argument of the iteration's kernels is a pointer into this
+    // solver's device state or a size, so one capture serves every solve.
+    std::lock_guard<std::mutex> capture_lock(g_capture_mu);
+    if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        enqueue_iteration(0, d.cap);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(d.cap, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(d.cap, &d.iter_graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&d.iter_exec, d.iter_graph, nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
 void gpu_solver::single_step(int iterations, hipStream_t s) const {
@@ -1280,8 +1293,8 @@ void gpu_solver::single_collect(double* d_x, solve_result& res, hipStream_t s) c
 
 // Block width for k columns (a power of two <= 64), with the block buffers
 // grown to it.
-int gpu_solver::prepare_block(int k) const {
-    auto& b = dev_->blk;
+int gpu_solver::prepare_block(int k, int which) const {
+    auto& b = block_in(*dev_, which);
     int kp = 1;
     while (kp < k) kp <<= 1;
     if (kp > 64) throw std::invalid_argument("gpu_solver: a block holds at most 64 columns");
@@ -1383,8 +1396,8 @@ void gpu_solver::apply_batch_device(int k, const double* d_R, double* d_Z, hipSt
 
 // The blocked sweep tables of block width kp (built, checked and uploaded on
 // first use).
-const block_sched* gpu_solver::block_scheds(int kp) const {
-    auto& b = dev_->blk;
+const block_sched* gpu_solver::block_scheds(int kp, int which) const {
+    auto& b = block_in(*dev_, which);
     auto it = b.tables.find(kp);
     if (it == b.tables.end()) {
         auto t = std::make_unique<device_state::block_tables>();
@@ -1491,13 +1504,13 @@ const block_sched* gpu_solver::block_scheds(int kp) const {
     return &it->second->f;
 }
 
-void gpu_solver::enqueue_precondition_k(bool first, int kp, hipStream_t s) const {
+void gpu_solver::enqueue_precondition_k(bool first, int kp, hipStream_t s, int which) const {
     auto& d = *dev_;
-    auto& b = d.blk;
+    auto& b = block_in(*dev_, which);
     const int n = static_cast<int>(n_);
     pcg_state* st = b.st.as<pcg_state>();
     batch_state* bs = b.bs.as<batch_state>();
-    block_scheds(kp);
+    block_scheds(kp, which);
     const auto& t = *b.tables.at(kp);
     launch_sweep_k(d.fwd, d.prec, d.fp16, t.f, b.r.get(), b.y.get(), st, bs, s);
     launch_sweep_k(d.bck, d.prec, d.fp16, t.b, b.y.get(), b.z.get(), st, bs, s);
@@ -1505,9 +1518,9 @@ void gpu_solver::enqueue_precondition_k(bool first, int kp, hipStream_t s) const
     launch_update_p_k(d.prec, n, kp, b.p.as<double>(), b.z.get(), st, bs, first, s);
 }
 
-void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s) const {
+void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s, int which) const {
     auto& d = *dev_;
-    auto& b = d.blk;
+    auto& b = block_in(*dev_, which);
     const int n = static_cast<int>(n_);
     pcg_state* st = b.st.as<pcg_state>();
     batch_state* bs = b.bs.as<batch_state>();
@@ -1519,16 +1532,16 @@ void gpu_solver::enqueue_iteration_k(int kp, hipStream_t s) const {
     launch_update_xr_k(n, kp, b.x.as<double>(), b.p.as<double>(), b.r.as<double>(), b.ap.as<double>(), st, bs,
                        b.part.as<double>(), s);
     launch_refresh_active_k(kp, st, bs, s);
-    enqueue_precondition_k(false, kp, s);
+    enqueue_precondition_k(false, kp, s, which);
     launch_end_iteration_k(kp, st, bs, s);
 }
 
 // The iteration graph of block width kp (captured on first use).
-void gpu_solver::replay_iterations_k(int kp, int count, hipStream_t s) const {
+void gpu_solver::replay_iterations_k(int kp, int count, hipStream_t s, int which) const {
     auto& d = *dev_;
-    auto& b = d.blk;
+    auto& b = block_in(*dev_, which);
     if (!opts_.graph) {
-        for (int j = 0; j < count; ++j) enqueue_iteration_k(kp, s);
+        for (int j = 0; j < count; ++j) enqueue_iteration_k(kp, s, which);
         return;
     }
     if (b.graph_kp != kp) {
@@ -1537,7 +1550,7 @@ void gpu_solver::replay_iterations_k(int kp, int count, hipStream_t s) const {
         if (!d.cap) hip_check(hipStreamCreateWithFlags(&d.cap, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamBeginCapture(d.cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
-            enqueue_iteration_k(kp, d.cap);
+            enqueue_iteration_k(kp, d.cap, which);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(d.cap, &g);
@@ -1640,11 +1653,11 @@ gpu_solver::autotune_result gpu_solver::autotune(int max_block_width, hipStream_
 
 // ---- serving sessions: a running block that admits and releases columns ----
 
-int gpu_solver::session_begin(int slots, hipStream_t s) const {
+int gpu_solver::session_begin(int slots, hipStream_t s, int which) const {
     auto& d = *dev_;
-    auto& b = d.blk;
-    const int kp = prepare_block(slots);
-    block_scheds(kp);
+    auto& b = block_in(*dev_, which);
+    const int kp = prepare_block(slots, which);
+    block_scheds(kp, which);
     for (int c = 0; c < kp; ++c) {
         std::memset(&b.host_st[c], 0, sizeof(pcg_state));
         b.host_st[c].done = 1;          // empty slot
@@ -1656,13 +1669,15 @@ int gpu_solver::session_begin(int slots, hipStream_t s) const {
     hip_check(hipMemcpyAsync(b.bs.get(), b.host_bs, sizeof(batch_state), hipMemcpyHostToDevice, s), "state upload");
     launch_zero_tail_k(d.prec, static_cast<int>(n_), static_cast<int>(T_.m), kp, b.z.get(), s);
     hip_check(hipStreamSynchronize(s), "session begin");
+    // The iteration graph, captured now rather than on the first request.
+    replay_iterations_k(kp, 0, s, which);
     b.session_kp = kp;
     return kp;
 }
 
-void gpu_solver::session_admit(const std::vector<session_column>& cols, hipStream_t s) const {
+void gpu_solver::session_admit(const std::vector<session_column>& cols, hipStream_t s, int which) const {
     auto& d = *dev_;
-    auto& b = d.blk;
+    auto& b = block_in(*dev_, which);
     const int kp = b.session_kp;
     if (kp == 0) throw std::logic_error("gpu_solver: no session");
     if (cols.empty()) return;
@@ -1694,12 +1709,12 @@ void gpu_solver::session_admit(const std::vector<session_column>& cols, hipStrea
     hip_check(hipStreamSynchronize(s), "admit");
 }
 
-void gpu_solver::session_step(int iterations, hipStream_t s) const {
-    replay_iterations_k(dev_->blk.session_kp, iterations, s);
+void gpu_solver::session_step(int iterations, hipStream_t s, int which) const {
+    replay_iterations_k(block_in(*dev_, which).session_kp, iterations, s, which);
 }
 
-void gpu_solver::session_read(std::vector<session_slot>& out, hipStream_t s) const {
-    auto& b = dev_->blk;
+void gpu_solver::session_read(std::vector<session_slot>& out, hipStream_t s, int which) const {
+    auto& b = block_in(*dev_, which);
     const int kp = b.session_kp;
     hip_check(hipMemcpyAsync(b.host_st, b.st.get(), static_cast<std::size_t>(kp) * sizeof(pcg_state),
                              hipMemcpyDeviceToHost, s), "state read");
@@ -1713,9 +1728,10 @@ void gpu_solver::session_read(std::vector<session_slot>& out, hipStream_t s) con
     }
 }
 
-void gpu_solver::session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s) const {
+void gpu_solver::session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s,
+                                 int which) const {
     auto& d = *dev_;
-    auto& b = d.blk;
+    auto& b = block_in(*dev_, which);
     const int kp = b.session_kp;
     const int nc = static_cast<int>(slots.size());
     if (nc == 0) return;

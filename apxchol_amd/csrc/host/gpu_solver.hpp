@@ -151,17 +151,23 @@ public:
         std::int64_t iterations;
         double residual;
     };
-    int session_begin(int slots, hipStream_t s) const;
-    void session_admit(const std::vector<session_column>& cols, hipStream_t s) const;
-    void session_step(int iterations, hipStream_t s) const;
-    void session_read(std::vector<session_slot>& out, hipStream_t s) const;
-    void session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s) const;
+    // which: 0 or 1, two independent sessions (their own block buffers,
+    // tables and graphs), which may step side by side on two streams.
+    int session_begin(int slots, hipStream_t s, int which = 0) const;
+    void session_admit(const std::vector<session_column>& cols, hipStream_t s, int which = 0) const;
+    void session_step(int iterations, hipStream_t s, int which = 0) const;
+    void session_read(std::vector<session_slot>& out, hipStream_t s, int which = 0) const;
+    void session_collect(const std::vector<int>& slots, const std::vector<double*>& d_x, hipStream_t s,
+                         int which = 0) const;
     // A single-vector solve in steps (a serving loop's single lane runs one
     // beside its block): solve_device is single_begin, single_step until
     // single_poll says done, then single_collect, so a request run this way
     // gets the bits of solve_device with the same tol and max_iter. It uses
     // the solver's single-vector buffers and state, which a session leaves
     // alone; its steps may run on another stream than the session's.
+    // single_prepare captures the iteration graph beforehand (single_begin
+    // does it on first use otherwise).
+    void single_prepare() const;
     void single_begin(const double* d_b, const double* d_x0, double tol, int max_iter, hipStream_t s) const;
     void single_step(int iterations, hipStream_t s) const;
     bool single_poll(session_slot& out, hipStream_t s) const;
@@ -240,14 +246,14 @@ private:
     void enqueue_precondition(bool first, int step, hipStream_t s) const;
     std::vector<solve_result> solve_block(int k, const double* d_B, double* d_X, double tol, int max_iter,
                                           hipStream_t s) const;
-    int prepare_block(int k) const;
-    void replay_iterations_k(int kp, int count, hipStream_t s) const;
+    int prepare_block(int k, int which = 0) const;
+    void replay_iterations_k(int kp, int count, hipStream_t s, int which = 0) const;
     void fresh_state(pcg_state& h, double tol, int max_iter) const;
-    const block_sched* block_scheds(int kp) const;
+    const block_sched* block_scheds(int kp, int which = 0) const;
     void spmv_block(int k, const double* d_X, double* d_Y, hipStream_t s) const;
     void apply_block(int k, const double* d_R, double* d_Z, bool center, hipStream_t s) const;
-    void enqueue_iteration_k(int kp, hipStream_t s) const;
-    void enqueue_precondition_k(bool first, int kp, hipStream_t s) const;
+    void enqueue_iteration_k(int kp, hipStream_t s, int which = 0) const;
+    void enqueue_precondition_k(bool first, int kp, hipStream_t s, int which = 0) const;
 
     solve_options opts_;
     std::vector<std::uint32_t> relabel_;   // factor position -> storage position (empty: identity)

@@ -27,6 +27,30 @@ solve_server::solve_server(const server_options& opts) : opts_(opts) {
     opts_.iters_per_check = std::max(1, opts_.iters_per_check);
 }
 
+namespace {
+
+// Frees a lane's streams and buffers (after its worker has stopped).
+template <class Lane>
+void free_lane(Lane& l) {
+    for (auto& b : l.blocks) {
+        if (b.stream) (void)hipStreamSynchronize(b.stream);
+        if (b.d_b) (void)hipFree(b.d_b);
+        if (b.d_x) (void)hipFree(b.d_x);
+        if (b.stream) (void)hipStreamDestroy(b.stream);
+        b.stream = nullptr;
+        b.d_b = b.d_x = nullptr;
+    }
+    if (l.single_stream) (void)hipStreamSynchronize(l.single_stream);
+    if (l.d_sb) (void)hipFree(l.d_sb);
+    if (l.d_sx) (void)hipFree(l.d_sx);
+    if (l.single_stream) (void)hipStreamDestroy(l.single_stream);
+    if (l.h_stage) (void)hipHostFree(l.h_stage);
+    l.single_stream = nullptr;
+    l.d_sb = l.d_sx = l.h_stage = nullptr;
+}
+
+}  // namespace
+
 solve_server::~solve_server() {
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -36,15 +60,7 @@ solve_server::~solve_server() {
     for (auto& l : lanes_)
         if (l->worker.joinable()) l->worker.join();
     for (auto& l : lanes_) {
-        if (l->stream) (void)hipStreamSynchronize(l->stream);
-        if (l->single_stream) (void)hipStreamSynchronize(l->single_stream);
-        if (l->d_b) (void)hipFree(l->d_b);
-        if (l->d_x) (void)hipFree(l->d_x);
-        if (l->d_sb) (void)hipFree(l->d_sb);
-        if (l->d_sx) (void)hipFree(l->d_sx);
-        if (l->single_stream) (void)hipStreamDestroy(l->single_stream);
-        if (l->h_stage) (void)hipHostFree(l->h_stage);
-        if (l->stream) (void)hipStreamDestroy(l->stream);
+        free_lane(*l);
         if (l->solver) l->solver->release_serving();
     }
 }
@@ -59,16 +75,27 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
         const gpu_solver* s;
         ~release_on_throw() { if (s) s->release_serving(); }
     } guard{solver.get()};
+    auto pow2 = [](int want) {
+        int kp = 1;
+        while (kp < want) kp <<= 1;
+        return kp;
+    };
     // slots = 0: the factor's autotuned block width (32 if not tuned).
-    const int want = opts_.slots > 0 ? opts_.slots : (solver->preferred_block_width() > 0 ? solver->preferred_block_width() : 32);
-    int kp = 1;
-    while (kp < want) kp <<= 1;
+    const int kp = pow2(opts_.slots > 0 ? opts_.slots : (solver->preferred_block_width() > 0 ? solver->preferred_block_width() : 32));
+    int small_slots;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        small_slots = opts_.small_slots;
+    }
+    // The narrow block only when it is narrower than the main one.
+    const int kp_small = small_slots > 0 && pow2(small_slots) < kp ? pow2(small_slots) : 0;
     const std::int64_t n = solver->rows();
-    // Solver arrays + the block (r, x, p, Ap, y, z) + the b / x staging of
-    // the block and of the single lane.
+    const std::uint64_t nn = static_cast<std::uint64_t>(std::max<std::int64_t>(n, 1));
+    // Solver arrays + each block (r, x, p, Ap, y, z and its b / x staging) +
+    // the single lane's b / x.
     const std::uint64_t need = solver->stats().device_bytes +
-                               static_cast<std::uint64_t>(8) * static_cast<std::uint64_t>(n) * kp * sizeof(double) +
-                               static_cast<std::uint64_t>(2) * static_cast<std::uint64_t>(n) * sizeof(double);
+                               static_cast<std::uint64_t>(8) * nn * static_cast<std::uint64_t>(kp + kp_small) * sizeof(double) +
+                               static_cast<std::uint64_t>(2) * nn * sizeof(double);
     auto ln = std::make_unique<lane>();
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -77,26 +104,35 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
                                      " > " + std::to_string(opts_.memory_budget) + " bytes)");
         used_ += need;
     }
-    ln->solver = std::move(solver);
-    ln->kp = kp;
-    ln->n = n;
-    ln->slot.assign(static_cast<std::size_t>(kp), nullptr);
-    check(hipSetDevice(device_), "hipSetDevice");
-    check(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const std::size_t vb = static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * kp * sizeof(double);
-    check(hipMalloc(&ln->d_b, vb), "hipMalloc");
-    check(hipMalloc(&ln->d_x, vb), "hipMalloc");
-    check(hipStreamCreateWithFlags(&ln->single_stream, hipStreamNonBlocking), "hipStreamCreate");
-    const std::size_t v1 = static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * sizeof(double);
-    check(hipMalloc(&ln->d_sb, v1), "hipMalloc");
-    check(hipMalloc(&ln->d_sx, v1), "hipMalloc");
-    // Pinned staging of one vector: every host <-> device copy of the lane is
-    // a DMA from / to it, never a copy from pageable memory (which the
-    // runtime stages internally and may serialize behind other streams).
-    check(hipHostMalloc(&ln->h_stage, static_cast<std::size_t>(std::max<std::int64_t>(n, 1)) * sizeof(double),
-                        hipHostMallocDefault),
-          "hipHostMalloc");
-    ln->solver->session_begin(kp, ln->stream);
+    try {
+        ln->solver = solver;
+        ln->n = n;
+        check(hipSetDevice(device_), "hipSetDevice");
+        // Sessions in admission order: the narrow block (solver session 1),
+        // then the main one (session 0).
+        if (kp_small > 0) ln->blocks.push_back(block_lane{1, kp_small});
+        ln->blocks.push_back(block_lane{0, kp});
+        for (auto& b : ln->blocks) {
+            b.slot.assign(static_cast<std::size_t>(b.kp), nullptr);
+            check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking), "hipStreamCreate");
+            check(hipMalloc(&b.d_b, nn * b.kp * sizeof(double)), "hipMalloc");
+            check(hipMalloc(&b.d_x, nn * b.kp * sizeof(double)), "hipMalloc");
+        }
+        check(hipStreamCreateWithFlags(&ln->single_stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipMalloc(&ln->d_sb, nn * sizeof(double)), "hipMalloc");
+        check(hipMalloc(&ln->d_sx, nn * sizeof(double)), "hipMalloc");
+        // Pinned staging of one vector: every host <-> device copy of the lane
+        // is a DMA from / to it, never a copy from pageable memory (which the
+        // runtime stages internally and may serialize behind other streams).
+        check(hipHostMalloc(&ln->h_stage, nn * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        for (auto& b : ln->blocks) ln->solver->session_begin(b.kp, b.stream, b.which);
+        ln->solver->single_prepare();
+    } catch (...) {
+        free_lane(*ln);
+        std::lock_guard<std::mutex> lk(mu_);
+        used_ -= need;
+        throw;
+    }
     lane* raw = ln.get();
     int id;
     {
@@ -160,6 +196,12 @@ void solve_server::set_single_lane(bool on) {
     work_cv_.notify_all();
 }
 
+void solve_server::set_small_slots(int slots) {
+    if (slots < 0 || slots > 64) throw std::invalid_argument("server: narrow block slots must be in [0, 64]");
+    std::lock_guard<std::mutex> lk(mu_);
+    opts_.small_slots = slots;
+}
+
 server_stats solve_server::stats() const {
     std::lock_guard<std::mutex> lk(mu_);
     server_stats s;
@@ -194,10 +236,13 @@ void solve_server::finish(lane& ln, const std::vector<std::shared_ptr<request>>&
     for (const auto& rq : done) {
         rq->t_done = now;
         rq->done = true;
-        latencies_ms_.push_back(std::chrono::duration<double, std::milli>(now - rq->t_submit).count());
+        const double ms = std::chrono::duration<double, std::milli>(now - rq->t_submit).count();
+        rq->res.timings.pcg_ms = ms;
+        latencies_ms_.push_back(ms);
         last_done_ = now;
-        for (auto& c : ln.slot)
-            if (c == rq) c = nullptr;
+        for (auto& b : ln.blocks)
+            for (auto& c : b.slot)
+                if (c == rq) c = nullptr;
         if (ln.single == rq) {
             ln.single = nullptr;
             ++single_completed_;
@@ -209,11 +254,8 @@ void solve_server::finish(lane& ln, const std::vector<std::shared_ptr<request>>&
 void solve_server::run(lane& ln) {
     (void)hipSetDevice(device_);
     const std::size_t n = static_cast<std::size_t>(ln.n);
-    double* db = static_cast<double*>(ln.d_b);
-    double* dx = static_cast<double*>(ln.d_x);
     double* stage = static_cast<double*>(ln.h_stage);
     const int J = opts_.iters_per_check;
-    std::vector<gpu_solver::session_slot> states;
     // Host <-> device copies of one vector through the pinned staging vector
     // (reused, so each copy completes before the next).
     auto upload = [&](const std::vector<double>& b, double* dst, hipStream_t s) {
@@ -226,35 +268,59 @@ void solve_server::run(lane& ln) {
         check(hipStreamSynchronize(s), "download x");
         x.assign(stage, stage + n);
     };
+    const std::size_t nb = ln.blocks.size();
     for (;;) {
-        std::vector<std::pair<int, std::shared_ptr<request>>> admit;
+        // Per block: the columns admitted this turn, and whether any is busy.
+        std::vector<std::vector<std::pair<int, std::shared_ptr<request>>>> admit(nb);
+        std::vector<char> busy(nb, 0);
         std::shared_ptr<request> single_new, single;
-        bool busy = false;
         {
             std::unique_lock<std::mutex> lk(mu_);
             work_cv_.wait(lk, [&] {
                 if (stop_ || !ln.queue.empty() || ln.single) return true;
-                for (auto& r : ln.slot)
-                    if (r) return true;
+                for (auto& b : ln.blocks)
+                    for (auto& r : b.slot)
+                        if (r) return true;
                 return false;
             });
             if (stop_) return;
-            // The single lane takes the oldest waiting request when it is free;
-            // the rest join free columns of the block.
+            // The oldest waiting request takes the single lane when it is
+            // free; the rest fill free columns, the narrow block first.
             if (opts_.single_lane && !ln.single && !ln.queue.empty() && ln.n > 0) {
                 ln.single = ln.queue.front();
                 ln.queue.pop_front();
                 single_new = ln.single;
             }
             single = ln.single;
-            for (int c = 0; c < ln.kp && !ln.queue.empty(); ++c)
-                if (!ln.slot[c]) {
-                    ln.slot[c] = ln.queue.front();
-                    ln.queue.pop_front();
-                    admit.emplace_back(c, ln.slot[c]);
+            // The narrow block takes the waiting requests only while the main
+            // block is idle and they all fit: a running main block costs the
+            // same with more columns, so under load it takes them.
+            auto active = [](const block_lane& b) {
+                for (auto& r : b.slot)
+                    if (r) return true;
+                return false;
+            };
+            auto free_slots = [](const block_lane& b) {
+                std::size_t f = 0;
+                for (auto& r : b.slot) f += r ? 0 : 1;
+                return f;
+            };
+            const bool use_narrow = nb > 1 && !active(ln.blocks[nb - 1]) && ln.queue.size() <= free_slots(ln.blocks[0]);
+            for (std::size_t k = 0; k < nb; ++k) {
+                auto& b = ln.blocks[k];
+                if (nb > 1 && k == 0 && !use_narrow) {
+                    if (active(b)) busy[k] = 1;
+                    continue;
                 }
-            for (auto& r : ln.slot)
-                if (r) busy = true;
+                for (int c = 0; c < b.kp && !ln.queue.empty(); ++c)
+                    if (!b.slot[c]) {
+                        b.slot[c] = ln.queue.front();
+                        ln.queue.pop_front();
+                        admit[k].emplace_back(c, b.slot[c]);
+                    }
+                for (auto& r : b.slot)
+                    if (r) busy[k] = 1;
+            }
         }
         try {
             if (single_new) {
@@ -262,19 +328,22 @@ void solve_server::run(lane& ln) {
                 ln.solver->single_begin(static_cast<const double*>(ln.d_sb), nullptr, single_new->tol,
                                         single_new->max_iter, ln.single_stream);
             }
-            if (!admit.empty()) {
+            for (std::size_t k = 0; k < nb; ++k) {
+                if (admit[k].empty()) continue;
+                auto& b = ln.blocks[k];
                 std::vector<gpu_solver::session_column> cols;
-                for (auto& [c, rq] : admit) {
-                    double* dst = db + static_cast<std::size_t>(c) * n;
-                    upload(rq->b, dst, ln.stream);
+                for (auto& [c, rq] : admit[k]) {
+                    double* dst = static_cast<double*>(b.d_b) + static_cast<std::size_t>(c) * n;
+                    upload(rq->b, dst, b.stream);
                     cols.push_back({c, dst, rq->tol, rq->max_iter});
                 }
-                ln.solver->session_admit(cols, ln.stream);
+                ln.solver->session_admit(cols, b.stream, b.which);
             }
             // One iteration group of each, on their own streams, so the device
             // runs them side by side; then each is read back.
             if (single) ln.solver->single_step(J, ln.single_stream);
-            if (busy) ln.solver->session_step(J, ln.stream);
+            for (std::size_t k = 0; k < nb; ++k)
+                if (busy[k]) ln.solver->session_step(J, ln.blocks[k].stream, ln.blocks[k].which);
             std::vector<std::shared_ptr<request>> finished;
             if (single) {
                 gpu_solver::session_slot st1;
@@ -289,31 +358,29 @@ void solve_server::run(lane& ln) {
                     finished.push_back(single);
                 }
             }
-            if (busy) {
-                ln.solver->session_read(states, ln.stream);
+            for (std::size_t k = 0; k < nb; ++k) {
+                if (!busy[k]) continue;
+                auto& b = ln.blocks[k];
+                ln.solver->session_read(b.states, b.stream, b.which);
                 std::vector<int> fin;
                 std::vector<double*> dst;
-                {
-                    std::lock_guard<std::mutex> lk(mu_);
-                    for (int c = 0; c < ln.kp; ++c)
-                        if (ln.slot[c] && states[c].done) {
-                            fin.push_back(c);
-                            dst.push_back(dx + static_cast<std::size_t>(c) * n);
-                        }
-                }
-                if (!fin.empty()) {
-                    ln.solver->session_collect(fin, dst, ln.stream);
-                    for (std::size_t j = 0; j < fin.size(); ++j) {
-                        auto rq = ln.slot[fin[j]];
-                        download(dst[j], rq->x, ln.stream);
-                        const auto& st = states[fin[j]];
-                        rq->res.iterations = st.iterations;
-                        rq->res.residual = st.residual;
-                        rq->res.status = st.status;
-                        rq->res.lumped_offdiag = ln.solver->factor().lumped_offdiag;
-                        if (st.status == 5) std::fill(rq->x.begin(), rq->x.end(), 0.0);
-                        finished.push_back(rq);
+                for (int c = 0; c < b.kp; ++c)
+                    if (b.slot[c] && b.states[c].done) {
+                        fin.push_back(c);
+                        dst.push_back(static_cast<double*>(b.d_x) + static_cast<std::size_t>(c) * n);
                     }
+                if (fin.empty()) continue;
+                ln.solver->session_collect(fin, dst, b.stream, b.which);
+                for (std::size_t j = 0; j < fin.size(); ++j) {
+                    auto rq = b.slot[fin[j]];
+                    download(dst[j], rq->x, b.stream);
+                    const auto& st = b.states[fin[j]];
+                    rq->res.iterations = st.iterations;
+                    rq->res.residual = st.residual;
+                    rq->res.status = st.status;
+                    rq->res.lumped_offdiag = ln.solver->factor().lumped_offdiag;
+                    if (st.status == 5) std::fill(rq->x.begin(), rq->x.end(), 0.0);
+                    finished.push_back(rq);
                 }
             }
             if (!finished.empty()) finish(ln, finished);
@@ -329,7 +396,8 @@ void solve_server::run(lane& ln) {
                 rq->done = true;
                 rq = nullptr;
             };
-            for (auto& r : ln.slot) fail(r);
+            for (auto& b : ln.blocks)
+                for (auto& r : b.slot) fail(r);
             fail(ln.single);
             for (auto& r : ln.queue) fail(r);
             ln.queue.clear();

@@ -11,7 +11,14 @@
 // arithmetic does not depend on its neighbours or its slot, so a request that
 // runs in the block gets the same bits alone or in a full block.
 //
-// Single lane (on by default): beside its block, each factor runs one request
+// Narrow block (4 columns by default): a second session of the factor, on a
+// third stream, that takes the waiting requests while the main block is idle
+// and they fit it, so a few requests in flight iterate at a narrow block's
+// cost rather than the main block's (under load the running main block takes
+// them: its cost does not grow with its columns). A request's bits are those
+// of the block that took it.
+//
+// Single lane (on by default): beside its blocks, each factor runs one request
 // at a time as a single-vector solve on a second stream. The oldest waiting
 // request takes it whenever it is free, so at light load a request costs a
 // single-vector iteration instead of a whole block's; its result is the bits
@@ -43,6 +50,8 @@ struct server_options {
                                        // 0: each factor's autotuned block width)
     int iters_per_check = 2;           // PCG iterations between admissions / completions
     bool single_lane = true;           // one request per factor at a time as a single-vector solve
+    int small_slots = 4;               // columns of a second, narrower block filled before the main
+                                       // one (0: none; used only when narrower than `slots`)
 };
 
 struct server_stats {
@@ -72,6 +81,8 @@ public:
     server_stats stats() const;
     // Turns the single lanes on or off (requests already on one finish there).
     void set_single_lane(bool on);
+    // Columns of the narrow block of factors added from now on (0: none).
+    void set_small_slots(int slots);
 
 private:
     struct request {
@@ -84,20 +95,26 @@ private:
         bool done = false;
         std::chrono::steady_clock::time_point t_submit, t_done;
     };
+    // One session of a factor: a block of kp columns on its own stream.
+    struct block_lane {
+        int which = 0;                                // the solver's session index
+        int kp = 0;
+        hipStream_t stream = nullptr;
+        std::vector<std::shared_ptr<request>> slot;   // request in each column
+        void* d_b = nullptr;                          // kp x n staging (device)
+        void* d_x = nullptr;
+        std::vector<gpu_solver::session_slot> states;
+    };
     struct lane {
         std::shared_ptr<gpu_solver> solver;
-        hipStream_t stream = nullptr;
         std::thread worker;
         std::deque<std::shared_ptr<request>> queue;
-        std::vector<std::shared_ptr<request>> slot;   // request in each column
+        std::vector<block_lane> blocks;               // filled in order: the narrow block, then the main one
         std::shared_ptr<request> single;              // request on the single lane
         hipStream_t single_stream = nullptr;
-        void* d_b = nullptr;                          // slots x n staging (device)
-        void* d_x = nullptr;
         void* d_sb = nullptr;                         // single lane's b and x (device, n doubles each)
         void* d_sx = nullptr;
         void* h_stage = nullptr;                      // n doubles, pinned host memory
-        int kp = 0;
         std::int64_t n = 0;
         std::string error;                            // set when a device failure ended the lane
     };

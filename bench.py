@@ -84,8 +84,9 @@ def percentile(xs, q):
 def run_serving(args, ax, mg, np) -> dict:
     """The serving mix: 8 resident factors, Poisson arrivals at a fixed rate,
     one N(0,1) right-hand side per request with tol drawn from {1e-6, 1e-8,
-    1e-10}; mixed precision, continuous batching with the single lanes on
-    (and, for comparison, off). Every result is graded by its true residual."""
+    1e-10}; mixed precision, continuous batching with the single lanes and
+    narrow blocks on (and, for comparison, the main blocks alone). Every
+    result is graded by its true residual."""
     import threading
     t0 = time.perf_counter()
     mats, solvers = [], []
@@ -107,11 +108,14 @@ def run_serving(args, ax, mg, np) -> dict:
         plan.append((f, float(tols[int(rng.integers(3))]), float(rng.exponential(1.0 / args.serving_qps))))
     rhs = [mg.request_rhs(mats[f][1], mats[f][2], 1000 + i) for i, (f, _, _) in enumerate(plan)]
 
-    def offered(single_lane: bool) -> dict:
+    def offered(lanes: bool) -> dict:
         """The Poisson arrivals into a server over the resident solvers (a
-        solver serves one server at a time, so each pass has its own)."""
-        label = "serving" if single_lane else "serving, blocks only"
-        srv = ax.Server(slots=args.serving_slots, iters_per_check=2, single_lane=single_lane)
+        solver serves one server at a time, so each pass has its own); lanes:
+        the single lane and the narrow block on (the defaults), else the main
+        block alone."""
+        label = "serving" if lanes else "serving, main block only"
+        srv = ax.Server(slots=args.serving_slots, iters_per_check=2, single_lane=lanes,
+                        narrow_slots=args.serving_narrow if lanes else 0)
         fids = [srv.add(S) for S in solvers]
         ids = [None] * len(plan)
         errors = []
@@ -133,6 +137,7 @@ def run_serving(args, ax, mg, np) -> dict:
             raise errors[0]
         print(f"[bench] {label}: {len(plan)} requests submitted", file=sys.stderr, flush=True)
         worst = 0.0
+        lat = [[] for _ in mats]
         for i, (f, tol, _) in enumerate(plan):
             # A request the server has not finished within 120 s ends the
             # measurement with that reason (the serving section records it).
@@ -146,6 +151,7 @@ def run_serving(args, ax, mg, np) -> dict:
             L = mats[f][1]
             tr = float(np.linalg.norm(rhs[i] - L @ x) / np.linalg.norm(rhs[i]))
             worst = max(worst, tr / tol)
+            lat[f].append(r.timings["pcg_ms"])
             if (i + 1) % 64 == 0:
                 print(f"[bench] {label}: {i + 1} of {len(plan)} results graded ({time.perf_counter() - t1:.1f}s)",
                       file=sys.stderr, flush=True)
@@ -157,17 +163,20 @@ def run_serving(args, ax, mg, np) -> dict:
               flush=True)
         return {"solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                 "mean_ms": st["mean_ms"], "wall_s": wall, "worst_true_residual_over_tol": worst,
-                "memory_used_bytes": st["memory_used"], "single_lane_completed": st["single_lane_completed"]}
+                "memory_used_bytes": st["memory_used"], "single_lane_completed": st["single_lane_completed"],
+                "by_factor": {mats[f][0]: {"requests": len(v), "p50_ms": percentile(v, 50), "p99_ms": percentile(v, 99)}
+                              for f, v in enumerate(lat) if v}}
 
     main = offered(True)
     out = {"factors": [m[0] for m in mats], "requests": len(plan), "qps_offered": args.serving_qps,
-           "slots": args.serving_slots, "single_lane": True, **main, "setup_s": t_setup}
+           "slots": args.serving_slots, "narrow_slots": args.serving_narrow, "single_lane": True, **main,
+           "setup_s": t_setup}
     if args.serving_blocks_only:
-        out["blocks_only"] = offered(False)
+        out["main_block_only"] = offered(False)
     # Capacity: the same requests submitted at once to a fresh server over the
     # same resident solvers (the first server released them), so the rate is
     # what the device sustains rather than the offered load.
-    cap = ax.Server(slots=args.serving_slots, iters_per_check=2, single_lane=True)
+    cap = ax.Server(slots=args.serving_slots, iters_per_check=2, single_lane=True, narrow_slots=args.serving_narrow)
     cfids = [cap.add(S) for S in solvers]
     t2 = time.perf_counter()
     cids = [cap.submit(cfids[f], rhs[i], tol) for i, (f, tol, _) in enumerate(plan)]
@@ -398,8 +407,9 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serving-qps", type=float, default=25.0)
     ap.add_argument("--serving-slots", type=int, default=32)
+    ap.add_argument("--serving-narrow", type=int, default=4, help="columns of each factor's narrow block")
     ap.add_argument("--serving-blocks-only", type=int, default=1,
-                    help="also run the arrivals with the single lanes off (blocks only), for comparison")
+                    help="also run the arrivals on the main blocks alone (no single lane, no narrow block)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 256 MiB (calibration of the HBM byte counters)")
     args = ap.parse_args()

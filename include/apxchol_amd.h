@@ -79,7 +79,8 @@ typedef struct {
     double factorize_ms;     /* setup/factorize */
     double sptrsv_setup_ms;  /* setup/sptrsv_setup: trsv preparation, device layouts and uploads */
     double pcg_setup_ms;     /* setup/gpu_pcg_setup: the permuted operator (CSR + sliced ELL) */
-    double pcg_ms;           /* pcg/gpu_pcg_loop: this solve, its final synchronization included */
+    double pcg_ms;           /* pcg/gpu_pcg_loop: this solve, its final synchronization included
+                                (a served request: its submit -> result latency) */
 } apx_solve_info;
 
 typedef struct {
@@ -198,14 +199,15 @@ apx_status apx_solver_autotune(apx_solver_t s, int32_t max_block_width, apx_auto
  * `slots` columns running on its own stream; requests are admitted into free
  * columns every `iters_per_check` iterations and released when they finish
  * (continuous batching), and factors are admitted against `memory_budget`
- * bytes of HBM (0: 90% of the free memory). A request run in the block gets
- * the same bits alone or among others. Beside its block, each factor has a
- * single lane (on by default): one request at a time runs there as a
- * single-vector solve on a second stream, side by side with the block, and
- * the oldest waiting request takes it whenever it is free; such a request
- * gets the bits of apx_solver_solve with its tol and max_iter. A solver added
- * to a server is used by the server only (the server keeps it alive after
- * apx_solver_destroy). */
+ * bytes of HBM (0: 90% of the free memory). A request run in a block gets
+ * the same bits alone or among others in a block of that width. Each factor
+ * also has a narrow block (apx_server_set_narrow_block) filled before the
+ * main one, and a single lane (on by default): one request at a time runs
+ * there as a single-vector solve, side by side with the blocks on a stream
+ * of its own, and the oldest waiting request takes it whenever it is free;
+ * such a request gets the bits of apx_solver_solve with its tol and
+ * max_iter. A solver added to a server is used by the server only (the
+ * server keeps it alive after apx_solver_destroy). */
 typedef struct apx_server_s* apx_server_t;
 typedef struct {
     int64_t submitted, completed, factors;
@@ -218,6 +220,10 @@ apx_status apx_server_create(uint64_t memory_budget, int32_t slots, int32_t iter
 void apx_server_destroy(apx_server_t srv);
 /* Single lanes on (1, the default) or off (0); a request already on one finishes there. */
 apx_status apx_server_set_single_lane(apx_server_t srv, int32_t on);
+/* Columns of the narrow block of factors added afterwards (default 4; 0: none;
+ * used only when narrower than `slots`): a second block per factor, on its
+ * own stream, that takes requests before the main block does. */
+apx_status apx_server_set_narrow_block(apx_server_t srv, int32_t slots);
 apx_status apx_server_add(apx_server_t srv, apx_solver_t solver, int32_t* factor_id);
 /* b: n host doubles (copied); tol / max_iter as in apx_solver_solve. */
 apx_status apx_server_submit(apx_server_t srv, int32_t factor, const double* b, double tol, int32_t max_iter,

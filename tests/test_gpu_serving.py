@@ -1,11 +1,12 @@
 """GPU tests of the serving loop (continuous batching over resident factors).
 
-Bars: with the single lanes off, every request's result is bit-identical
-whether it ran alone in its factor's block or admitted mid-flight among
-others (the per-column arithmetic does not depend on the other columns or
-the slot); with them on (the default), a request alone gets the bits of its
-own single-vector solve, and in the mix the bits of either that solve or its
-block column; every result meets its own tolerance (true residual below
+Bars: with one block per factor (no single lane, no narrow block), every
+request's result is bit-identical whether it ran alone in its factor's block
+or admitted mid-flight among others (the per-column arithmetic does not
+depend on the other columns or the slot); with the defaults, a request alone
+gets the bits of its own single-vector solve, and in the mix the bits of
+exactly one of its paths (that solve, or its column in the narrow or the
+main block); every result meets its own tolerance (true residual below
 2 x tol) and agrees with a single-vector solve within one iteration; factors
 beyond the memory budget are refused."""
 import threading
@@ -59,7 +60,7 @@ def test_serving_mix_equals_alone(gpu, slots):
     reqs = make_requests(36, 7)
     # In the mix: Poisson-like arrivals from a producer thread while the
     # server iterates.
-    mix = ax.Server(slots=slots, iters_per_check=2, single_lane=False)
+    mix = ax.Server(slots=slots, iters_per_check=2, single_lane=False, narrow_slots=0)
     fid = {name: mix.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
     ids = [None] * len(reqs)
 
@@ -76,7 +77,7 @@ def test_serving_mix_equals_alone(gpu, slots):
     assert st["completed"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0 and st["solves_per_s"] > 0
     mix.close()
     # Alone: the same requests one at a time, each alone in its block.
-    alone = ax.Server(slots=slots, iters_per_check=2, single_lane=False)
+    alone = ax.Server(slots=slots, iters_per_check=2, single_lane=False, narrow_slots=0)
     fid2 = {name: alone.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
     for i, (name, b, tol, _) in enumerate(reqs):
         x1, r1 = alone.wait(alone.submit(fid2[name], b, tol), timeout=120)
@@ -89,21 +90,30 @@ def test_serving_mix_equals_alone(gpu, slots):
     alone.close()
 
 
-def test_single_lane_results_are_own_or_block_solves(gpu):
-    """Single lanes on: a request submitted alone runs on its factor's single
-    lane and gets the bits of Solver.solve(b, tol); among Poisson arrivals
-    each request gets the bits of either that solve or its column of the
-    block alone (whichever path took it), and some take each path."""
-    print("[case] serving single lane", flush=True)
+def block_alone(reqs, width):
+    """Each request alone in a block of `width` columns (no single lane, no
+    narrow block)."""
+    srv = ax.Server(slots=width, iters_per_check=2, single_lane=False, narrow_slots=0)
+    f = {name: srv.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
+    out = [srv.wait(srv.submit(f[name], b, tol), timeout=120) for name, b, tol, _ in reqs]
+    srv.close()
+    return out
+
+
+def test_requests_take_the_single_lane_or_a_block(gpu):
+    """Defaults (single lane on, a 4-column narrow block before an 8-column
+    main block): a request submitted alone runs on its factor's single lane
+    and gets the bits of Solver.solve(b, tol); among Poisson arrivals each
+    request gets the bits of exactly one of its paths -- that solve, its
+    column alone in a 4-wide block, or alone in an 8-wide block -- and the
+    single lane and the narrow block both take requests."""
+    print("[case] serving single lane + narrow block", flush=True)
     reqs = make_requests(36, 7)
     own = {name: ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32") for name in FACTORS}
     singles = [own[name].solve(b, tol=tol, maxiter=200) for name, b, tol, _ in reqs]
     for o in own.values():
         o.close()
-    block = ax.Server(slots=8, iters_per_check=2, single_lane=False)
-    fb = {name: block.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
-    blocks = [block.wait(block.submit(fb[name], b, tol), timeout=120) for name, b, tol, _ in reqs]
-    block.close()
+    narrow, main = block_alone(reqs, 4), block_alone(reqs, 8)
     alone = ax.Server(slots=8, iters_per_check=2)
     fa = {name: alone.add(ax.Solver(factor(name)[0], factor=factor(name)[1], precision="fp32")) for name in FACTORS}
     for i, (name, b, tol, _) in enumerate(reqs[:9]):
@@ -117,28 +127,28 @@ def test_single_lane_results_are_own_or_block_solves(gpu):
 
     def produce():
         for i, (name, b, tol, gap) in enumerate(reqs):
-            time.sleep(gap)
+            time.sleep(gap / 4)
             ids[i] = mix.submit(fm[name], b, tol)
 
     th = threading.Thread(target=produce)
     th.start()
     th.join()
     mixed = [mix.wait(r, timeout=120) for r in ids]
+    st = mix.stats()
     mix.close()
-    paths = {"single": 0, "block": 0}
+    paths = {"single": 0, "narrow": 0, "main": 0}
     for i, (name, b, tol, _) in enumerate(reqs):
         xm, rm = mixed[i]
         A, _ = factor(name)
         assert rm.status == 1 and np.linalg.norm(b - A @ xm) <= 2 * tol * np.linalg.norm(b), (i, name, tol)
-        if np.array_equal(bits(xm), bits(singles[i].x)):
-            assert rm.iterations == singles[i].iterations, i
-            paths["single"] += 1
-        else:
-            xb, rb = blocks[i]
-            assert np.array_equal(bits(xm), bits(xb)) and rm.iterations == rb.iterations, i
-            paths["block"] += 1
+        refs = {"single": (singles[i].x, singles[i].iterations), "narrow": (narrow[i][0], narrow[i][1].iterations),
+                "main": (main[i][0], main[i][1].iterations)}
+        hit = [k for k, (x, it) in refs.items() if np.array_equal(bits(xm), bits(x)) and rm.iterations == it]
+        assert hit, (i, name)
+        paths[hit[0]] += 1
     print(f"[case] paths taken: {paths}", flush=True)
-    assert paths["single"] >= len(FACTORS)
+    assert paths["single"] >= len(FACTORS) and paths["single"] == st["single_lane_completed"]
+    assert paths["narrow"] > 0
 
 
 def test_serving_agrees_with_single_solves(gpu):
