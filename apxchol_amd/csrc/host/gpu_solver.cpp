@@ -1129,7 +1129,10 @@ void gpu_solver::enqueue_precondition(bool first, int step, hipStream_t s) const
     double* part = d.part.as<double>();
     int* err = d.err.as<int>();
     const vec_prec pr = d.prec;
-    if (first) launch_fill_outputs(pr, n, m, d.y.get(), d.z.get(), st, s);
+    // The pending sentinels only for the dataflow schedule (the only one
+    // that polls the outputs); the zero past the sweep rows of z always.
+    void* sentinel_y = opts_.schedule == sweep_schedule::dataflow ? d.y.get() : nullptr;
+    if (first) launch_fill_outputs(pr, n, m, sentinel_y, d.z.get(), st, s);
     tm.time(0, step, s, [&] {
         launch_sweep(d.fwd, pr, d.fp16, d.r.get(), d.y.get(), st, err, d.f_sched, s, tm.hook(step));
     });
@@ -1162,7 +1165,7 @@ void gpu_solver::enqueue_iteration(int it, hipStream_t s) const {
     spmv(d.p.as<double>(), d.ap.as<double>(), st, true, s, tm.hook(it - 1));
     tm.time(3, it - 1, s, [&] {
         launch_update_xr(d.prec, n, m, d.x.as<double>(), d.p.as<double>(), d.r.as<double>(), d.ap.as<double>(),
-                         d.y.get(), d.z.get(), st, part, s);
+                         opts_.schedule == sweep_schedule::dataflow ? d.y.get() : nullptr, d.z.get(), st, part, s);
     });
     enqueue_precondition(false, it, s);
 }

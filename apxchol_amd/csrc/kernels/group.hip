@@ -7,7 +7,8 @@
 // size. A member's arithmetic -- every row, every partial, every fixed-order
 // reduction -- is therefore the bits of its own solve, and members that have
 // converged return at once from their units as they do from their own
-// launches.
+// launches. Members run the hybrid schedule, so no sweep output is given the
+// pending sentinel.
 #include "common.hpp"
 
 namespace apxchol_amd {
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void group_update_xr(const group_member* __
     const int i = group_unit_of(starts, count, b);
     const group_member& g = mem[i];
     update_xr_body<TO>(b - starts[i], starts[i + 1] - starts[i], g.n, g.m, g.x, g.p, g.r, g.ap,
-                       static_cast<TO*>(g.y), static_cast<TO*>(g.z), g.st, g.part);
+                       static_cast<TO*>(nullptr), static_cast<TO*>(g.z), g.st, g.part);
 }
 
 template <class TO>
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void group_fill_outputs(const group_member*
     const int b = static_cast<int>(blockIdx.x);
     const int i = group_unit_of(starts, count, b);
     const group_member& g = mem[i];
-    fill_outputs_body<TO>(b - starts[i], starts[i + 1] - starts[i], g.n, g.m, static_cast<TO*>(g.y),
+    fill_outputs_body<TO>(b - starts[i], starts[i + 1] - starts[i], g.n, g.m, static_cast<TO*>(nullptr),
                           static_cast<TO*>(g.z), g.st);
 }
 

@@ -66,6 +66,7 @@ struct launch_hook {
 void launch_sweep(const sweep_desc& d, vec_prec p, bool fp16, const void* rhs, void* out, const pcg_state* st,
                   int* err, const sweep_sched& sc, hipStream_t s, const launch_hook& hook = {});
 
+// z past the sweep rows = 0; with y given, y and z pending (dataflow only).
 void launch_fill_outputs(vec_prec p, int n, int m, void* y, void* z, const pcg_state* st, hipStream_t s);
 
 void launch_spmv(int n, const unsigned* ptr, const int* idx, const void* val, bool val32, int lanes, const double* x,
@@ -80,6 +81,7 @@ void launch_scatter_init(int n, const unsigned* perm, const double* b, const dou
                          hipStream_t s);
 void launch_norm2_b(int n, const double* b, pcg_state* st, double* part, hipStream_t s);
 void launch_init_residual(int n, double* r, const double* ax0, pcg_state* st, double* part, hipStream_t s);
+// x, r update and its reductions; y, z as in launch_fill_outputs (y null: no sentinels).
 void launch_update_xr(vec_prec p, int n, int m, double* x, const double* pv, double* r, const double* ap, void* y,
                       void* z, pcg_state* st, double* part, hipStream_t s);
 void launch_dot_center(vec_prec p, int n, const double* r, const void* out, pcg_state* st, double* part, bool first, hipStream_t s);

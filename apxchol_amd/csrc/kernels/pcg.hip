@@ -181,7 +181,9 @@ __global__ __launch_bounds__(kBlock) void init_residual(int n, double* __restric
 
 // x += alpha p; r -= alpha Ap; partials of rr, rs of the new r
 // (solve.cpp:169-208), plus the sentinel fill of the next application's sweep
-// outputs. The loop's bookkeeping follows in pcg_finish (kFinXR).
+// outputs when y is given (the dataflow schedule polls them; the launched
+// schedules never read an output before it is written) and the zero past the
+// sweep rows of z. The loop's bookkeeping follows in pcg_finish (kFinXR).
 template <class TO>
 __device__ __forceinline__ void update_xr_body(int vb, int vg, int n, int m, double* __restrict__ x,
                                                const double* __restrict__ p, double* __restrict__ r,
@@ -200,8 +202,8 @@ __device__ __forceinline__ void update_xr_body(int vb, int vg, int n, int m, dou
         r[i] = ri;
         q = fma(ri, ri, q);
         s += ri;
-        if (i < m) { yb[i] = flag_bits<TO>::pending; zb[i] = flag_bits<TO>::pending; }
-        else z[i] = static_cast<TO>(0);
+        if (i >= m) z[i] = static_cast<TO>(0);
+        else if (yb) { yb[i] = flag_bits<TO>::pending; zb[i] = flag_bits<TO>::pending; }
     }
     double v[2] = {q, s};
     block_partials<2>(v, part, sh, vb);
@@ -214,8 +216,9 @@ __global__ __launch_bounds__(kBlock) void update_xr(int n, int m, double* __rest
     update_xr_body<TO>(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), n, m, x, p, r, ap, y, z, st, part);
 }
 
-// Sentinel fill for the first application of a solve (update_xr does it for
-// every later one).
+// Sentinel fill for the first application of a solve, when y is given
+// (update_xr does it for every later one), and the zero past the sweep rows
+// of z.
 template <class TO>
 __device__ __forceinline__ void fill_outputs_body(int vb, int vg, int n, int m, TO* __restrict__ y,
                                                   TO* __restrict__ z, const pcg_state* st) {
@@ -224,8 +227,8 @@ __device__ __forceinline__ void fill_outputs_body(int vb, int vg, int n, int m, 
     B* yb = reinterpret_cast<B*>(y);
     B* zb = reinterpret_cast<B*>(z);
     for (int i = vb * kBlock + threadIdx.x; i < n; i += vg * kBlock) {
-        if (i < m) { yb[i] = flag_bits<TO>::pending; zb[i] = flag_bits<TO>::pending; }
-        else z[i] = static_cast<TO>(0);
+        if (i >= m) z[i] = static_cast<TO>(0);
+        else if (yb) { yb[i] = flag_bits<TO>::pending; zb[i] = flag_bits<TO>::pending; }
     }
 }
 template <class TO>
