@@ -32,14 +32,15 @@ namespace {
 // Frees a lane's streams and buffers (after its worker has stopped).
 template <class Lane>
 void free_lane(Lane& l) {
+    if (l.block_stream) (void)hipStreamSynchronize(l.block_stream);
     for (auto& b : l.blocks) {
-        if (b.stream) (void)hipStreamSynchronize(b.stream);
         if (b.d_b) (void)hipFree(b.d_b);
         if (b.d_x) (void)hipFree(b.d_x);
-        if (b.stream) (void)hipStreamDestroy(b.stream);
         b.stream = nullptr;
         b.d_b = b.d_x = nullptr;
     }
+    if (l.block_stream) (void)hipStreamDestroy(l.block_stream);
+    l.block_stream = nullptr;
     if (l.single_stream) (void)hipStreamSynchronize(l.single_stream);
     if (l.d_sb) (void)hipFree(l.d_sb);
     if (l.d_sx) (void)hipFree(l.d_sx);
@@ -112,9 +113,13 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
         // then the main one (session 0).
         if (kp_small > 0) ln->blocks.push_back(block_lane{1, kp_small});
         ln->blocks.push_back(block_lane{0, kp});
+        // The blocks share one stream (one factor's block work runs in order;
+        // every stream beyond the process's few hardware queues shares one
+        // with others).
+        check(hipStreamCreateWithFlags(&ln->block_stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& b : ln->blocks) {
             b.slot.assign(static_cast<std::size_t>(b.kp), nullptr);
-            check(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking), "hipStreamCreate");
+            b.stream = ln->block_stream;
             check(hipMalloc(&b.d_b, nn * b.kp * sizeof(double)), "hipMalloc");
             check(hipMalloc(&b.d_x, nn * b.kp * sizeof(double)), "hipMalloc");
         }

@@ -11,8 +11,8 @@
 // arithmetic does not depend on its neighbours or its slot, so a request that
 // runs in the block gets the same bits alone or in a full block.
 //
-// Narrow block (4 columns by default): a second session of the factor, on a
-// third stream, that takes the waiting requests while the main block is idle
+// Narrow block (4 columns by default): a second session of the factor, on the
+// main block's stream, that takes the waiting requests while the main block is idle
 // and they fit it, so a few requests in flight iterate at a narrow block's
 // cost rather than the main block's (under load the running main block takes
 // them: its cost does not grow with its columns). A request's bits are those
@@ -95,11 +95,11 @@ private:
         bool done = false;
         std::chrono::steady_clock::time_point t_submit, t_done;
     };
-    // One session of a factor: a block of kp columns on its own stream.
+    // One session of a factor: a block of kp columns.
     struct block_lane {
         int which = 0;                                // the solver's session index
         int kp = 0;
-        hipStream_t stream = nullptr;
+        hipStream_t stream = nullptr;                 // the lane's block stream
         std::vector<std::shared_ptr<request>> slot;   // request in each column
         void* d_b = nullptr;                          // kp x n staging (device)
         void* d_x = nullptr;
@@ -110,6 +110,7 @@ private:
         std::thread worker;
         std::deque<std::shared_ptr<request>> queue;
         std::vector<block_lane> blocks;               // filled in order: the narrow block, then the main one
+        hipStream_t block_stream = nullptr;           // the blocks' stream
         std::shared_ptr<request> single;              // request on the single lane
         hipStream_t single_stream = nullptr;
         void* d_sb = nullptr;                         // single lane's b and x (device, n doubles each)
