@@ -12,7 +12,9 @@ test suite:
   in fp64 and mixed precision, with iteration counts within 2 of each other
   (the reference's +-2 bar between precisions);
 * a blocked solve of k right-hand sides gives each column the single solve's
-  convergence (true residual, iterations within 1)."""
+  convergence (true residual, iterations within 1);
+* the one-launch-per-sweep dataflow schedule gives the hybrid schedule's
+  bits (application and PCG solve)."""
 import numpy as np
 import pytest
 
@@ -104,3 +106,29 @@ def test_full_size_blocked_columns_converge_like_single_solves(gpu):
         single = s.solve(torch.tensor(B[c], device=gpu))
         assert abs(single.iterations - res[c].iterations) <= 1, (c, single.iterations, res[c].iterations)
     s.close()
+
+
+@pytest.mark.parametrize("name", ["sddm3d_128", "rmat_18"])
+def test_full_size_dataflow_schedule_gives_the_hybrid_bits(gpu, name):
+    """The one-launch-per-sweep dataflow schedule at full size, on its default
+    resident grid (2 workgroups per CU, the largest validated; larger grids
+    are refused): the mixed-precision preconditioner application and a mixed
+    PCG solve are the bits of the hybrid schedule's, since every schedule
+    runs the same per-row arithmetic."""
+    import torch
+    A, b, F = workload(name)
+    print(f"[case] full-size dataflow schedule {name}", flush=True)
+    out = {}
+    for sched in ("hybrid", "dataflow"):
+        s = ax.Solver(A, factor=F, precision="fp32", schedule=sched)
+        bd = torch.tensor(b, device=gpu)
+        z = s.apply(bd)
+        r = s.solve(bd)
+        torch.cuda.synchronize()
+        out[sched] = (z.view(torch.int64).cpu(), r.x.view(torch.int64).cpu(), r.iterations, r.status)
+        s.close()
+    (zh, xh, ih, sh), (zd, xd, idf, sd) = out["hybrid"], out["dataflow"]
+    assert sh == sd == 1
+    assert ih == idf
+    assert torch.equal(zh, zd)
+    assert torch.equal(xh, xd)
