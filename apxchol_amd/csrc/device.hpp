@@ -38,6 +38,7 @@ struct sweep_desc {
     const std::int32_t* idx = nullptr;      // fp16 storage: ids
     const std::uint16_t* h = nullptr;       // fp16 storage: values
     const float* diag = nullptr;            // m
+    const float* inv_diag = nullptr;        // m, fp32 1 / diag (the fp32-vector sweeps multiply by it)
     const float* inv_scale = nullptr;       // m, fp16 back sweep input scale (else null)
     const std::uint32_t* batch_pos = nullptr;  // nb+1
     const std::uint64_t* batch_mask = nullptr; // nb
@@ -115,7 +116,8 @@ struct alignas(16) tail_lane {
     std::int32_t row;    // output row the lane writes, or -1
     std::int32_t slot;   // the row's LDS slot (position minus the run's first)
     std::int32_t gc;     // G | (entries inside the run) << 8
-    float diag;          // the row's diagonal (1 for lanes that do not write)
+    float diag;          // the row's diagonal operand: diag (tail_desc::lane) or 1 / diag
+                         // (tail_desc::lane_recip); 1 for lanes that do not write
 };
 struct alignas(16) tail_pair {
     std::int32_t i0;
@@ -138,6 +140,7 @@ struct alignas(8) tail_walk {
 };
 struct tail_desc {
     const tail_lane* lane = nullptr;
+    const tail_lane* lane_recip = nullptr;   // the same lanes with 1 / diag (fp32-vector sweeps)
     const tail_pair* ent = nullptr;
     const std::int32_t* ovf = nullptr;
     const entry32* ovf_ent = nullptr;

@@ -388,9 +388,11 @@ int append_tail_run(const trsv_factor& T, const batch_table& bt, bool backward, 
             const bool writer = x.active && x.sub == 0;
             const int c = static_cast<int>(ch.size());
             if (c >= (1 << 22)) fail("row too long");
-            tt.lane.push_back({writer ? static_cast<std::int32_t>(x.row) : -1,
+            const tail_lane tl{writer ? static_cast<std::int32_t>(x.row) : -1,
                                writer ? static_cast<std::int32_t>(x.pos - p0) : 0, x.G | (c << 8),
-                               writer ? T.diag[x.row] : 1.0f});
+                               writer ? T.diag[x.row] : 1.0f};
+            tt.lane.push_back(tl);
+            tt.lane_recip.push_back({tl.row, tl.slot, tl.gc, writer ? 1.0f / T.diag[x.row] : 1.0f});
             maxc = std::max(maxc, c);
         }
         for (int k = 0; k < kTailPairs; ++k)
@@ -495,11 +497,12 @@ struct gpu_solver::device_state {
     vec_prec prec = vec_prec::fp64;
     bool fp16 = false;
     int grid = 0;
-    dbuf f_ptr, f_ent, f_idx, f_h, b_ptr, b_ent, b_idx, b_h, diag, inv_scale;
+    dbuf f_ptr, f_ent, f_idx, f_h, b_ptr, b_ent, b_idx, b_h, diag, inv_diag, inv_scale;
     dbuf f_pos, f_mask, b_pos, b_mask;
     std::vector<int> f_levels, b_levels;
     dbuf f_levels_dev, b_levels_dev, tail_val;
     dbuf f_tlane, f_tent, f_tovf, f_tovf_ent, f_twalk, b_tlane, b_tent, b_tovf, b_tovf_ent, b_twalk;
+    dbuf f_tlane_r, b_tlane_r;   // tail lanes with 1 / diag (fp32-vector sweeps)
     dbuf f_lrec, f_lent, b_lrec, b_lent;
     std::vector<int> f_level_lb, b_level_lb;   // first lane batch of each level, or -1
     std::vector<int> f_seg[7], b_seg[7];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch, barrier phases
@@ -746,6 +749,13 @@ void gpu_solver::setup(const csc_view& L) {
         d.b_ent.upload(be);
     }
     d.diag.upload(T_.diag);
+    {
+        // fp32 reciprocals of the diagonal: the fp32-vector sweeps multiply
+        // by them (the tail layout stores the same values).
+        std::vector<float> inv(T_.diag.size());
+        for (std::size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0f / T_.diag[i];
+        d.inv_diag.upload(inv);
+    }
     const batch_table fb = build_batches(T_, false), bb = build_batches(T_, true);
     validate_sweep_layout(T_, fb, false);
     validate_sweep_layout(T_, bb, true);
@@ -861,6 +871,7 @@ void gpu_solver::setup(const csc_view& L) {
             s.ent = (backward ? d.b_ent : d.f_ent).as<entry32>();
         }
         s.diag = d.diag.as<float>();
+        s.inv_diag = d.inv_diag.as<float>();
         s.batch_pos = (backward ? d.b_pos : d.f_pos).as<std::uint32_t>();
         s.batch_mask = (backward ? d.b_mask : d.f_mask).as<std::uint64_t>();
         s.nb = static_cast<int>(bt.mask.size());
@@ -908,7 +919,7 @@ void gpu_solver::setup(const csc_view& L) {
                          : opts_.lane_batches > 0 ? opts_.lane_batches
                                                   : env_int("APXCHOL_AMD_LANE_BATCHES", 4096);
     auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
-                          dbuf& lf_dev, std::vector<int> (&seg)[7], dbuf* const (&tbuf)[5], std::vector<int>& level_lb,
+                          dbuf& lf_dev, std::vector<int> (&seg)[7], dbuf* const (&tbuf)[6], std::vector<int>& level_lb,
                           dbuf& lrec, dbuf& lent) {
         lf_dev.upload(lf);
         tail_tables tt;
@@ -967,15 +978,18 @@ void gpu_solver::setup(const csc_view& L) {
             l = e;
         }
         const std::size_t tb = static_cast<std::size_t>(tt.batches);
-        if (tt.lane.size() != tb * kWave || tt.ent.size() != tb * kTailPairs * kWave || tt.ovf.size() != tb ||
-            tt.ovf_ent.size() % kWave != 0 || tt.walk.size() != tb)
+        if (tt.lane.size() != tb * kWave || tt.lane_recip.size() != tb * kWave ||
+            tt.ent.size() != tb * kTailPairs * kWave || tt.ovf.size() != tb || tt.ovf_ent.size() % kWave != 0 ||
+            tt.walk.size() != tb)
             throw std::runtime_error("apxchol_amd: inconsistent tail layout");
         tbuf[0]->upload(tt.lane);
         tbuf[1]->upload(tt.ent);
         tbuf[2]->upload(tt.ovf);
         tbuf[3]->upload(tt.ovf_ent);
         tbuf[4]->upload(tt.walk);
+        tbuf[5]->upload(tt.lane_recip);
         sc.tail.lane = tbuf[0]->as<tail_lane>();
+        sc.tail.lane_recip = tbuf[5]->as<tail_lane>();
         sc.tail.ent = tbuf[1]->as<tail_pair>();
         sc.tail.ovf = tbuf[2]->as<std::int32_t>();
         sc.tail.ovf_ent = tbuf[3]->as<entry32>();
@@ -1060,8 +1074,8 @@ void gpu_solver::setup(const csc_view& L) {
         sc.nseg = static_cast<int>(seg[0].size());
         sc.grid = d.grid;
     };
-    dbuf* f_tb[5] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk};
-    dbuf* b_tb[5] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk};
+    dbuf* f_tb[6] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk, &d.f_tlane_r};
+    dbuf* b_tb[6] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk, &d.b_tlane_r};
     make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb, d.f_level_lb, d.f_lrec, d.f_lent);
     make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb, d.b_level_lb, d.b_lrec, d.b_lent);
     // Coupling output, shared by the two directions (their sweeps never overlap).
@@ -1090,11 +1104,12 @@ void gpu_solver::setup(const csc_view& L) {
     stats_.spmv_format = d.use_sell ? 1 : 0;
     stats_.dataflow_grid = d.grid;
     std::uint64_t bytes = 0;
-    for (const dbuf* b : {&d.f_ptr, &d.f_ent, &d.f_idx, &d.f_h, &d.b_ptr, &d.b_ent, &d.b_idx, &d.b_h, &d.diag,
+    for (const dbuf* b : {&d.f_ptr, &d.f_ent, &d.f_idx, &d.f_h, &d.b_ptr, &d.b_ent, &d.b_idx, &d.b_h, &d.diag, &d.inv_diag,
                           &d.inv_scale, &d.f_pos, &d.f_mask, &d.b_pos, &d.b_mask, &d.perm, &d.op_ptr, &d.op_idx,
                           &d.op_val, &d.r, &d.x, &d.p, &d.ap, &d.rp, &d.y, &d.z, &d.xout, &d.f_lrec, &d.f_lent,
                           &d.b_lrec, &d.b_lent, &d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk,
-                          &d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk, &d.tail_val, &d.relabel, &d.sell_soff,
+                          &d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk, &d.f_tlane_r, &d.b_tlane_r,
+                          &d.tail_val, &d.relabel, &d.sell_soff,
                           &d.sell_idx, &d.sell_val, &d.sell_map, &d.sell_long})
         bytes += b->bytes();
     stats_.device_bytes = bytes;

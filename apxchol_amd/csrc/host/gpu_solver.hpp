@@ -300,6 +300,7 @@ void validate_block_batches(const trsv_factor& T, const block_batch_table& bt, b
 // Static tail layout of one direction (device.hpp's tail_desc), run by run.
 struct tail_tables {
     std::vector<tail_lane> lane;
+    std::vector<tail_lane> lane_recip;   // the same lanes with 1 / diag (fp32-vector sweeps)
     std::vector<tail_pair> ent;
     std::vector<std::int32_t> ovf;
     std::vector<entry32> ovf_ent;

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
     const bool writer = active && s == 0;
     const size_t o = static_cast<size_t>(row) * kp + c;
     const double rin = writer ? static_cast<double>(rhs[o]) : 0.0;
-    const float dg = writer ? d.diag[row] : 1.0f;
+    const float dg = writer ? diag_operand<TO>(d, row) : 1.0f;
     const double shift = (writer && !d.backward && st) ? st[c].shift : 0.0;
     const double sc = (writer && d.backward && d.inv_scale) ? static_cast<double>(d.inv_scale[row]) : 0.0;
     TO part = 0;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void sweep_block(sweep_desc d, const unsign
     if (!d.backward) r = static_cast<TO>(rin - shift);
     else if (d.inv_scale) r = static_cast<TO>(rin * (sc * sc));
     else r = static_cast<TO>(rin);
-    out[o] = (r - part) / static_cast<TO>(dg);
+    out[o] = finish_row<TO>(r - part, dg);
 }
 
 // One level of a blocked sweep with one thread per (row, column) (rows of at
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void sweep_block_rows(sweep_desc d, int p0,
     const size_t o = static_cast<size_t>(row) * kp + c;
     const double shift = (!d.backward && st) ? st[c].shift : 0.0;
     const double rin = static_cast<double>(rhs[o]);
-    const float dg = d.diag[row];
+    const float dg = diag_operand<TO>(d, row);
     const double sc = (d.backward && d.inv_scale) ? static_cast<double>(d.inv_scale[row]) : 0.0;
     const unsigned rb = d.ptr[row];
     const unsigned len = d.ptr[row + 1] - rb;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void sweep_block_rows(sweep_desc d, int p0,
     if (!d.backward) r = static_cast<TO>(rin - shift);
     else if (d.inv_scale) r = static_cast<TO>(rin * (sc * sc));
     else r = static_cast<TO>(rin);
-    out[o] = (r - v[0]) / static_cast<TO>(dg);
+    out[o] = finish_row<TO>(r - v[0], dg);
 }
 
 // A run of small levels of a blocked sweep, walked by ONE workgroup of
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kTailBlock) void sweep_block_tail(sweep_desc d, con
                 } else {
                     r = static_cast<TO>(rhs[o]);
                 }
-                const TO v = (r - part) / static_cast<TO>(d.diag[row]);
+                const TO v = finish_row<TO>(r - part, diag_operand<TO>(d, row));
                 out[o] = v;
                 lds[static_cast<size_t>(pos - p0) * kp + c] = v;
             }

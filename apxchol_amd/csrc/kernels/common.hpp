@@ -88,6 +88,23 @@ __device__ __forceinline__ T group_tree_sum(T part, int G) {
     return tree_step<1>(part, G);
 }
 
+// A sweep row's output from its numerator (right-hand side minus the row's
+// sum) and the row's diagonal operand: fp32 vectors multiply by the fp32
+// reciprocal of the diagonal (sweep_desc::inv_diag; one dependent multiply
+// instead of the dozen-instruction IEEE division, at most one more rounding),
+// fp64 vectors divide by the diagonal. Every sweep kernel forms its outputs
+// through these two, so all schedules and layouts keep the same bits.
+template <class AT>
+__device__ __forceinline__ float diag_operand(const sweep_desc& d, int row) {
+    if constexpr (sizeof(AT) == sizeof(float)) return d.inv_diag[row];
+    else return d.diag[row];
+}
+template <class AT>
+__device__ __forceinline__ AT finish_row(AT num, float operand) {
+    if constexpr (sizeof(AT) == sizeof(float)) return num * operand;
+    else return num / static_cast<AT>(operand);
+}
+
 // Sum over a workgroup of kBlock threads: per-wave butterfly, then wave 0..3
 // in order. Valid in thread 0 only. `sh` holds kWavesPerBlock * N values.
 template <int N>

@@ -216,7 +216,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         part = group_tree_sum(part, kWave);
         if (lane == 0) {
             const AT r = row_rhs<TI, AT>(d, rhs, row, shift);
-            publish<TO, WAIT>(out + row, (r - part) / static_cast<AT>(d.diag[row]));
+            publish<TO, WAIT>(out + row, finish_row<AT>(r - part, diag_operand<AT>(d, row)));
         }
         return ok;
     }
@@ -235,10 +235,10 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
         pend |= valid ? (1u << k) : 0u;
     }
     AT r = 0;
-    AT dg = 1;
+    float dg = 1.0f;
     if (active && sub == 0) {
         r = row_rhs<TI, AT>(d, rhs, row, shift);
-        dg = static_cast<AT>(d.diag[row]);
+        dg = diag_operand<AT>(d, row);
     }
     if constexpr (!WAIT) {
         // Dependencies complete: gather once, then the same per-row sum.
@@ -247,7 +247,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
 #pragma unroll
         for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
         part = group_tree_sum(part, G);
-        if (active && sub == 0) publish<TO, WAIT>(out + row, (r - part) / dg);
+        if (active && sub == 0) publish<TO, WAIT>(out + row, finish_row<AT>(r - part, dg));
         return true;
     }
     bool done = !active;
@@ -262,7 +262,7 @@ __device__ bool sweep_batch(const sweep_desc& d, int b, const TI* __restrict__ r
             for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(vv[k]), xv[k], part);
         part = group_tree_sum(part, G);
         if (go) {
-            if (sub == 0) publish<TO, WAIT>(out + row, (r - part) / dg);
+            if (sub == 0) publish<TO, WAIT>(out + row, finish_row<AT>(r - part, dg));
             done = true;
         }
         if (__ballot(!done) == 0) break;
@@ -368,16 +368,16 @@ __device__ __forceinline__ void sweep_lanes_body(int vb, const sweep_desc& d, in
         const int sub = static_cast<int>((rc[i].info >> 3) & 63u);
         const bool head = row >= 0 && sub == 0 && first + i < nlb;
         AT r = 0;
-        AT dg = 1;
+        float dg = 1.0f;
         if (head) {
             r = row_rhs<TI, AT>(d, rhs, row, shift);
-            dg = static_cast<AT>(d.diag[row]);
+            dg = diag_operand<AT>(d, row);
         }
         AT part = 0;
 #pragma unroll
         for (int k = 0; k < kEpl; ++k) part = fma(static_cast<AT>(e[i][k].val), xv[i][k], part);
         part = group_tree_sum(part, G);
-        if (head) out[row] = (r - part) / dg;
+        if (head) out[row] = finish_row<AT>(r - part, dg);
     }
 }
 template <class TI, class TO, bool F16, int BS = kBlock, int BPW = 1>
@@ -530,7 +530,8 @@ __device__ __forceinline__ void tail_fetch(const tail_desc& td, const tail_val<T
                                            tail_item<TO>& it) {
     const size_t i = static_cast<size_t>(t) * kWave + (threadIdx.x & (kWave - 1));
     const i32x4* ent = reinterpret_cast<const i32x4*>(td.ent);
-    it.ln = reinterpret_cast<const i32x4*>(td.lane)[i];
+    // fp32 outputs take the lanes whose diagonal operand is 1 / diag (finish_row).
+    it.ln = reinterpret_cast<const i32x4*>(sizeof(TO) == sizeof(float) ? td.lane_recip : td.lane)[i];
     it.e0 = ent[tail_at(t, 0)];
     it.e1 = ent[tail_at(t, 1)];
     it.e2 = ent[tail_at(t, 2)];
@@ -592,7 +593,7 @@ __device__ __forceinline__ void tail_step(const tail_desc& td, int t, const tail
         }
     }
     part = group_tree_sum(part, G);
-    if (it.ln.x >= 0) lds[it.ln.y] = (it.v.y - part) / static_cast<TO>(as_float(it.ln.w));
+    if (it.ln.x >= 0) lds[it.ln.y] = finish_row<TO>(it.v.y - part, as_float(it.ln.w));
 }
 
 // base: the run's first tail batch; nb: its batches; nlev: its barrier
