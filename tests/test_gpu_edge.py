@@ -6,7 +6,8 @@ projected per component), each in fp64 and mixed precision.
 Bars: every solve converges (status 1) to a true residual ||b - A x|| / ||b||
 <= 1e-8 (a zero right-hand side returns status 5 and x = 0); the fp64
 preconditioner application equals the oracle's to 1e-12; blocked solves and
-a serving request give each column the convergence of its own solve."""
+serving requests converge the same way; a grouped solve over all of them
+gives every member the bits of its own solve."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -94,3 +95,30 @@ def test_degenerate_factors_serve(gpu):
         assert r.status == 1, (name, r.status)
         assert true_res(A, b, x) <= 2e-8, (name, true_res(A, b, x))
     srv.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_degenerate_factors_in_one_group(gpu, precision):
+    """A grouped solve over the degenerate systems gives every member the bits
+    of its own solve."""
+    import torch
+    print(f"[case] degenerate group {precision}", flush=True)
+    solvers, bs, mats = [], [], []
+    for i, make in enumerate(CASES.values()):
+        A = make()
+        F = ax.factorize(A, seed=0)
+        solvers.append(ax.Solver(A, factor=F, precision=precision))
+        bs.append(torch.tensor(rhs(A, F, 40 + i), device=gpu))
+        mats.append(A)
+    g = ax.Group(solvers)
+    xs, rs = g.solve(bs, tol=1e-8, maxiter=200)
+    torch.cuda.synchronize()
+    for i, s in enumerate(solvers):
+        own = s.solve(bs[i], tol=1e-8, maxiter=200)
+        torch.cuda.synchronize()
+        assert (rs[i].status, rs[i].iterations) == (own.status, own.iterations), i
+        assert torch.equal(xs[i].view(torch.int64), own.x.view(torch.int64)), i
+        assert true_res(mats[i], bs[i].cpu().numpy(), xs[i].cpu().numpy()) <= 1.05e-8, i
+    g.close()
+    for s in solvers:
+        s.close()
