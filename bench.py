@@ -288,6 +288,58 @@ def run_batched(args, ax, mg, np, torch, dev, name, A, F, plain_rate, seed0) -> 
                          "block_width": tune["block_width"]}}
 
 
+# The reference's own published result cells for unit 3-D grid Laplacians
+# (results/cells/grids/grid3d_{side}__apxchol_v1__bg_tree_vec_pool___t16__{cpu,gpu}.json):
+# iterations and solve time (wall minus setup) of its GPU path (CUDA dataflow
+# sweeps, RTX 4090 Laptop GPU) and its CPU path (16 threads, Ryzen 9 7945HX).
+REFERENCE_CELLS = {
+    200: {"gpu": {"iterations": 30, "solve_s": 0.7292422}, "cpu": {"iterations": 31, "solve_s": 4.493158}},
+    250: {"gpu": {"iterations": 32, "solve_s": 1.460878}, "cpu": {"iterations": 31, "solve_s": 10.11121}},
+}
+
+
+def run_reference_cell(args, ax, mg, np, torch, dev, side: int) -> dict:
+    """A reference result cell's workload: the unit 3-D grid Laplacian of the
+    given side with the reference benchmark's right-hand side (mt19937_64
+    normal draws, b = L g, centred, normalised) and factor seed 42, solved to
+    1e-8 in mixed precision and in fp64 (median of 3 timed solves after a
+    warm-up), beside the cell's published iterations and solve time."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import refcells
+    t0 = time.perf_counter()
+    print(f"[bench] reference cell grid3d_{side}: generation and host factorization start", file=sys.stderr,
+          flush=True)
+    A = mg.grid_laplacian((side, side, side))
+    b = refcells.make_rhs(A)
+    F = ax.factorize(A, seed=refcells.FACTOR_SEED)
+    print(f"[bench] reference cell grid3d_{side}: factor ready after {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    bd = torch.tensor(b, device=dev)
+    out = {"n": int(A.shape[0]), "published": REFERENCE_CELLS.get(side)}
+    for prec in ("fp32", "fp64"):
+        s = ax.Solver(A, factor=F, precision=prec)
+        x = torch.empty_like(bd)
+        s.solve(bd, tol=1e-8, maxiter=500, out=x)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            r = s.solve(bd, tol=1e-8, maxiter=500, out=x)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t1)
+        s.close()
+        xs = x.cpu().numpy()
+        key = "mixed" if prec == "fp32" else "fp64"
+        out[key] = {"solve_s": float(np.median(ts)), "iterations": r.iterations,
+                    "graded_residual": refcells.graded_residual(A, b, xs)}
+        print(f"[bench] reference cell grid3d_{side} {key}: {1e3 * out[key]['solve_s']:.1f} ms, "
+              f"{r.iterations} iterations", file=sys.stderr, flush=True)
+    pub = REFERENCE_CELLS.get(side)
+    if pub:
+        out["mixed_speedup_vs_published_gpu"] = pub["gpu"]["solve_s"] / out["mixed"]["solve_s"]
+    return out
+
+
 def run_second_workload(args, ax, mg, np, torch, dev) -> dict:
     """The R-MAT configuration (default scale 22, mean-projected per component
     N(0,1) right-hand side, tol 1e-8): single solves in mixed precision and
@@ -410,6 +462,8 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=256, help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serving-qps", type=float, default=25.0)
     ap.add_argument("--serving-slots", type=int, default=32)
+    ap.add_argument("--reference-cell", type=int, default=200,
+                    help="side of the unit 3-D grid of a reference result cell to solve (0: skip)")
     ap.add_argument("--serving-narrow", type=int, default=4, help="columns of each factor's narrow block")
     ap.add_argument("--serving-capacity-reps", type=int, default=4,
                     help="times the request set is submitted at once for the capacity pass")
@@ -541,6 +595,10 @@ def main() -> int:
     second = None
     if args.second_workload:
         second = run_second_workload(args, ax, mg, np, torch, dev)
+    cells = None
+    if args.reference_cell > 0:
+        cells = {f"grid3d_{args.reference_cell}": run_reference_cell(args, ax, mg, np, torch, dev,
+                                                                     args.reference_cell)}
     group = None if args.group_reps <= 0 else {w: run_group(args, ax, mg, np, torch, dev, w)
                                                for w in ("serving", "grids16")}
 
@@ -616,6 +674,7 @@ def main() -> int:
             "mixed_over_fp64": None if not plain else value / (args.steps / plain["total"]),
             "batched_mixed": batched,
             "second_workload": None if second is None else {args.second_workload: second},
+            "reference_cell": cells,
             "group_small_factors": group,
             "serving": serving,
             "kernels_mixed": per_kernel,
