@@ -178,7 +178,7 @@ def test_describe_and_diagnose():
     d = mtx.describe_input("adjacency", mtx.scan_input(grid_adjacency(4, 4)), "r")
     assert "ADJACENCY" in d and "L = D - A" in d
     d = mtx.describe_input("laplacian", mtx.scan_input(grid_laplacian(4, 4)), "r")
-    assert "LAPLACIAN/SDDM" in d and "pure Laplacian" in d
+    assert "LAPLACIAN/SDDM" in d and "graph Laplacian: all row sums are zero" in d
     assert mtx.diagnose_failure("laplacian", mtx.scan_input(grid_laplacian(4, 4))) == ""
     assert "--input-kind adjacency" in mtx.diagnose_failure("laplacian", mtx.scan_input(grid_adjacency(4, 4)))
 
