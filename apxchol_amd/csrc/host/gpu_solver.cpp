@@ -505,6 +505,8 @@ struct gpu_solver::device_state {
     dbuf f_tlane_r, b_tlane_r;   // tail lanes with 1 / diag (fp32-vector sweeps)
     dbuf f_lrec, f_lent, b_lrec, b_lent;
     std::vector<int> f_level_lb, b_level_lb;   // first lane batch of each level, or -1
+    std::vector<int> f_level_pos, b_level_pos;   // first sweep position of each level
+    std::vector<unsigned char> f_level_diag, b_level_diag;   // levels without entries
     std::vector<int> f_seg[7], b_seg[7];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch, barrier phases
     sweep_sched f_sched, b_sched;
     dbuf perm, op_ptr, op_idx, op_val;
@@ -920,7 +922,7 @@ void gpu_solver::setup(const csc_view& L) {
                                                   : env_int("APXCHOL_AMD_LANE_BATCHES", 4096);
     auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
                           dbuf& lf_dev, std::vector<int> (&seg)[7], dbuf* const (&tbuf)[6], std::vector<int>& level_lb,
-                          dbuf& lrec, dbuf& lent) {
+                          dbuf& lrec, dbuf& lent, std::vector<int>& level_pos, std::vector<unsigned char>& level_diag) {
         lf_dev.upload(lf);
         tail_tables tt;
         const int L = static_cast<int>(lf.size()) - 1;
@@ -1066,6 +1068,25 @@ void gpu_solver::setup(const csc_view& L) {
         sc.level_first = lf.data();
         sc.level_first_dev = lf_dev.as<int>();
         sc.nlevels = L;
+        // Levels none of whose rows has an entry (the forward sweep's first
+        // level) are launched as one load per row (sweep_diag_level, the same
+        // bits); APXCHOL_AMD_DIAG_LEVELS=0 keeps them on the batch kernels.
+        {
+            const auto& ptr = backward ? T_.b_ptr : T_.f_ptr;
+            level_pos.resize(static_cast<std::size_t>(L) + 1);
+            for (int q = 0; q <= L; ++q) level_pos[q] = pos(q);
+            level_diag.assign(static_cast<std::size_t>(L), 0);
+            for (int q = 0; q < L; ++q) {
+                bool none = pos(q + 1) > pos(q);
+                for (int t = pos(q); t < pos(q + 1) && none; ++t) {
+                    const int row = backward ? m - 1 - t : t;
+                    none = ptr[row + 1] == ptr[row];
+                }
+                level_diag[q] = none ? 1 : 0;
+            }
+            sc.level_pos = level_pos.data();
+            sc.level_diag = env_flag("APXCHOL_AMD_DIAG_LEVELS", 1) != 0 ? level_diag.data() : nullptr;
+        }
         sc.seg_l0 = seg[0].data();
         sc.seg_l1 = seg[1].data();
         sc.seg_tail = seg[2].data();
@@ -1076,8 +1097,10 @@ void gpu_solver::setup(const csc_view& L) {
     };
     dbuf* f_tb[6] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk, &d.f_tlane_r};
     dbuf* b_tb[6] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk, &d.b_tlane_r};
-    make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb, d.f_level_lb, d.f_lrec, d.f_lent);
-    make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb, d.b_level_lb, d.b_lrec, d.b_lent);
+    make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb, d.f_level_lb, d.f_lrec, d.f_lent,
+               d.f_level_pos, d.f_level_diag);
+    make_sched(d.b_sched, bb, true, d.b_levels, d.b_levels_dev, d.b_seg, b_tb, d.b_level_lb, d.b_lrec, d.b_lent,
+               d.b_level_pos, d.b_level_diag);
     // Coupling output, shared by the two directions (their sweeps never overlap).
     const std::size_t tail_batches = std::max<std::size_t>(
         1, static_cast<std::size_t>(std::max(d.f_sched.tail_batches, d.b_sched.tail_batches)));
@@ -1440,18 +1463,23 @@ const block_sched* gpu_solver::block_scheds(int kp, int which) const {
             for (std::size_t l = 0; l <= L; ++l)
                 lpos[l] = static_cast<int>(backward ? T_.m - T_.level_ptr[L - l] : T_.level_ptr[l]);
             // Lanes of each level's widest row (the per-(row, column) level
-            // kernel takes levels up to 16; 0 marks a row past 64 lanes).
+            // kernel takes levels up to 16; 0 marks a row past 64 lanes, -1
+            // a level without any entry, run as one load per (row, column)
+            // unless APXCHOL_AMD_DIAG_LEVELS=0).
             std::vector<int>& lg = backward ? t->b_lgmax : t->f_lgmax;
             lg.assign(L, 0);
             const auto& rptr = backward ? T_.b_ptr : T_.f_ptr;
+            const bool diag_levels = env_flag("APXCHOL_AMD_DIAG_LEVELS", 1) != 0;
             for (std::size_t l = 0; l < L; ++l) {
                 int gm = 1;
+                std::uint64_t entries = 0;
                 for (int q = lpos[l]; q < lpos[l + 1]; ++q) {
                     const std::uint32_t row = backward ? T_.m - 1 - static_cast<std::uint32_t>(q) : static_cast<std::uint32_t>(q);
                     const std::uint32_t len = rptr[row + 1] - rptr[row];
                     gm = len > static_cast<std::uint32_t>(kWave * kEpl) ? 1 << 30 : std::max(gm, lanes_for_host(len));
+                    entries += len;
                 }
-                lg[l] = gm > kWave ? 0 : gm;
+                lg[l] = (diag_levels && entries == 0 && lpos[l + 1] > lpos[l]) ? -1 : gm > kWave ? 0 : gm;
             }
             // Segments: runs of levels of at most block_tail_waves table
             // waves whose rows fit the LDS go to one workgroup; the others

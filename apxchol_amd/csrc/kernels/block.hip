@@ -165,6 +165,33 @@ __global__ __launch_bounds__(kBlock) void sweep_block_rows(sweep_desc d, int p0,
     out[o] = finish_row<TO>(r - v[0], dg);
 }
 
+// A blocked level none of whose rows has an entry: every (row, column) sum is
+// the empty chain's +0, so the output is finish_row(rhs - 0) --
+// sweep_block_rows' bits without its row pointers and padding entries.
+// Thread t: sweep position p0 + t / kp, column t % kp, grid-strided.
+template <class TI, class TO>
+__global__ __launch_bounds__(kBlock) void sweep_block_diag(sweep_desc d, int p0, int p1, int kp,
+                                                           const TI* __restrict__ rhs, TO* out, const pcg_state* st,
+                                                           const batch_state* bs) {
+    if (bs && bs->active == 0) return;
+    const long long total = static_cast<long long>(p1 - p0) * kp;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; t < total; t += stride) {
+        const int pos = p0 + static_cast<int>(t / kp);
+        const int c = static_cast<int>(t & (kp - 1));
+        const int row = d.backward ? d.m - 1 - pos : pos;
+        const size_t o = static_cast<size_t>(row) * kp + c;
+        const double rin = static_cast<double>(rhs[o]);
+        TO r;
+        if (!d.backward) r = static_cast<TO>(rin - ((st) ? st[c].shift : 0.0));
+        else if (d.inv_scale) {
+            const double sc = static_cast<double>(d.inv_scale[row]);
+            r = static_cast<TO>(rin * (sc * sc));
+        } else r = static_cast<TO>(rin);
+        out[o] = finish_row<TO>(r - static_cast<TO>(0), diag_operand<TO>(d, row));
+    }
+}
+
 // A run of small levels of a blocked sweep, walked by ONE workgroup of
 // kTailBlock threads: levels [l0, l1) of the wave table (each at most
 // kBlockTailMaxWaves table waves; wave w of the workgroup takes table waves
@@ -446,6 +473,111 @@ __global__ __launch_bounds__(kBlock) void spmm_sell_k(sell_desc a, int kp, const
             // waves sit in column-dependent workgroups).
             if (lane == 0) y[static_cast<size_t>(row) * kp + col] = s;
         }
+    }
+    if (!want_dot) return;
+    double v[1] = {acc};
+    col_block_partials<1>(v, kp, part, sh);
+}
+
+// Entries per row of one chunk of a slice staged in LDS by spmm_sell_staged_k.
+inline constexpr int kSpmmChunk = 16;
+
+// spmm_sell_k for kp >= 4 with each slice's entries read from HBM once per
+// slice instead of once per item: a workgroup takes whole slices of its XCD
+// band, copies kSpmmChunk entries per row (indices and values, coalesced) to
+// LDS, and its four waves take the slice's kp items (wave w: items w, w + 4,
+// ...; item q covers rows [q 64/kp, (q+1) 64/kp) for all kp columns, as in
+// spmm_sell_k), reading the entries from LDS. An item's sums run over the
+// same entries in the same order as spmm_sell_k's, so every column still gets
+// spmv_sell's bits for its rows; the p.Ap partials are one per workgroup and
+// grouped the same way for every column, so a column's total does not depend
+// on its slot. The band's long rows follow, one wave per (row, column).
+template <class VT>
+__global__ __launch_bounds__(kBlock) void spmm_sell_staged_k(sell_desc a, int kp, const double* __restrict__ x,
+                                                             double* __restrict__ y, const batch_state* bs,
+                                                             double* part, int want_dot) {
+    if (bs->active == 0) return;
+    __shared__ double sh[kBlock];
+    __shared__ int s_idx[kSpmmChunk * kWave];
+    __shared__ VT s_val[kSpmmChunk * kWave];
+    constexpr int kMaxItems = kWave / kWavesPerBlock;   // items per wave at kp = 64
+    const int tid = static_cast<int>(threadIdx.x);
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
+    const int c = lane & (kp - 1);
+    const int nq = kp / kWavesPerBlock;
+    const int rpi = kWave / kp;
+    const int band = static_cast<int>(blockIdx.x) % kXcds;
+    const int per = static_cast<int>(gridDim.x) / kXcds;
+    const int slot = static_cast<int>(blockIdx.x) / kXcds;
+    const int sl0 = static_cast<int>(static_cast<long long>(a.nslices) * band / kXcds);
+    const int sl1 = static_cast<int>(static_cast<long long>(a.nslices) * (band + 1) / kXcds);
+    const int* __restrict__ sidx = a.idx;
+    const VT* __restrict__ sval = static_cast<const VT*>(a.val);
+    double acc = 0.0;
+    for (int sl = sl0 + slot; sl < sl1; sl += per) {
+        const unsigned c0 = a.soff[sl], c1 = a.soff[sl + 1];
+        const int w = static_cast<int>(c1 - c0);
+        double s[kMaxItems];
+#pragma unroll
+        for (int i = 0; i < kMaxItems; ++i) s[i] = 0.0;
+        for (int k0 = 0; k0 < w; k0 += kSpmmChunk) {
+            const int cw = w - k0 < kSpmmChunk ? w - k0 : kSpmmChunk;
+            const size_t g0 = static_cast<size_t>(c0 + static_cast<unsigned>(k0)) * kWave;
+            for (int e = tid; e < cw * kWave; e += kBlock) {
+                s_idx[e] = sidx[g0 + static_cast<size_t>(e)];
+                s_val[e] = sval[g0 + static_cast<size_t>(e)];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kMaxItems; ++i) {
+                if (i >= nq) break;
+                const int l = (wv + i * kWavesPerBlock) * rpi + lane / kp;
+                double t = s[i];
+                for (int kk = 0; kk < cw; kk += 8) {
+                    int j[8];
+                    double v[8], g[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const bool ok = kk + u < cw;
+                        j[u] = ok ? s_idx[(kk + u) * kWave + l] : 0;
+                        v[u] = ok ? static_cast<double>(s_val[(kk + u) * kWave + l]) : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) g[u] = kk + u < cw ? x[static_cast<size_t>(j[u]) * kp + c] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (kk + u < cw) t = fma(v[u], g[u], t);
+                }
+                s[i] = t;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < kMaxItems; ++i) {
+            if (i >= nq) break;
+            const int l = (wv + i * kWavesPerBlock) * rpi + lane / kp;
+            const int row = a.rowmap[static_cast<size_t>(sl) * kWave + l];
+            if (row >= 0 && row < a.n) {
+                const size_t o = static_cast<size_t>(row) * kp + c;
+                y[o] = s[i];
+                acc = fma(x[o], s[i], acc);
+            }
+        }
+    }
+    const long long li0 = static_cast<long long>(a.nlong) * kp * band / kXcds;
+    const long long li1 = static_cast<long long>(a.nlong) * kp * (band + 1) / kXcds;
+    for (long long li = li0 + static_cast<long long>(slot) * kWavesPerBlock + wv; li < li1;
+         li += static_cast<long long>(per) * kWavesPerBlock) {
+        const int row = a.longrows[li / kp];
+        const int col = static_cast<int>(li % kp);
+        const unsigned b = a.ptr[row], e = a.ptr[row + 1];
+        const VT* __restrict__ cval = static_cast<const VT*>(a.cval);
+        double s = 0.0;
+        for (unsigned p = b + lane; p < e; p += kWave)
+            s = fma(static_cast<double>(cval[p]), x[static_cast<size_t>(a.cidx[p]) * kp + col], s);
+        s = wave_sum(s);
+        if (lane == 0) y[static_cast<size_t>(row) * kp + col] = s;
     }
     if (!want_dot) return;
     double v[1] = {acc};

@@ -69,6 +69,13 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
     auto level_launch = [&](int l) {
         const int b0 = sc.level_first[l], b1 = sc.level_first[l + 1];
         if (b1 <= b0) return;
+        if (sc.level_diag && sc.level_diag[l]) {
+            // No entries in the level: one load per row.
+            const int p0 = sc.level_pos[l], p1 = sc.level_pos[l + 1];
+            launch_timed(hook, 0, sweep_diag_level<TI, TO>, dim3(vec_blocks(p1 - p0)), dim3(kBlock), 0, s, d, p0, p1,
+                         in, o, st);
+            return;
+        }
         int g = (b1 - b0 + kWavesPerBlock - 1) / kWavesPerBlock;
         const int lb = sc.level_lb ? sc.level_lb[l] : -1;
         if (lb >= 0) {
@@ -341,6 +348,13 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
         const int w0 = b.level_first[l], w1 = b.level_first[l + 1];
         if (w1 <= w0) return;
         const int gmax = b.level_gmax ? b.level_gmax[l] : 0;
+        if (gmax < 0) {
+            // No entries in the level: one load per (row, column).
+            const int p0 = b.level_pos[l], p1 = b.level_pos[l + 1];
+            sweep_block_diag<TI, TO><<<vec_blocks(static_cast<long long>(p1 - p0) * b.kp), kBlock, 0, s>>>(
+                d, p0, p1, b.kp, static_cast<const TI*>(rhs), static_cast<TO*>(out), st, bs);
+            return;
+        }
         if (gmax >= 1 && gmax <= b.rows_max_g) {
             // One thread per (row, column).
             const int p0 = b.level_pos[l], p1 = b.level_pos[l + 1];
@@ -478,9 +492,21 @@ void launch_spmm_sell_k(const sell_desc& a, bool val32, int kp, const double* x,
     const int w = want_dot ? 1 : 0;
     const dim3 grid(static_cast<unsigned>(g));
     if (!a.rowmap) throw std::invalid_argument("apxchol_amd: sliced ELL without a row map");
-    if (val32) spmm_sell_k<float><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
-    else spmm_sell_k<double><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
-    check_launch("spmm_sell_k");
+    // From 4 columns on, a workgroup's four waves share each slice's entries
+    // through LDS (APXCHOL_AMD_SPMM_STAGED=0: every wave loads them itself).
+    static const bool staged = [] {
+        const char* e = std::getenv("APXCHOL_AMD_SPMM_STAGED");
+        return !(e && *e && std::atoi(e) == 0);
+    }();
+    if (staged && kp >= kWavesPerBlock) {
+        if (val32) spmm_sell_staged_k<float><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+        else spmm_sell_staged_k<double><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+        check_launch("spmm_sell_staged_k");
+    } else {
+        if (val32) spmm_sell_k<float><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+        else spmm_sell_k<double><<<grid, kBlock, 0, s>>>(a, kp, x, y, bs, part, w);
+        check_launch("spmm_sell_k");
+    }
     if (!want_dot) return;
     int nb = static_cast<int>(g);
     if (a.nlong > 0) {

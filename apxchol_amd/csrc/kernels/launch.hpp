@@ -43,6 +43,8 @@ struct sweep_sched {
     void* tail_val = nullptr;              // coupling output, tail_val<T> per (tail batch, lane) (device)
     size_t tail_val_bytes = 0;
     const int* level_lb = nullptr;         // host: first lane batch of a launched level, or -1 (batch table)
+    const int* level_pos = nullptr;        // host: first sweep position of each level (nlevels + 1)
+    const unsigned char* level_diag = nullptr;  // host: 1 when no row of the level has an entry
     const lane_rec* lane_rec_dev = nullptr;  // static lane layout of the launched levels (device)
     const entry32* lane_ent_dev = nullptr;
     long long lane_batches = 0;

@@ -324,6 +324,24 @@ __global__ __launch_bounds__(kBlock) void sweep_levels(sweep_desc d, int b0, int
                                    st, err);
 }
 
+// A launched level none of whose rows has an entry (the first forward level:
+// the rows that depend on nothing): a row's sum is then the empty chain's +0,
+// so its output is finish_row(rhs - 0) -- sweep_batch's bits for it, read
+// with one load per row instead of the batch tables, row pointers, padding
+// entries and gathers. Sweep positions [p0, p1), grid-strided.
+template <class TI, class TO>
+__global__ __launch_bounds__(kBlock) void sweep_diag_level(sweep_desc d, int p0, int p1, const TI* __restrict__ rhs,
+                                                           TO* out, const pcg_state* st) {
+    if (st && st->done) return;
+    const double shift = (st && !d.backward) ? st->shift : 0.0;
+    const int stride = static_cast<int>(gridDim.x) * kBlock;
+    for (int t = p0 + static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x); t < p1; t += stride) {
+        const int row = d.backward ? d.m - 1 - t : t;
+        const TO r = row_rhs<TI, TO>(d, rhs, row, shift);
+        out[row] = finish_row<TO>(r - static_cast<TO>(0), diag_operand<TO>(d, row));
+    }
+}
+
 // One launched level from the static lane layout: wave w runs lane batches
 // lb0 + w * BPW .. + BPW - 1 (BPW batches per wave: all their layout loads,
 // then all their gathers, are in flight together, and a big level needs BPW
