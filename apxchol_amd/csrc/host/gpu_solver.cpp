@@ -507,7 +507,12 @@ struct gpu_solver::device_state {
     std::vector<int> f_level_lb, b_level_lb;   // first lane batch of each level, or -1
     std::vector<int> f_level_pos, b_level_pos;   // first sweep position of each level
     std::vector<unsigned char> f_level_diag, b_level_diag;   // levels without entries
-    std::vector<int> f_seg[7], b_seg[7];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch, barrier phases
+    std::vector<int> f_seg[8], b_seg[8];   // hybrid segments: l0, l1, tail flag, p0, p1, first tail batch, barrier
+                                           // phases, coupling split position (-1: none)
+    // Side stream and fork / join events of the split couplings (both sweep
+    // directions; their sweeps never overlap).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     sweep_sched f_sched, b_sched;
     dbuf perm, op_ptr, op_idx, op_val;
     bool op32 = false;
@@ -569,6 +574,9 @@ struct gpu_solver::device_state {
         if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
         if (iter_graph) (void)hipGraphDestroy(iter_graph);
         if (cap) (void)hipStreamDestroy(cap);
+        if (side) (void)hipStreamDestroy(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         if (host_st) (void)hipHostFree(host_st);
         if (host_err) (void)hipHostFree(host_err);
     }
@@ -921,7 +929,7 @@ void gpu_solver::setup(const csc_view& L) {
                          : opts_.lane_batches > 0 ? opts_.lane_batches
                                                   : env_int("APXCHOL_AMD_LANE_BATCHES", 4096);
     auto make_sched = [&](sweep_sched& sc, const batch_table& bt, bool backward, const std::vector<int>& lf,
-                          dbuf& lf_dev, std::vector<int> (&seg)[7], dbuf* const (&tbuf)[6], std::vector<int>& level_lb,
+                          dbuf& lf_dev, std::vector<int> (&seg)[8], dbuf* const (&tbuf)[6], std::vector<int>& level_lb,
                           dbuf& lrec, dbuf& lent, std::vector<int>& level_pos, std::vector<unsigned char>& level_diag) {
         lf_dev.upload(lf);
         tail_tables tt;
@@ -999,6 +1007,35 @@ void gpu_solver::setup(const csc_view& L) {
         sc.tail_batches = tt.batches;
         sc.seg_base = seg[5].data();
         sc.seg_phases = seg[6].data();
+        // A tail run right after another tail run sums its entries before
+        // that run on the side stream while that run's tail walks (one
+        // workgroup: the rest of the device is idle), and only the entries
+        // into it afterwards -- the same chains (APXCHOL_AMD_COUPLE_SPLIT=0:
+        // one coupling launch per run).
+        // Only a coupling of at least split_min entries before that run is
+        // split: a light one costs less than the extra launch and the join
+        // (APXCHOL_AMD_COUPLE_SPLIT_MIN, entries).
+        seg[7].assign(seg[0].size(), -1);
+        if (opts_.schedule == sweep_schedule::hybrid && env_flag("APXCHOL_AMD_COUPLE_SPLIT", 1) != 0) {
+            const std::uint64_t split_min =
+                static_cast<std::uint64_t>(std::max(env_flag("APXCHOL_AMD_COUPLE_SPLIT_MIN", 1 << 20), 0));
+            for (std::size_t k = 1; k < seg[0].size(); ++k) {
+                if (!seg[2][k] || !seg[2][k - 1] || seg[3][k - 1] <= 0) continue;
+                const int ps = seg[3][k - 1];
+                std::uint64_t before = 0;
+                for (int t = seg[3][k]; t < seg[4][k]; ++t) {
+                    const int row = backward ? m - 1 - t : t;
+                    const auto b0 = sidx.begin() + sptr[row], b1 = sidx.begin() + sptr[row + 1];
+                    before += static_cast<std::uint64_t>(
+                        std::partition_point(b0, b1, [&](std::int32_t j) { return (backward ? m - 1 - j : j) < ps; }) - b0);
+                }
+                if (before >= split_min) seg[7][k] = ps;
+            }
+        }
+        sc.seg_split = seg[7].data();
+        sc.side = d.side;
+        sc.ev_fork = d.ev_fork;
+        sc.ev_join = d.ev_join;
         sc.kind = static_cast<int>(opts_.schedule == sweep_schedule::dataflow ? 0
                                    : opts_.schedule == sweep_schedule::levels ? 1 : 2);
         // Static lane layout of the launched levels.
@@ -1095,6 +1132,9 @@ void gpu_solver::setup(const csc_view& L) {
         sc.nseg = static_cast<int>(seg[0].size());
         sc.grid = d.grid;
     };
+    hip_check(hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&d.ev_fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&d.ev_join, hipEventDisableTiming), "hipEventCreate");
     dbuf* f_tb[6] = {&d.f_tlane, &d.f_tent, &d.f_tovf, &d.f_tovf_ent, &d.f_twalk, &d.f_tlane_r};
     dbuf* b_tb[6] = {&d.b_tlane, &d.b_tent, &d.b_tovf, &d.b_tovf_ent, &d.b_twalk, &d.b_tlane_r};
     make_sched(d.f_sched, fb, false, d.f_levels, d.f_levels_dev, d.f_seg, f_tb, d.f_level_lb, d.f_lrec, d.f_lent,

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kWave) void group_sweep_couple(const group_unit* __
     const int b = static_cast<int>(blockIdx.x);
     const int i = group_unit_of(starts, count, b);
     const group_unit& u = units[i];
-    sweep_couple_body<TI, TO, F16, kWave>(b - starts[i], 0, u.d, u.b0, u.b1, u.p0, static_cast<const TI*>(u.rhs),
+    sweep_couple_body<TI, TO, F16, kWave>(b - starts[i], 0, u.d, u.b0, u.b1, 0, u.p0, 0, static_cast<const TI*>(u.rhs),
                                           static_cast<const TO*>(u.out), static_cast<tail_val<TO>*>(u.tv), u.st, 1,
                                           nullptr);
 }

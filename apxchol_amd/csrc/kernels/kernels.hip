@@ -45,6 +45,10 @@ void check_launch(const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("apxchol_amd: launch of ") + what + " failed: " + hipGetErrorString(e));
 }
 
+void hip_check_launch(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("apxchol_amd: ") + what + " failed: " + hipGetErrorString(e));
+}
+
 template <class TI, class TO, bool F16>
 int dataflow_grid_for() {
     int dev = 0;
@@ -107,34 +111,66 @@ void sweep_t(const sweep_desc& d, const void* rhs, void* out, const pcg_state* s
         for (int l = 0; l < sc.nlevels; ++l) level_launch(l);
         check_launch("sweep_levels");
     } else {
+        static const bool attr = [] {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TO>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
+            return true;
+        }();
+        (void)attr;
+        // Tail run k's batches [bf, bl) and its coupling values.
+        auto run_of = [&](int k, int& bf, int& bl, tail_val<TO>*& tv) {
+            bf = sc.level_first[sc.seg_l0[k]];
+            bl = sc.level_first[sc.seg_l1[k]];
+            const int base = sc.seg_base[k];
+            if (bl <= bf || base < 0 || base + (bl - bf) > sc.tail_batches ||
+                static_cast<size_t>(sc.tail_batches) * kWave * sizeof(tail_val<TO>) > sc.tail_val_bytes)
+                throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
+            tv = static_cast<tail_val<TO>*>(sc.tail_val) + static_cast<size_t>(base) * kWave;
+        };
+        const bool can_split = sc.seg_split && sc.side && sc.ev_fork && sc.ev_join;
+        const auto split_at = [&](int k) { return can_split && k < sc.nseg && sc.seg_tail[k] ? sc.seg_split[k] : -1; };
+        // One wave per workgroup (a batch each), as the lane levels.
+        const auto couple = [&](hipStream_t on, int bf, int bl, int plo, int phi, int cont, tail_val<TO>* tv) {
+            launch_timed(hook, 2, sweep_couple<TI, TO, F16, kWave>, dim3(bl - bf), dim3(kWave), 0, on, d, bf, bl, plo,
+                         phi, cont, in, static_cast<const TO*>(o), tv, st, 1, static_cast<const batch_state*>(nullptr));
+        };
         for (int k = 0; k < sc.nseg; ++k) {
-            if (sc.seg_tail[k]) {
-                const int p0 = sc.seg_p0[k], p1 = sc.seg_p1[k];
-                const int bf = sc.level_first[sc.seg_l0[k]], bl = sc.level_first[sc.seg_l1[k]];
-                const size_t lds = tail_lds_bytes(p1 - p0, bl - bf, sizeof(TO));
-                if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: tail run exceeds LDS");
-                static const bool attr = [] {
-                    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_tail<TO>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsBytes);
-                    return true;
-                }();
-                (void)attr;
-                const int base = sc.seg_base[k];
-                if (bl <= bf || base < 0 || base + (bl - bf) > sc.tail_batches ||
-                    static_cast<size_t>(sc.tail_batches) * kWave * sizeof(tail_val<TO>) > sc.tail_val_bytes)
-                    throw std::runtime_error("apxchol_amd: tail run outside the tail layout");
-                tail_val<TO>* tv = static_cast<tail_val<TO>*>(sc.tail_val);
-                tv += static_cast<size_t>(base) * kWave;
-                // One wave per workgroup (a batch each), as the lane levels.
-                launch_timed(hook, 2, sweep_couple<TI, TO, F16, kWave>, dim3(bl - bf), dim3(kWave), 0, s, d, bf, bl, p0,
-                             in, static_cast<const TO*>(o), tv, st, 1, static_cast<const batch_state*>(nullptr));
-                launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
-                             base, bl - bf, sc.seg_phases[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
-                             static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st, 1,
-                             static_cast<const batch_state*>(nullptr));
-            } else {
+            if (!sc.seg_tail[k]) {
                 for (int l = sc.seg_l0[k]; l < sc.seg_l1[k]; ++l) level_launch(l);
+                continue;
             }
+            const int p0 = sc.seg_p0[k], p1 = sc.seg_p1[k];
+            int bf, bl;
+            tail_val<TO>* tv;
+            run_of(k, bf, bl, tv);
+            const size_t lds = tail_lds_bytes(p1 - p0, bl - bf, sizeof(TO));
+            if (lds > static_cast<size_t>(kTailLdsBytes)) throw std::runtime_error("apxchol_amd: tail run exceeds LDS");
+            const int ps = split_at(k);
+            if (ps > 0) {
+                // The entries before ps were summed on the side stream while
+                // the run before this one walked; add the rest.
+                hip_check_launch(hipStreamWaitEvent(s, sc.ev_join, 0), "hipStreamWaitEvent");
+                couple(s, bf, bl, ps, p0, 1, tv);
+            } else {
+                couple(s, bf, bl, 0, p0, 0, tv);
+            }
+            const int pn = split_at(k + 1);
+            if (pn > 0) {
+                // Everything before this run is final here: fork the next
+                // run's coupling over the entries before this run.
+                if (pn != p0) throw std::runtime_error("apxchol_amd: coupling split not at the run before");
+                int nbf, nbl;
+                tail_val<TO>* ntv;
+                run_of(k + 1, nbf, nbl, ntv);
+                hip_check_launch(hipEventRecord(sc.ev_fork, s), "hipEventRecord");
+                hip_check_launch(hipStreamWaitEvent(sc.side, sc.ev_fork, 0), "hipStreamWaitEvent");
+                couple(sc.side, nbf, nbl, 0, pn, 0, ntv);
+                hip_check_launch(hipEventRecord(sc.ev_join, sc.side), "hipEventRecord");
+            }
+            launch_timed(hook, 1, sweep_tail<TO>, dim3(1), dim3(kTailBlock), static_cast<unsigned>(lds), s, sc.tail,
+                         sc.seg_base[k], bl - bf, sc.seg_phases[k], p1 - p0, d.backward ? d.m - 1 - p0 : p0,
+                         static_cast<int>(d.backward), o, static_cast<const tail_val<TO>*>(tv), st, 1,
+                         static_cast<const batch_state*>(nullptr));
         }
         check_launch("sweep_hybrid");
     }
@@ -398,7 +434,7 @@ void sweep_k_t(const sweep_desc& d, const block_sched& b, const void* rhs, void*
                 throw std::runtime_error("apxchol_amd: column tail run outside its tables");
             tail_val<TO>* tv = static_cast<tail_val<TO>*>(b.col_tail_val);
             const int g = (bl - bf + kWavesPerBlock - 1) / kWavesPerBlock;
-            sweep_couple<TI, TO, F16><<<dim3(g, b.kp), kBlock, 0, s>>>(d, bf, bl, p0, static_cast<const TI*>(rhs),
+            sweep_couple<TI, TO, F16><<<dim3(g, b.kp), kBlock, 0, s>>>(d, bf, bl, 0, p0, 0, static_cast<const TI*>(rhs),
                                                                      static_cast<const TO*>(out), tv, st, b.kp, bs);
             sweep_tail<TO><<<b.kp, kTailBlock, lds, s>>>(sc.tail, base, bl - bf, sc.seg_phases[k], p1 - p0,
                                                          d.backward ? d.m - 1 - p0 : p0, static_cast<int>(d.backward),

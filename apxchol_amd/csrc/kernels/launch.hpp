@@ -36,6 +36,12 @@ struct sweep_sched {
     const int* seg_p1 = nullptr;
     const int* seg_base = nullptr;         // tail segments: first tail batch
     const int* seg_phases = nullptr;       // tail segments: barrier phases of the walk
+    // Tail segments whose coupling is split (host; -1 or a position ps): the
+    // entries before ps are summed on `side` while the segment before runs,
+    // forked and joined through the two events; only the rest after it.
+    const int* seg_split = nullptr;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int nseg = 0;
     int grid = 0;                          // dataflow grid / cap of a level launch
     tail_desc tail;                        // static layout of the tail runs (device)

@@ -97,13 +97,21 @@ __device__ __forceinline__ void gather(const TO* out, const int (&jj)[kEpl], TO 
 // (the order of the fmas is the chunk-by-chunk one). value(j) reads column
 // j; an entry with use(j) false adds nothing, and neither does a slot past
 // the row's end (fma(0, 0, part) = part: a chain never holds -0).
+// long_row_chain_from continues a chain: from `part`, starting at entry c_begin
+// (a multiple of 64 kLongUnroll kEpl, so every entry keeps its lane and its
+// place in the lane's order), and stops at the first round whose first entry
+// satisfies past(j) (entries ascend in sweep position, so with past(j) = the
+// entry lies at or after some position, nothing later is used either).
+// Skipped entries are ones use(j) rejects, so the chain is the one a walk of
+// the whole row gives.
 inline constexpr int kLongUnroll = 4;
-template <bool F16, class TO, class V, class U>
-__device__ __forceinline__ TO long_row_chain(const sweep_desc& d, unsigned rb, unsigned len, V value, U use) {
+inline constexpr unsigned kLongRound = static_cast<unsigned>(kWave * kLongUnroll * kEpl);
+template <bool F16, class TO, class V, class U, class P>
+__device__ __forceinline__ TO long_row_chain_from(const sweep_desc& d, unsigned rb, unsigned len, V value, U use,
+                                                  TO part, unsigned c_begin, P past) {
     const int lane = threadIdx.x & (kWave - 1);
     constexpr int R = kLongUnroll * kEpl;
-    TO part = 0;
-    for (unsigned c0 = 0; c0 < len; c0 += static_cast<unsigned>(kWave * R)) {
+    for (unsigned c0 = c_begin; c0 < len; c0 += kLongRound) {
         int jj[R];
         float vv[R];
         TO xv[R];
@@ -112,6 +120,8 @@ __device__ __forceinline__ TO long_row_chain(const sweep_desc& d, unsigned rb, u
             const unsigned q = c0 + static_cast<unsigned>(lane + i * kWave);
             load_entry<F16>(d, q < len ? rb + q : rb, jj[i], vv[i]);
         }
+        // Lane 0's first slot holds the round's first entry (c0 < len).
+        if (past(__builtin_amdgcn_readfirstlane(jj[0]))) break;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             const bool ok = c0 + static_cast<unsigned>(lane + i * kWave) < len && use(jj[i]);
@@ -122,6 +132,38 @@ __device__ __forceinline__ TO long_row_chain(const sweep_desc& d, unsigned rb, u
         for (int i = 0; i < R; ++i) part = fma(static_cast<TO>(vv[i]), xv[i], part);
     }
     return part;
+}
+template <bool F16, class TO, class V, class U>
+__device__ __forceinline__ TO long_row_chain(const sweep_desc& d, unsigned rb, unsigned len, V value, U use) {
+    return long_row_chain_from<F16, TO>(d, rb, len, value, use, static_cast<TO>(0), 0u, [](int) { return false; });
+}
+
+// First entry of a row whose sweep position is at least plo (the row's
+// entries ascend in sweep position), by a 64-ary search over the wave: each
+// step probes 64 evenly spaced entries of the interval still open; len when
+// there is none. Every lane of the wave takes part and gets the answer.
+template <bool F16>
+__device__ __forceinline__ unsigned first_entry_at(const sweep_desc& d, unsigned rb, unsigned len, int plo) {
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    unsigned lo = 0, hi = len;   // entries before lo lie before plo; the answer is in [lo, hi]
+    while (lo < hi) {
+        const unsigned step = (hi - lo + kWave - 1) / kWave;
+        const unsigned q = lo + lane * step;
+        bool below = false;
+        if (q < hi) {
+            int j;
+            float v;
+            load_entry<F16>(d, rb + q, j, v);
+            below = (d.backward ? d.m - 1 - j : j) < plo;
+        }
+        const unsigned k = static_cast<unsigned>(__popcll(__ballot(below)));
+        if (k == 0) break;
+        const unsigned last_below = lo + (k - 1) * step;
+        const unsigned next = lo + k * step;
+        lo = last_below + 1;
+        hi = next < hi ? next : hi;
+    }
+    return lo;
 }
 
 // Publish a row: an agent-scope atomic store when other waves poll for it,
@@ -471,10 +513,16 @@ __device__ __forceinline__ lane_row map_lane(const sweep_desc& d, int b) {
 // bs, when given, stops the launch once no column is active). tv points at
 // the run's first tail batch; column c's values start nb * 64 further on per
 // column (nb = blast - bfirst).
+// The chain may be computed in two launches: the first (cont = 0) over the
+// entries before a position plo' < p0 with plo = 0, the second (cont = 1)
+// continuing each lane's stored value over the entries in [plo, p0) with
+// plo = plo'. A lane's entries before plo precede its entries in [plo, p0)
+// (ascending sweep position), so the two give the one-launch chain.
 template <class TI, class TO, bool F16, int BS = kBlock>
-__device__ __forceinline__ void sweep_couple_body(int vb, int c, const sweep_desc& d, int bfirst, int blast, int p0,
-                                                  const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
-                                                  const pcg_state* st, int kp, const batch_state* bs) {
+__device__ __forceinline__ void sweep_couple_body(int vb, int c, const sweep_desc& d, int bfirst, int blast, int plo,
+                                                  int p0, int cont, const TI* __restrict__ rhs, const TO* out,
+                                                  tail_val<TO>* tv, const pcg_state* st, int kp,
+                                                  const batch_state* bs) {
     if (bs ? bs->active == 0 : (st && st->done)) return;
     const double shift = (st && !d.backward) ? st[c].shift : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
@@ -482,11 +530,20 @@ __device__ __forceinline__ void sweep_couple_body(int vb, int c, const sweep_des
     if (b >= blast) return;
     const lane_row x = map_lane(d, b);
     const auto value = [&](int j) { return out[static_cast<size_t>(j) * kp + c]; };
-    TO part = 0;
-    const auto before = [&](int j) { return (d.backward ? d.m - 1 - j : j) < p0; };
-    if (p0 > 0 && x.len > static_cast<unsigned>(kWave * kEpl)) {
-        part = long_row_chain<F16, TO>(d, x.rb, x.len, value, before);
-    } else if (p0 > 0) {
+    const size_t at = (static_cast<size_t>(c) * (blast - bfirst) + (b - bfirst)) * kWave + lane;
+    TO part = cont ? tv[at].part0 : static_cast<TO>(0);
+    const auto pos_of = [&](int j) { return d.backward ? d.m - 1 - j : j; };
+    const auto before = [&](int j) {
+        const int q = pos_of(j);
+        return q >= plo && q < p0;
+    };
+    if (p0 > plo && x.len > static_cast<unsigned>(kWave * kEpl)) {
+        // Rounds wholly before plo are skipped, and the walk ends at the
+        // first round that starts at or past p0.
+        const unsigned e0 = plo > 0 ? first_entry_at<F16>(d, x.rb, x.len, plo) : 0u;
+        part = long_row_chain_from<F16, TO>(d, x.rb, x.len, value, before, part, e0 / kLongRound * kLongRound,
+                                            [&](int j) { return pos_of(j) >= p0; });
+    } else if (p0 > plo) {
         int jj[kEpl];
         float vv[kEpl];
         TO g[kEpl];
@@ -510,14 +567,14 @@ __device__ __forceinline__ void sweep_couple_body(int vb, int c, const sweep_des
     tail_val<TO> o;
     o.part0 = part;
     o.r = r;
-    tv[(static_cast<size_t>(c) * (blast - bfirst) + (b - bfirst)) * kWave + lane] = o;
+    tv[at] = o;
 }
 template <class TI, class TO, bool F16, int BS = kBlock>
-__global__ __launch_bounds__(BS) void sweep_couple(sweep_desc d, int bfirst, int blast, int p0,
+__global__ __launch_bounds__(BS) void sweep_couple(sweep_desc d, int bfirst, int blast, int plo, int p0, int cont,
                                                    const TI* __restrict__ rhs, const TO* out, tail_val<TO>* tv,
                                                    const pcg_state* st, int kp, const batch_state* bs) {
     sweep_couple_body<TI, TO, F16, BS>(static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), d, bfirst, blast,
-                                       p0, rhs, out, tv, st, kp, bs);
+                                       plo, p0, cont, rhs, out, tv, st, kp, bs);
 }
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
