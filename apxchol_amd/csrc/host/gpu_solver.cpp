@@ -1304,6 +1304,17 @@ void gpu_solver::single_begin(const double* d_b, const double* d_x0, double tol,
     d.single_max_iter = max_iter;
 }
 
+void gpu_solver::set_split_couplings(bool on) const {
+    auto& d = *dev_;
+    const int* f = on ? d.f_seg[7].data() : nullptr;
+    const int* b = on ? d.b_seg[7].data() : nullptr;
+    if (d.f_sched.seg_split == f && d.b_sched.seg_split == b) return;
+    d.f_sched.seg_split = f;
+    d.b_sched.seg_split = b;
+    if (d.iter_exec) { (void)hipGraphExecDestroy(d.iter_exec); d.iter_exec = nullptr; }
+    if (d.iter_graph) { (void)hipGraphDestroy(d.iter_graph); d.iter_graph = nullptr; }
+}
+
 void gpu_solver::single_prepare() const {
     auto& d = *dev_;
     if (!opts_.graph || d.iter_exec || n_ == 0) return;

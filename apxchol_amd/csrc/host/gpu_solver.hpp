@@ -168,6 +168,14 @@ public:
     // single_prepare captures the iteration graph beforehand (single_begin
     // does it on first use otherwise).
     void single_prepare() const;
+    // Split couplings (on by default): a tail run's coupling over the entries
+    // before the previous run runs on a side stream beside that run (a
+    // parallel branch of the captured iteration). A serving loop turns them
+    // off for the solvers it owns -- their extra branches contend for the
+    // process's few hardware queues with the other factors' streams -- and
+    // back on when it lets them go. Same bits either way; a change drops the
+    // captured iteration graph.
+    void set_split_couplings(bool on) const;
     void single_begin(const double* d_b, const double* d_x0, double tol, int max_iter, hipStream_t s) const;
     void single_step(int iterations, hipStream_t s) const;
     bool single_poll(session_slot& out, hipStream_t s) const;

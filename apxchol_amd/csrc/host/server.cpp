@@ -62,7 +62,10 @@ solve_server::~solve_server() {
         if (l->worker.joinable()) l->worker.join();
     for (auto& l : lanes_) {
         free_lane(*l);
-        if (l->solver) l->solver->release_serving();
+        if (l->solver) {
+            l->solver->set_split_couplings(true);
+            l->solver->release_serving();
+        }
     }
 }
 
@@ -74,8 +77,14 @@ int solve_server::add(std::shared_ptr<gpu_solver> solver) {
     solver->claim_serving();
     struct release_on_throw {
         const gpu_solver* s;
-        ~release_on_throw() { if (s) s->release_serving(); }
+        ~release_on_throw() {
+            if (!s) return;
+            s->set_split_couplings(true);
+            s->release_serving();
+        }
     } guard{solver.get()};
+    // No side-stream branches in the lanes' graphs (see set_split_couplings).
+    solver->set_split_couplings(false);
     auto pow2 = [](int want) {
         int kp = 1;
         while (kp < want) kp <<= 1;
