@@ -60,7 +60,8 @@ class SolverInfo(ctypes.Structure):
                 ("op_fp32", I32), ("spmv_lanes", I32), ("dataflow_grid", I32), ("fp16_storage", I32),
                 ("device_bytes", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64 * 2),
                 ("tail_bytes", ctypes.c_uint64 * 2), ("level_launches", I64 * 2), ("tail_runs", I64 * 2),
-                ("lane_levels", I64 * 2), ("spmv_format", I32), ("spmv_sell_slots", ctypes.c_uint64)]
+                ("lane_levels", I64 * 2), ("spmv_format", I32), ("spmv_sell_slots", ctypes.c_uint64),
+                ("split_couplings", I64 * 2)]
 
 
 _SIGS = {

@@ -269,6 +269,8 @@ apx_status apx_solver_info_get(apx_solver_t s, apx_solver_info* info) {
         info->device_bytes = st.device_bytes;
         info->spmv_format = st.spmv_format;
         info->spmv_sell_slots = st.spmv_sell_slots;
+        info->split_couplings[0] = st.split_couplings[0];
+        info->split_couplings[1] = st.split_couplings[1];
         for (int k = 0; k < 2; ++k) {
             info->level_bytes[k] = st.level_bytes[k];
             info->tail_bytes[k] = st.tail_bytes[k];

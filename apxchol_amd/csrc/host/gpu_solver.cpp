@@ -1029,7 +1029,10 @@ void gpu_solver::setup(const csc_view& L) {
                     before += static_cast<std::uint64_t>(
                         std::partition_point(b0, b1, [&](std::int32_t j) { return (backward ? m - 1 - j : j) < ps; }) - b0);
                 }
-                if (before >= split_min) seg[7][k] = ps;
+                if (before >= split_min) {
+                    seg[7][k] = ps;
+                    ++stats_.split_couplings[backward ? 1 : 0];
+                }
             }
         }
         sc.seg_split = seg[7].data();

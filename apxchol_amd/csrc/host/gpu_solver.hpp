@@ -87,6 +87,7 @@ struct solver_stats {
     std::int64_t lane_levels[2] = {0, 0};   // launched levels on the static lane layout
     int spmv_format = 0;                    // single SpMV: 0 CSR lanes, 1 sliced ELL
     std::uint64_t spmv_sell_slots = 0;      // padded entries of the sliced-ELL form (0: none)
+    std::int64_t split_couplings[2] = {0, 0};   // tail runs whose coupling is split, per direction
 };
 
 struct block_sched;

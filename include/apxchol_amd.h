@@ -99,6 +99,7 @@ typedef struct {
     int64_t lane_levels[2];  /* launched levels that read the static lane layout */
     int32_t spmv_format;     /* single SpMV: 0 CSR lanes, 1 sliced ELL (APXCHOL_AMD_SPMV=csr|sell forces) */
     uint64_t spmv_sell_slots; /* padded entries of the sliced-ELL form (0: none built) */
+    int64_t split_couplings[2]; /* per direction: tail runs whose coupling is split in two launches */
 } apx_solver_info;
 
 const char* apx_last_error(void);

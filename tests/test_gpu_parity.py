@@ -421,3 +421,43 @@ def test_lane_levels_several_batches_per_wave_same_bits(gpu, name, precision, mo
         s.close()
     for k in range(1, len(outs)):
         assert np.array_equal(outs[0][0], outs[k][0]) and np.array_equal(outs[0][1], outs[k][1]), k
+
+
+@pytest.mark.parametrize("name", ["rmat_16"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_split_couplings_give_the_same_bits(gpu, name, precision, monkeypatch):
+    """A tail run's coupling split in two launches (the entries before the
+    previous run on the side stream, beside that run's tail; the rest after
+    it) gives the one-launch chains: forced on every run that follows a run
+    (APXCHOL_AMD_COUPLE_SPLIT_MIN=1), the sweeps, an application and a graph-
+    replayed solve are the bits of the unsplit schedule."""
+    import torch
+    A, F, _ = case(name)
+    print(f"[case] split couplings {name} {precision}", flush=True)
+    out = {}
+    for mode in ("off", "forced"):
+        if mode == "off":
+            monkeypatch.setenv("APXCHOL_AMD_COUPLE_SPLIT", "0")
+        else:
+            monkeypatch.delenv("APXCHOL_AMD_COUPLE_SPLIT", raising=False)
+            monkeypatch.setenv("APXCHOL_AMD_COUPLE_SPLIT_MIN", "1")
+        s = ax.Solver(A, factor=F, precision=precision)
+        splits = sum(s.info["split_couplings"])
+        assert splits == 0 if mode == "off" else splits > 0, (mode, s.info["tail_runs"])
+        m = s.info["m"]
+        x = torch.tensor(np.random.default_rng(3).standard_normal(m), device=gpu)
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        s.sweep(x, y, backward=False)
+        s.sweep(y, z, backward=True)
+        b = torch.tensor(mg.request_rhs(A, F.sddm, 11), device=gpu)
+        s.reset_apply_count()
+        a = s.apply(b)
+        r = s.solve(b)
+        torch.cuda.synchronize()
+        out[mode] = (z.cpu().numpy().view(np.int64).copy(), a.cpu().numpy().view(np.int64).copy(),
+                     r.x.cpu().numpy().view(np.int64).copy(), r.iterations)
+        s.close()
+    for k in range(3):
+        assert np.array_equal(out["off"][k], out["forced"][k]), k
+    assert out["off"][3] == out["forced"][3]
